@@ -1,0 +1,165 @@
+/* librbc_hip -- C ABI of the MI355X (gfx950) Random Ball Cover k-NN engine.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status (RBC_OK = 0);
+ * on failure rbc_last_error() gives a message for the calling thread.  Matrices are
+ * row-major float32 with a leading dimension (elements per row) >= d.  `mem` tells
+ * whether the data pointers are host (RBC_MEM_HOST) or device (RBC_MEM_DEVICE)
+ * memory; host buffers are copied in and out by the call, device buffers are used in
+ * place.  Calls on one handle are not thread-safe; all work runs on the stream
+ * given (NULL = the default stream) and every call returns after its work is done.
+ *
+ * Each entry point names the reference interface it replaces
+ * (lcayton/RandomBallCover-GPU v0.2.6, file:line).
+ */
+#ifndef RBC_HIP_H
+#define RBC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rbc_defs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBC_OK 0
+#define RBC_ERR_ARG 1      /* invalid argument (replaces the reference's exit(1) paths) */
+#define RBC_ERR_HIP 2      /* HIP runtime error */
+#define RBC_ERR_NOMEM 3    /* device allocation failed (rbc.cu:151-154 exits instead) */
+#define RBC_ERR_NOCONV 4   /* range bisection hit its iteration cap (kernels.cu:368-409 has none) */
+#define RBC_ERR_LIMIT 5    /* a documented size limit was exceeded */
+
+#define RBC_L2 0 /* squared Euclidean, defs.h:21 */
+#define RBC_L1 1 /* sum of absolute differences, defs.h:20 */
+
+#define RBC_MEM_HOST 0
+#define RBC_MEM_DEVICE 1
+
+/* Largest dimension the kernels accept. */
+#define RBC_MAX_DIM 256
+/* Largest k of the exact k-query. */
+#define RBC_MAX_K 64
+
+const char *rbc_last_error(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int rbc_version(void);
+/* Select the HIP device used by subsequent calls of this thread (driver.cu:37-41). */
+int rbc_set_device(int device);
+/* The distance used by the reference-compatible entry points (the reference fixes it
+ * at compile time in defs.h:18-21; here it is a process-wide setting, default RBC_L2). */
+int rbc_set_metric(int metric);
+int rbc_get_metric(void);
+
+/* ---------------------------------------------------------------- brute force -- */
+
+/* bruteK (brute.cu:65-88 -> knnWrap kernelWrap.cu:95-111 -> knnKernel kernels.cu:229-282):
+ * for each of the m queries, the k nearest of the n database points, ascending by
+ * (distance, index).  Distances are FP32 with one fused multiply-add per coordinate in
+ * coordinate order, as the reference's kernel computes them.  k <= KMAX; rows of the
+ * outputs have leading dimension ldo >= k; slots past n hold (MAX_REAL, DUMMY_IDX). */
+int rbc_brute_k(const float *x, uint32_t n, uint32_t ldx, const float *q, uint32_t m,
+                uint32_t ldq, uint32_t d, int metric, uint32_t k, uint32_t *nn_ids,
+                float *nn_dists, uint32_t ldo, int mem, void *stream);
+
+/* bruteSearch (brute.cu:43-62 -> nnWrap -> nnKernel kernels.cu:165-222): 1-NN ids and
+ * distances; ties go to the lowest index.  nn_dists may be NULL. */
+int rbc_brute_search(const float *x, uint32_t n, uint32_t ldx, const float *q, uint32_t m,
+                     uint32_t ldq, uint32_t d, int metric, uint32_t *nn_ids, float *nn_dists,
+                     int mem, void *stream);
+
+/* bruteRangeCount (brute.cu:19-40 -> rangeCountKernel kernels.cu:426-471): for each
+ * query, the number of points with distance < ranges[i]. */
+int rbc_brute_range_count(const float *x, uint32_t n, uint32_t ldx, const float *q,
+                          uint32_t m, uint32_t ldq, uint32_t d, int metric,
+                          const float *ranges, uint32_t *counts, int mem, void *stream);
+
+/* bruteCPU (brute.cu:91-112): the reference's host 1-NN, kept as the host routine it
+ * is (single thread, product rounded before the add, strict '<', host pointers). */
+int rbc_brute_cpu(const float *x, uint32_t n, uint32_t ldx, const float *q, uint32_t m,
+                  uint32_t ldq, uint32_t d, int metric, uint32_t *nn_ids);
+
+/* ------------------------------------------------------- one-shot RBC (reference) -- */
+
+/* buildRBC (rbc.cu:126-208): picks num_reps representatives (rep_ids, or a seeded
+ * sample of distinct rows when rep_ids is NULL -- the reference seeds from the clock,
+ * utils.cu:17-37), computes each rep's bisection range so that between 0.9 s and s
+ * points fall inside (kernels.cu:322-414), and stores the ascending lists of points
+ * with distance < range.  Fills *out exactly like the reference: dx, dxMap (num_reps x
+ * PAD(s)) and dr on the device, groupCount[PAD(num_reps)] on the host.  x is host
+ * memory when mem = RBC_MEM_HOST.  Returns RBC_ERR_NOCONV if a bisection does not
+ * settle within 1000 steps (the reference loops forever, todo.txt:3-5). */
+int rbc_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32_t num_reps,
+              uint32_t s, const uint32_t *rep_ids, uint64_t seed, int mem, rbcStruct *out);
+
+/* queryRBC (rbc.cu:19-68): one-shot 1-NN (nearest rep, then that rep's list). */
+int rbc_query(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs,
+              uint32_t *nn_ids, float *nn_dists, int mem, void *stream);
+
+/* kqueryRBC (rbc.cu:72-123): one-shot k-NN, KMAX columns (ld ldo >= KMAX), ascending by
+ * (distance, index); unfilled slots hold (MAX_REAL, DUMMY_IDX). */
+int rbc_kquery(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs,
+               uint32_t *nn_ids, float *nn_dists, uint32_t ldo, int mem, void *stream);
+
+/* destroyRBC (rbc.cu:363-368). */
+int rbc_destroy(rbcStruct *rbcs);
+
+/* ---------------------------------------------------------------- exact RBC ----- */
+
+/* Opaque handle of an exact-search index: representatives, the database permuted
+ * into contiguous ownership lists (FP32 and BF16 copies), per-list radii and FP64
+ * norms.  No reference counterpart: the reference's exact-search pieces are unused
+ * or uncompiled (rbc.cu:250-256 computeRadii, deprecated.cu:354-464 pruneKernel,
+ * deprecated.cu:395-428 groupPoints). */
+typedef struct rbc_exact rbc_exact;
+
+#define RBC_PREC_FP32 0 /* difference-form FP32 distances on the vector ALUs */
+#define RBC_PREC_BF16 1 /* BF16 MFMA distance filter + exact FP32 re-rank */
+
+/* Build: rep_ids (num_reps distinct rows) or a seeded sample; every point is owned by
+ * its nearest rep (lowest rep index on ties); lists are stored contiguously in
+ * ascending original-index order. */
+int rbc_exact_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32_t num_reps,
+                    int metric, const uint32_t *rep_ids, uint64_t seed, int mem,
+                    rbc_exact **out);
+
+typedef struct {
+  double pairs_probe;     /* (query, point) distances evaluated by the first-probe pass */
+  double pairs_scan;      /* ... by the pruned list scan */
+  double pairs_rerank;    /* candidates re-ranked exactly */
+  double items_scan;      /* (list, query tile) work items of the scan */
+  double lists_scanned;   /* (query, list) pairs that survived pruning */
+  double bytes_scan;      /* algorithmic bytes of the scan kernel (list tiles + queries) */
+  double guard_fp64;      /* candidates whose order was settled in FP64 */
+  double fallbacks;       /* queries re-run because a candidate buffer overflowed */
+  double ms_qreps, ms_probe, ms_plan, ms_scan, ms_select, ms_total; /* hipEvent times */
+} rbc_query_stats;
+
+/* Exact k-NN: ids (original row numbers) and distances of the k nearest points, ascending
+ * by (distance, index), ties to the lowest index.  Distances are FP32 re-ranked values
+ * within 1e-6 relative of FP64.  precision selects the distance filter.  stats may be
+ * NULL. */
+int rbc_exact_kquery(rbc_exact *h, const float *q, uint32_t m, uint32_t ldq, uint32_t k,
+                     int precision, uint32_t *nn_ids, float *nn_dists, uint32_t ldo, int mem,
+                     void *stream, rbc_query_stats *stats);
+
+typedef struct {
+  uint32_t n, d, ld, num_reps, metric;
+  uint32_t max_list, min_list;
+  size_t device_bytes;
+} rbc_exact_info;
+
+int rbc_exact_get_info(const rbc_exact *h, rbc_exact_info *info);
+
+/* Copies the build's structure to host arrays (any may be NULL): rep_ids[num_reps],
+ * list_offsets[num_reps+1], perm[n] (original id of permuted row), owner[n] (by original
+ * id), radii[num_reps] (max FP64 distance of a list's points to its rep, rounded up). */
+int rbc_exact_get_lists(const rbc_exact *h, uint32_t *rep_ids, uint32_t *list_offsets,
+                        uint32_t *perm, uint32_t *owner, float *radii);
+
+int rbc_exact_destroy(rbc_exact *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RBC_HIP_H */

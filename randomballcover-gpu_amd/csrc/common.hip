@@ -1,0 +1,138 @@
+// Library-wide state and helpers: per-thread error text, device selection, metric
+// setting, input staging and small fill kernels.
+#include <cstring>
+#include <mutex>
+
+#include "launch.hpp"
+
+namespace rbc {
+
+namespace {
+thread_local char g_err[1024] = "";
+int g_metric = RBC_L2;
+
+__global__ void fill_u32_k(uint32_t *p, uint32_t v, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+__global__ void fill_f32_k(float *p, float v, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+}  // namespace
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int metric_setting() { return g_metric; }
+
+int max_lds_bytes() {
+  static int v = 0;
+  if (!v) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 65536;
+    v = (int)prop.sharedMemPerBlock;
+  }
+  return v;
+}
+
+int check_device_ptr(const void *p, const char *what) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice) {
+    (void)hipGetLastError();
+    set_error("%s is not a device pointer", what);
+    return RBC_ERR_ARG;
+  }
+  return RBC_OK;
+}
+
+int fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t st) {
+  if (!n) return RBC_OK;
+  uint32_t g = (uint32_t)std::min<size_t>(cdivz(n, 256), 4096);
+  hipLaunchKernelGGL(fill_u32_k, dim3(g), dim3(256), 0, st, p, v, n);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+int fill_f32(float *p, float v, size_t n, hipStream_t st) {
+  if (!n) return RBC_OK;
+  uint32_t g = (uint32_t)std::min<size_t>(cdivz(n, 256), 4096);
+  hipLaunchKernelGGL(fill_f32_k, dim3(g), dim3(256), 0, st, p, v, n);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+int DevMat::stage(const float *x, uint32_t rows, uint32_t d, uint32_t ldx, int mem, hipStream_t st) {
+  RBC_ARG_CHECK(x != nullptr || rows == 0, "matrix pointer is NULL");
+  RBC_ARG_CHECK(ldx >= d, "leading dimension %u < dimension %u", ldx, d);
+  if (mem == RBC_MEM_DEVICE && ldx % 4 == 0) {
+    p = x;
+    ld = ldx;
+    return RBC_OK;
+  }
+  ld = pad16(d);
+  RBC_TRY(own.alloc(size_t(rows) * ld));
+  RBC_TRY(fill_f32(own.get(), 0.f, size_t(rows) * ld, st));
+  if (rows)
+    RBC_HIP_TRY(hipMemcpy2DAsync(own.get(), ld * sizeof(float), x, size_t(ldx) * sizeof(float),
+                                 size_t(d) * sizeof(float), rows,
+                                 mem == RBC_MEM_DEVICE ? hipMemcpyDeviceToDevice
+                                                       : hipMemcpyHostToDevice,
+                                 st));
+  p = own.get();
+  return RBC_OK;
+}
+
+int copy_out(void *dst, const void *src, size_t bytes, int mem, hipStream_t st) {
+  if (!bytes) return RBC_OK;
+  RBC_HIP_TRY(hipMemcpyAsync(dst, src, bytes,
+                             mem == RBC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                             st));
+  return RBC_OK;
+}
+
+int copy_in(void *dst, const void *src, size_t bytes, int mem, hipStream_t st) {
+  if (!bytes) return RBC_OK;
+  RBC_HIP_TRY(hipMemcpyAsync(dst, src, bytes,
+                             mem == RBC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             st));
+  return RBC_OK;
+}
+
+int upload_items(const std::vector<ScanItem> &v, DevBuf<ScanItem> &buf, hipStream_t st) {
+  RBC_TRY(buf.ensure(v.size()));
+  if (!v.empty()) {
+    RBC_HIP_TRY(hipMemcpyAsync(buf.get(), v.data(), v.size() * sizeof(ScanItem),
+                               hipMemcpyHostToDevice, st));
+  }
+  return RBC_OK;
+}
+
+}  // namespace rbc
+
+extern "C" {
+
+const char *rbc_last_error(void) { return rbc::g_err; }
+
+int rbc_version(void) { return 100; }
+
+int rbc_set_device(int device) {
+  RBC_HIP_TRY(hipSetDevice(device));
+  return RBC_OK;
+}
+
+int rbc_set_metric(int metric) {
+  RBC_ARG_CHECK(metric == RBC_L2 || metric == RBC_L1, "unknown metric %d", metric);
+  rbc::g_metric = metric;
+  return RBC_OK;
+}
+
+int rbc_get_metric(void) { return rbc::g_metric; }
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// The exact-search index (struct rbc_exact) and the pieces shared by its build and
+// query code.
+#pragma once
+
+#include <vector>
+
+#include "launch.hpp"
+
+struct rbc_exact {
+  uint32_t n = 0, d = 0, dpad = 0, ld = 0, R = 0;
+  int metric = RBC_L2;
+  // Database permuted into contiguous ownership lists (row-major, ld floats per row).
+  rbc::DevBuf<float> xp;
+  // BF16 copy of xp (ldb = ld bf16 per row) and per-row FP64 squared norms of it.
+  rbc::DevBuf<uint16_t> xb;
+  rbc::DevBuf<float> xb_norm;   // ||bf16(x)||^2, rounded to nearest
+  rbc::DevBuf<float> list_qerr; // per list: max ||x - bf16(x)|| of its points (rounded up)
+  rbc::DevBuf<float> list_nmax; // per list: max ||bf16(x)||^2 (rounded up)
+  rbc::DevBuf<uint32_t> perm;     // permuted row -> original id
+  rbc::DevBuf<uint32_t> inv;      // original id -> permuted row
+  rbc::DevBuf<uint32_t> owner;    // original id -> owning rep
+  rbc::DevBuf<uint32_t> list_off; // R+1
+  rbc::DevBuf<uint32_t> list_len; // R
+  rbc::DevBuf<float> reps;        // R x ld
+  rbc::DevBuf<uint16_t> reps_b;   // R x ld BF16 copy
+  rbc::DevBuf<float> reps_bnorm;  // ||bf16(r)||^2
+  rbc::DevBuf<float> reps_qerr;   // ||r - bf16(r)||, rounded up
+  rbc::DevBuf<uint32_t> rep_ids;  // R
+  rbc::DevBuf<float> radius;      // R, metric distance, rounded up
+  rbc::DevBuf<float> rr;          // R x R FP32 rep-rep distances (metric value)
+  std::vector<uint32_t> h_list_off;
+  rbc::Workspace ws;
+};
+
+namespace rbc {
+
+// Relative error bound of an FP32 distance summed with one FMA (or add) per coordinate
+// over dpad coordinates: (dpad + 4) u with u = 2^-24, plus 1% headroom.
+inline double fma_rel_err(uint32_t dpad) { return (double(dpad) + 4.0) * 0x1p-24 * 1.01; }
+// Relative error bound of the tree-ordered re-rank over P (power of two) coordinates.
+inline double tree_rel_err(uint32_t P) {
+  int lg = 0;
+  while ((1u << lg) < P) ++lg;
+  return (double(lg) + 4.0) * 0x1p-24 * 1.01;
+}
+inline uint32_t tree_width(uint32_t d) {
+  uint32_t P = 64;
+  while (P < d) P <<= 1;
+  return P;
+}
+
+int nn_device(const float *x, size_t ldx, uint32_t n, const uint32_t *xid, const float *q,
+              size_t ldq, uint32_t m, uint32_t d, int metric, float *od, uint32_t *oi,
+              hipStream_t st);
+
+}  // namespace rbc

@@ -1,0 +1,566 @@
+// Exact k-NN query over an rbc_exact index.
+//
+// Pipeline (one stream):
+//  1. q->reps   : FP32 distances of every query to every representative.
+//  2. row stats : nearest rep of each query and the k-th smallest rep distance (reps are
+//                 database points, so that is an upper bound on the k-th NN distance).
+//  3. probe     : queries grouped by nearest rep scan that rep's list, keeping the best
+//                 K' = k+8 (distance, id) pairs -> an upper bound gamma on the k-th NN
+//                 distance and a cut-off tau for candidates.
+//  4. plan      : per (query, list) lower bounds on the distance to any point of the
+//                 list -- the triangle inequality with the list radius
+//                 (d(q,r_j) - psi_j) and, for L2, the Voronoi plane between the list's
+//                 rep and the query's nearest rep -- computed in FP64 with explicit FP32
+//                 error bounds; lists whose bound exceeds gamma are pruned.  The
+//                 surviving (list, query) pairs are compacted list-major.
+//  5. scan      : every (list, <=64 queries) tile computes all distances and appends the
+//                 ones <= tau to per-query candidate buffers.
+//  6. select    : one wave per query re-ranks the candidates with tree-ordered FP32 sums,
+//                 sorts by (distance, id) with wave-level bitonic merges, settles
+//                 near-ties in FP64 and writes the k results.
+// The BF16 variant (precision RBC_PREC_BF16) replaces steps 1, 3 and 5 with BF16 MFMA
+// tiles whose results carry explicit error bounds (exact_bf16.hip).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "exact.hpp"
+#include "exact_query.hpp"
+
+namespace rbc {
+
+namespace {
+
+// ---- 2. row stats ----------------------------------------------------------------
+// One wave per query over its row of R distances: nearest rep (lowest index on ties)
+// and the k-th smallest (distance, rep) pair.
+__global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uint32_t R, uint32_t k,
+                                                 uint32_t *near, float *kth) {
+  const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (q >= m) return;
+  const int l = lane_id();
+  float tv = FLT_MAX;
+  uint32_t ti = kDummy;
+  float thr = FLT_MAX;
+  uint32_t thri = kDummy;
+  const float *row = D + size_t(q) * R;
+  for (uint32_t c0 = 0; c0 < R; c0 += kWave) {
+    float v = c0 + l < R ? row[c0 + l] : FLT_MAX;
+    uint32_t id = c0 + l < R ? c0 + l : kDummy;
+    // Skip the chunk when nothing in it beats the current k-th.
+    bool any = pair_less(v, id, thr, thri);
+    if (!__any(any)) continue;
+    wave_bitonic_sort(v, id);
+    wave_merge_lower(tv, ti, v, id);
+    thr = __shfl(tv, (int)k - 1, kWave);
+    thri = __shfl(ti, (int)k - 1, kWave);
+  }
+  if (l == 0) near[q] = ti;
+  if (l == (int)k - 1) kth[q] = tv;
+}
+
+// ---- 3b. bounds from the probe ----------------------------------------------------
+// gamma_hi: upper bound on the true k-th NN distance value (squared for L2);
+// tau: candidate cut-off on the computed FP32 values; Gam: gamma_hi as a metric
+// distance (sqrt for L2) for the pruning tests; cert: whether the probe's K' kept every
+// point of the probed list that could pass tau.
+__global__ void probe_bounds(const float *probe_d, uint32_t Kp, uint32_t k, const float *rep_kth,
+                             const uint32_t *near, const uint32_t *list_len, uint32_t m, double e_rel,
+                             int metric, float *tau, double *gam, uint32_t *cert) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  double g = (double)rep_kth[q];
+  double gp = (double)probe_d[size_t(q) * Kp + k - 1];
+  if (gp < g) g = gp;
+  double ghi = g >= (double)FLT_MAX ? INFINITY : g / (1.0 - e_rel) * (1.0 + 1e-12);
+  double t = ghi * (1.0 + e_rel) * (1.0 + 1e-12);
+  float tf = t >= (double)FLT_MAX ? INFINITY : (float)t;
+  if ((double)tf < t) tf = nextafterf(tf, INFINITY);
+  tau[q] = tf;
+  gam[q] = metric == 0 ? sqrt(ghi) * (1.0 + 1e-12) : ghi;
+  uint32_t len = list_len[near[q]];
+  cert[q] = (len <= Kp || probe_d[size_t(q) * Kp + Kp - 1] > tf) ? 1u : 0u;
+}
+
+// ---- 4. plan ------------------------------------------------------------------------
+// Lower bound (metric distance) on d(q, x) for every x in list j.
+__device__ __forceinline__ double list_lower_bound(double Dqj, double Dq0, double rr_j0, double psi,
+                                                   double e, bool same, int metric) {
+  double lb;
+  if (metric == 0) {
+    double dlo = sqrt(fmax(Dqj * (1.0 - e), 0.0));
+    lb = dlo - psi;
+    if (!same && rr_j0 > 0.0) {
+      // Voronoi plane between r_j and the query's nearest rep r_0: every x owned by r_j
+      // satisfies |x-r_j|^2 <= |x-r_0|^2 + 2e psi^2 (the owner test ran in FP32), so
+      // d(q,x) >= (|q-r_j|^2 - |q-r_0|^2 - 2e psi^2) / (2 |r_j - r_0|).
+      double num = Dqj * (1.0 - e) - Dq0 * (1.0 + e) - 2.0 * e * psi * psi;
+      double den = 2.0 * sqrt(rr_j0 * (1.0 + e));
+      if (num > 0.0) lb = fmax(lb, num / den);
+    }
+  } else {
+    lb = Dqj * (1.0 - e) - psi;
+  }
+  return lb * (1.0 - 1e-12);
+}
+
+// Pass A: keep bits (m x W words) and per-list query counts.  Block = 256 threads over
+// kPlanQ queries; thread t covers lists t, t+256, ...
+constexpr uint32_t kPlanQ = 32;
+
+__global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, uint32_t R,
+                                                  const uint32_t *near, const double *gam,
+                                                  const uint32_t *cert, const float *radius,
+                                                  const float *rr, double e, int metric,
+                                                  uint32_t W, uint32_t *bits, uint32_t *list_cnt) {
+  const uint32_t q0 = blockIdx.x * kPlanQ;
+  const int l = lane_id();
+  for (uint32_t jb = (threadIdx.x / kWave) * kWave; jb < R; jb += 256) {
+    const uint32_t j = jb + l;
+    uint32_t cnt = 0;
+    const double psi = j < R ? (double)radius[j] : 0.0;
+    for (uint32_t qi = 0; qi < kPlanQ && q0 + qi < m; ++qi) {
+      const uint32_t q = q0 + qi;
+      const uint32_t j0 = near[q];
+      bool keep = false;
+      if (j < R && (j != j0 || cert[q] == 0)) {
+        if (j == j0) {
+          keep = true;
+        } else {
+          double Dqj = (double)Dqr[size_t(q) * R + j];
+          double Dq0 = (double)Dqr[size_t(q) * R + j0];
+          double lb = list_lower_bound(Dqj, Dq0, (double)rr[size_t(j0) * R + j], psi, e, false,
+                                       metric);
+          keep = lb <= gam[q];
+        }
+      }
+      cnt += keep;
+      uint64_t b = __ballot(keep);
+      if (l == 0) {
+        uint32_t w = jb / 32;
+        if (w < W) bits[size_t(q) * W + w] = (uint32_t)b;
+        if (w + 1 < W) bits[size_t(q) * W + w + 1] = (uint32_t)(b >> 32);
+      }
+    }
+    if (j < R && cnt) atomicAdd(&list_cnt[j], cnt);
+  }
+}
+
+// Pass C: scatter query ids list-major.  Same block shape as plan_count.
+__global__ __launch_bounds__(256) void plan_scatter(uint32_t m, uint32_t R, uint32_t W,
+                                                    const uint32_t *bits, const uint32_t *list_qoff,
+                                                    uint32_t *cursor, uint32_t *qlist) {
+  const uint32_t q0 = blockIdx.x * kPlanQ;
+  for (uint32_t j = threadIdx.x; j < R; j += 256) {
+    uint32_t cnt = 0;
+    for (uint32_t qi = 0; qi < kPlanQ && q0 + qi < m; ++qi)
+      cnt += (bits[size_t(q0 + qi) * W + j / 32] >> (j % 32)) & 1u;
+    if (!cnt) continue;
+    uint32_t base = list_qoff[j] + atomicAdd(&cursor[j], cnt);
+    for (uint32_t qi = 0; qi < kPlanQ && q0 + qi < m; ++qi)
+      if ((bits[size_t(q0 + qi) * W + j / 32] >> (j % 32)) & 1u) qlist[base++] = q0 + qi;
+  }
+}
+
+}  // namespace
+
+// ---- 6. select ----------------------------------------------------------------------
+
+// Sorts (run, key64, id) triples held one per lane ascending (run, key, id), carrying
+// a float payload.
+__device__ void wave_sort_runs(uint32_t &run, double &key, uint32_t &id, float &pay) {
+  const int l = lane_id();
+  for (int k = 2; k <= kWave; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      uint32_t pr = __shfl_xor(run, j, kWave);
+      double pk = __shfl_xor(key, j, kWave);
+      uint32_t pi = __shfl_xor(id, j, kWave);
+      float pp = __shfl_xor(pay, j, kWave);
+      bool p_less = pr < run || (pr == run && (pk < key || (pk == key && pi < id)));
+      bool m_less = run < pr || (run == pr && (key < pk || (key == pk && id < pi)));
+      bool keep_min = ((l & j) == 0) == ((l & k) == 0);
+      if (keep_min ? p_less : m_less) {
+        run = pr;
+        key = pk;
+        id = pi;
+        pay = pp;
+      }
+    }
+}
+
+template <int METRIC, int NPL>
+__device__ float tree_dist_row(const float *xrow, const float (&qv)[NPL], uint32_t d) {
+  const int l = lane_id();
+  float t[NPL];
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    uint32_t c = l + kWave * j;
+    t[j] = c < d ? tree_term<METRIC>(xrow[c], qv[j]) : 0.f;
+  }
+  // Strides >= 64 inside the lane: t[j] covers coordinate l + 64 j.
+#pragma unroll
+  for (int s = NPL / 2; s >= 1; s >>= 1)
+#pragma unroll
+    for (int j = 0; j < s; ++j) t[j] = __fadd_rn(t[j], t[j + s]);
+  return wave_tree_sum(t[0]);
+}
+
+template <int METRIC>
+__device__ double fp64_dist(const float *xrow, const float *qrow, uint32_t d) {
+  double acc = 0.0;
+  for (uint32_t c = 0; c < d; ++c) {
+    double t = __dsub_rn((double)xrow[c], (double)qrow[c]);
+    acc = __dadd_rn(acc, METRIC == 0 ? __dmul_rn(t, t) : fabs(t));
+  }
+  return acc;
+}
+
+template <int METRIC, int NPL>
+__global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
+  const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (q >= a.m) return;
+  const int l = lane_id();
+  const float *qrow = a.q + size_t(q) * a.ldq;
+  float qv[NPL];
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    uint32_t c = l + kWave * j;
+    qv[j] = c < a.d ? qrow[c] : 0.f;
+  }
+  float tv = FLT_MAX;
+  uint32_t ti = kDummy;
+  const bool use_probe = a.cert == nullptr || a.cert[q] != 0;
+  const uint32_t np = use_probe ? a.Kp : 0;
+  const uint32_t nc = a.cand_n ? min(a.cand_n[q], a.cap) : 0;
+  const uint32_t total = np + nc;
+  uint32_t reranked = 0;
+  for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
+    uint32_t e = b0 + l;
+    uint32_t myid = kDummy;
+    if (e < np)
+      myid = a.probe_i[size_t(q) * a.Kp + e];
+    else if (e < total)
+      myid = a.cand_i[size_t(q) * a.cap + (e - np)];
+    float myv = FLT_MAX;
+    for (int cc = 0; cc < kWave && b0 + cc < total; ++cc) {
+      uint32_t id = __shfl(myid, cc, kWave);
+      if (id == kDummy) continue;
+      const float *xrow = a.xp + size_t(a.inv[id]) * a.ldx;
+      float v = tree_dist_row<METRIC, NPL>(xrow, qv, a.d);
+      if (l == cc) myv = v;
+      ++reranked;
+    }
+    wave_bitonic_sort(myv, myid);
+    wave_merge_lower(tv, ti, myv, myid);
+  }
+  // Near-tie guard: adjacent results whose error intervals overlap are ordered by their
+  // FP64 distances (computed exactly as the FP64 oracle does).
+  const float er = (float)a.e_tree;
+  float nv = __shfl_down(tv, 1, kWave);
+  bool ov = l < kWave - 1 && tv < FLT_MAX && nv < FLT_MAX && nv * (1.f - er) <= tv * (1.f + er);
+  bool ov_prev = __shfl_up(ov, 1, kWave) && l > 0;
+  uint64_t bmask = __ballot(!ov_prev);  // run starts
+  uint64_t upto = bmask & (l == 63 ? ~0ull : ((2ull << l) - 1));
+  int start = 63 - __clzll(upto);
+  uint32_t run = (uint32_t)__popcll(upto);
+  bool amb = (ov || ov_prev) && start < (int)a.k;
+  double key = (double)tv;
+  if (__any(amb)) {
+    if (amb) {
+      const float *xrow = a.xp + size_t(a.inv[ti]) * a.ldx;
+      key = fp64_dist<METRIC>(xrow, qrow, a.d);
+    }
+    uint32_t cnt = (uint32_t)__popcll(__ballot(amb));
+    if (l == 0) atomicAdd(&a.stats[0], (unsigned long long)cnt);
+    // A near-tie run that reaches the last kept slot could continue past it.
+    if (__shfl(amb ? 1 : 0, kWave - 1, kWave) && total > (uint32_t)kWave && l == 0)
+      atomicAdd(&a.stats[2], 1ull);
+    // Reported distances stay the FP32 re-ranked values, carried with their ids.
+    wave_sort_runs(run, key, ti, tv);
+  }
+  if (l == 0) atomicAdd(&a.stats[1], (unsigned long long)reranked);
+  if (l < (int)a.k) {
+    a.out_d[size_t(q) * a.ldo + l] = tv;
+    a.out_i[size_t(q) * a.ldo + l] = ti;
+  }
+}
+
+int launch_select(const SelectArgs &a, int metric, hipStream_t st) {
+  uint32_t P = tree_width(a.d);
+  dim3 g(cdiv(a.m, 4)), b(256);
+#define RBC_SEL(M, N) hipLaunchKernelGGL((select_kernel<M, N>), g, b, 0, st, a)
+  if (metric == 0) {
+    if (P <= 64) RBC_SEL(0, 1);
+    else if (P == 128) RBC_SEL(0, 2);
+    else RBC_SEL(0, 4);
+  } else {
+    if (P <= 64) RBC_SEL(1, 1);
+    else if (P == 128) RBC_SEL(1, 2);
+    else RBC_SEL(1, 4);
+  }
+#undef RBC_SEL
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+// ---- host orchestration ---------------------------------------------------------------
+
+struct Timer {
+  hipEvent_t ev[8];
+  int n = 0;
+  bool on = false;
+  hipStream_t st;
+  void init(bool enable, hipStream_t s) {
+    on = enable;
+    st = s;
+    if (on)
+      for (auto &e : ev) (void)hipEventCreate(&e);
+  }
+  void mark() {
+    if (on && n < 8) (void)hipEventRecord(ev[n++], st);
+  }
+  float ms(int a, int b) {
+    float t = 0.f;
+    if (on && b < n) (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+    return t;
+  }
+  ~Timer() {
+    if (on)
+      for (auto &e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, uint32_t k, int precision,
+                       float *od, uint32_t *oi, uint32_t ldo, hipStream_t st, rbc_query_stats *stats) {
+  const uint32_t R = h->R, d = h->d;
+  Workspace &ws = h->ws;
+  Timer tm;
+  tm.init(stats != nullptr, st);
+  tm.mark();
+  const double e = fma_rel_err(h->dpad);
+  const uint32_t Kp = std::min<uint32_t>(RBC_MAX_K, k + 8);
+  void *p;
+  // 1. q -> reps
+  RBC_TRY(ws.get(20, size_t(m) * R * 4, &p));
+  float *Dqr = static_cast<float *>(p);
+  if (precision == RBC_PREC_BF16) {
+    RBC_TRY(bf16_qreps(h, qd, ldq, m, Dqr, st));
+  } else {
+    std::vector<ScanItem> iv;
+    for (uint32_t q0 = 0; q0 < m; q0 += kTQ) iv.push_back(ScanItem{0, R, q0, std::min((uint32_t)kTQ, m - q0), 0});
+    DevBuf<ScanItem> items;
+    RBC_TRY(upload_items(iv, items, st));
+    TileSrc src{h->reps.get(), h->ld, nullptr, nullptr, qd, ldq, nullptr, d, h->dpad};
+    EpiDense epi{Dqr, R};
+    RBC_TRY(launch_tiles_m(h->metric, src, items.get(), (uint32_t)iv.size(), epi, st));
+    RBC_HIP_TRY(hipStreamSynchronize(st));
+  }
+  // 2. row stats
+  RBC_TRY(ws.get(21, size_t(m) * 4, &p));
+  uint32_t *near = static_cast<uint32_t *>(p);
+  RBC_TRY(ws.get(22, size_t(m) * 4, &p));
+  float *rep_kth = static_cast<float *>(p);
+  RBC_TRY(fill_f32(rep_kth, FLT_MAX, m, st));
+  hipLaunchKernelGGL(row_stats, dim3(cdiv(m, 4)), dim3(256), 0, st, Dqr, m, R, std::min(k, R), near,
+                     rep_kth);
+  RBC_HIP_TRY(hipGetLastError());
+  if (R < k) RBC_TRY(fill_f32(rep_kth, FLT_MAX, m, st));
+  tm.mark();
+  // 3. probe: group queries by nearest rep, scan that list.
+  RBC_TRY(ws.get(23, size_t(m) * 4, &p));
+  uint32_t *qorder = static_cast<uint32_t *>(p);
+  RBC_TRY(ws.get(24, (size_t(R) + 1) * 4, &p));
+  uint32_t *qoff = static_cast<uint32_t *>(p);
+  RBC_TRY(counting_sort(near, m, R, qorder, qoff, ws, 0, st));
+  DevBuf<ScanItem> pitems;
+  uint32_t npitems = 0;
+  RBC_TRY(make_bucket_items(qoff, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st));
+  RBC_TRY(ws.get(25, size_t(m) * Kp * 4, &p));
+  float *probe_d = static_cast<float *>(p);
+  RBC_TRY(ws.get(26, size_t(m) * Kp * 4, &p));
+  uint32_t *probe_i = static_cast<uint32_t *>(p);
+  double pairs_probe = 0;
+  for (uint32_t j = 0; j < R; ++j) (void)j;
+  if (precision == RBC_PREC_BF16) {
+    RBC_TRY(bf16_probe(h, qd, ldq, qorder, pitems.get(), npitems, Kp, probe_d, probe_i, st));
+  } else {
+    TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, qorder, d, h->dpad};
+    EpiTopK epi{Kp, probe_d, probe_i, Kp, 0};
+    RBC_TRY(launch_tiles_m(h->metric, src, pitems.get(), npitems, epi, st));
+  }
+  // 3b. bounds
+  RBC_TRY(ws.get(27, size_t(m) * 4, &p));
+  float *tau = static_cast<float *>(p);
+  RBC_TRY(ws.get(28, size_t(m) * 8, &p));
+  double *gam = static_cast<double *>(p);
+  RBC_TRY(ws.get(29, size_t(m) * 4, &p));
+  uint32_t *cert = static_cast<uint32_t *>(p);
+  const double e_filter = precision == RBC_PREC_BF16 ? 0.0 : e;
+  if (precision == RBC_PREC_BF16) {
+    RBC_TRY(bf16_bounds(h, qd, ldq, m, k, Kp, probe_d, rep_kth, near, tau, gam, cert, st));
+  } else {
+    hipLaunchKernelGGL(probe_bounds, dim3(cdiv(m, 256)), dim3(256), 0, st, probe_d, Kp, k, rep_kth, near,
+                       h->list_len.get(), m, e, h->metric, tau, gam, cert);
+    RBC_HIP_TRY(hipGetLastError());
+  }
+  (void)e_filter;
+  tm.mark();
+  // 4. plan
+  const uint32_t W = cdiv(R, 32);
+  RBC_TRY(ws.get(30, size_t(m) * W * 4, &p));
+  uint32_t *bits = static_cast<uint32_t *>(p);
+  RBC_TRY(ws.get(31, (size_t(R) + 1) * 4, &p));
+  uint32_t *lcnt = static_cast<uint32_t *>(p);
+  RBC_TRY(ws.get(32, (size_t(R) + 1) * 4, &p));
+  uint32_t *cursor = static_cast<uint32_t *>(p);
+  RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
+  RBC_TRY(fill_u32(cursor, 0, R + 1, st));
+  const double e_plan = precision == RBC_PREC_BF16 ? bf16_plan_err(h) : e;
+  hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, Dqr, m, R, near, gam, cert,
+                     h->radius.get(), h->rr.get(), e_plan, h->metric, W, bits, lcnt);
+  RBC_HIP_TRY(hipGetLastError());
+  RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));
+  RBC_TRY(ws.get(33, 4, &p));
+  uint32_t total_pairs = 0;
+  RBC_HIP_TRY(hipMemcpyAsync(&total_pairs, lcnt + R, 4, hipMemcpyDeviceToHost, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  RBC_TRY(ws.get(34, size_t(std::max(total_pairs, 1u)) * 4, &p));
+  uint32_t *qlist = static_cast<uint32_t *>(p);
+  hipLaunchKernelGGL(plan_scatter, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, m, R, W, bits, lcnt, cursor,
+                     qlist);
+  RBC_HIP_TRY(hipGetLastError());
+  DevBuf<ScanItem> sitems;
+  uint32_t nsitems = 0;
+  RBC_TRY(make_bucket_items(lcnt, h->list_off.get(), h->list_len.get(), R, sitems, &nsitems, ws, 16, st));
+  tm.mark();
+  // 5. scan with candidate emission; re-run with a larger buffer on overflow.
+  uint32_t cap = std::max<uint32_t>(256, 4 * k);
+  RBC_TRY(ws.get(36, size_t(m) * 4, &p));
+  uint32_t *cand_n = static_cast<uint32_t *>(p);
+  float *cand_d = nullptr;
+  uint32_t *cand_i = nullptr;
+  double fallbacks = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    RBC_TRY(ws.get(37, size_t(m) * cap * 4, &p));
+    cand_d = static_cast<float *>(p);
+    RBC_TRY(ws.get(38, size_t(m) * cap * 4, &p));
+    cand_i = static_cast<uint32_t *>(p);
+    RBC_TRY(fill_u32(cand_n, 0, m, st));
+    if (precision == RBC_PREC_BF16) {
+      RBC_TRY(bf16_scan(h, qd, ldq, qlist, sitems.get(), nsitems, tau, cand_d, cand_i, cand_n, cap, st));
+    } else {
+      TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, qlist, d, h->dpad};
+      EpiEmit epi{tau, cand_d, cand_i, cand_n, cap};
+      RBC_TRY(launch_tiles_m(h->metric, src, sitems.get(), nsitems, epi, st));
+    }
+    // Largest candidate count.
+    std::vector<uint32_t> hn(m);
+    RBC_HIP_TRY(hipMemcpyAsync(hn.data(), cand_n, size_t(m) * 4, hipMemcpyDeviceToHost, st));
+    RBC_HIP_TRY(hipStreamSynchronize(st));
+    uint32_t mx = 0;
+    for (uint32_t v : hn) {
+      mx = std::max(mx, v);
+      if (v > cap) fallbacks += 1;
+    }
+    if (mx <= cap) break;
+    if (attempt == 1) {
+      set_error("candidate buffer overflow persisted (%u > %u)", mx, cap);
+      return RBC_ERR_LIMIT;
+    }
+    cap = mx;
+  }
+  tm.mark();
+  // 6. select
+  RBC_TRY(ws.get(39, 4 * 8, &p));
+  unsigned long long *sdev = static_cast<unsigned long long *>(p);
+  RBC_HIP_TRY(hipMemsetAsync(sdev, 0, 4 * 8, st));
+  SelectArgs sa;
+  sa.q = qd;
+  sa.ldq = ldq;
+  sa.xp = h->xp.get();
+  sa.ldx = h->ld;
+  sa.inv = h->inv.get();
+  sa.m = m;
+  sa.d = d;
+  sa.k = k;
+  sa.Kp = Kp;
+  sa.probe_i = probe_i;
+  sa.cert = cert;
+  sa.cand_i = cand_i;
+  sa.cand_n = cand_n;
+  sa.cap = cap;
+  sa.e_tree = tree_rel_err(tree_width(d));
+  sa.out_d = od;
+  sa.out_i = oi;
+  sa.ldo = ldo;
+  sa.stats = sdev;
+  RBC_TRY(launch_select(sa, h->metric, st));
+  tm.mark();
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  if (stats) {
+    unsigned long long hs[4];
+    RBC_HIP_TRY(hipMemcpy(hs, sdev, sizeof(hs), hipMemcpyDeviceToHost));
+    // Pairs evaluated by the probe and the scan, from the host copies of the items.
+    std::vector<ScanItem> hp(npitems), hsi(nsitems);
+    if (npitems) RBC_HIP_TRY(hipMemcpy(hp.data(), pitems.get(), npitems * sizeof(ScanItem), hipMemcpyDeviceToHost));
+    if (nsitems) RBC_HIP_TRY(hipMemcpy(hsi.data(), sitems.get(), nsitems * sizeof(ScanItem), hipMemcpyDeviceToHost));
+    double pp = 0, ps = 0, bytes = 0;
+    const double bx = precision == RBC_PREC_BF16 ? 2.0 : 4.0;
+    for (auto &it : hp) pp += double(it.x_count) * it.q_count;
+    for (auto &it : hsi) {
+      ps += double(it.x_count) * it.q_count;
+      bytes += bx * d * (double(it.x_count) + it.q_count);
+    }
+    pairs_probe = pp;
+    stats->pairs_probe = pp;
+    stats->pairs_scan = ps;
+    stats->pairs_rerank = (double)hs[1];
+    stats->items_scan = nsitems;
+    stats->lists_scanned = total_pairs;
+    stats->bytes_scan = bytes;
+    stats->guard_fp64 = (double)hs[0];
+    stats->fallbacks = fallbacks;
+    stats->ms_qreps = tm.ms(0, 1);
+    stats->ms_probe = tm.ms(1, 2);
+    stats->ms_plan = tm.ms(2, 3);
+    stats->ms_scan = tm.ms(3, 4);
+    stats->ms_select = tm.ms(4, 5);
+    stats->ms_total = tm.ms(0, 5);
+  }
+  (void)pairs_probe;
+  return RBC_OK;
+}
+
+}  // namespace rbc
+
+using namespace rbc;
+
+extern "C" int rbc_exact_kquery(rbc_exact *h, const float *q, uint32_t m, uint32_t ldq, uint32_t k,
+                                int precision, uint32_t *nn_ids, float *nn_dists, uint32_t ldo, int mem,
+                                void *stream, rbc_query_stats *stats) {
+  RBC_ARG_CHECK(h, "NULL handle");
+  RBC_ARG_CHECK(k >= 1 && k <= RBC_MAX_K, "k=%u outside [1, %d]", k, RBC_MAX_K);
+  RBC_ARG_CHECK(k <= h->n, "k=%u exceeds the database size %u", k, h->n);
+  RBC_ARG_CHECK(ldo >= k, "output leading dimension %u < k", ldo);
+  RBC_ARG_CHECK(precision == RBC_PREC_FP32 || precision == RBC_PREC_BF16, "unknown precision %d",
+                precision);
+  RBC_ARG_CHECK(precision == RBC_PREC_FP32 || h->metric == RBC_L2, "BF16 filter needs the L2 metric");
+  RBC_ARG_CHECK(nn_ids && nn_dists, "NULL output");
+  if (stats) *stats = rbc_query_stats{};
+  if (m == 0) return RBC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DevMat Q;
+  RBC_TRY(Q.stage(q, m, h->d, ldq, mem, st));
+  if (mem == RBC_MEM_DEVICE)
+    return exact_query_common(h, Q.p, Q.ld, m, k, precision, nn_dists, nn_ids, ldo, st, stats);
+  DevBuf<float> od;
+  DevBuf<uint32_t> oi;
+  RBC_TRY(od.alloc(size_t(m) * k));
+  RBC_TRY(oi.alloc(size_t(m) * k));
+  RBC_TRY(exact_query_common(h, Q.p, Q.ld, m, k, precision, od.get(), oi.get(), k, st, stats));
+  RBC_HIP_TRY(hipMemcpy2D(nn_dists, size_t(ldo) * 4, od.get(), size_t(k) * 4, size_t(k) * 4, m,
+                          hipMemcpyDeviceToHost));
+  RBC_HIP_TRY(hipMemcpy2D(nn_ids, size_t(ldo) * 4, oi.get(), size_t(k) * 4, size_t(k) * 4, m,
+                          hipMemcpyDeviceToHost));
+  return RBC_OK;
+}

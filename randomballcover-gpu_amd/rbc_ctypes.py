@@ -1,0 +1,288 @@
+"""ctypes binding of librbc_hip.so (the engine's C ABI, include/rbc_hip.h).
+
+The Python names mirror the reference's C entry points (bruteK, bruteSearch,
+bruteRangeCount, bruteCPU, buildRBC, queryRBC, kqueryRBC, destroyRBC) plus the exact
+search handle.  Host arrays are numpy; device arrays are anything with a
+``data_ptr()`` (torch tensors on the GPU).  Every call raises RBCError on a non-zero
+status.  Loading fails loudly when the library has not been built: there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librbc_hip.so")
+
+KMAX = 32
+DUMMY_IDX = 0xFFFFFFFF
+L2, L1 = 0, 1
+MEM_HOST, MEM_DEVICE = 0, 1
+PREC_FP32, PREC_BF16 = 0, 1
+
+_u32 = ctypes.c_uint32
+_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+
+
+class RBCError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"rbc status {status}: {msg}")
+        self.status = status
+
+
+class Matrix(ctypes.Structure):  # defs.h:56-63
+    _fields_ = [("mat", _vp), ("r", _u32), ("c", _u32), ("pr", _u32), ("pc", _u32), ("ld", _u32)]
+
+
+class IntMatrix(ctypes.Structure):  # defs.h:76-83
+    _fields_ = [("mat", _vp), ("r", _u32), ("c", _u32), ("pr", _u32), ("pc", _u32), ("ld", _u32)]
+
+
+class RBCStruct(ctypes.Structure):  # defs.h:95-100
+    _fields_ = [("dx", Matrix), ("dxMap", IntMatrix), ("dr", Matrix), ("groupCount", _vp)]
+
+
+class QueryStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "pairs_probe", "pairs_scan", "pairs_rerank", "items_scan", "lists_scanned", "bytes_scan",
+        "guard_fp64", "fallbacks", "ms_qreps", "ms_probe", "ms_plan", "ms_scan", "ms_select",
+        "ms_total")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class ExactInfo(ctypes.Structure):
+    _fields_ = [("n", _u32), ("d", _u32), ("ld", _u32), ("num_reps", _u32), ("metric", _u32),
+                ("max_list", _u32), ("min_list", _u32), ("device_bytes", ctypes.c_size_t)]
+
+
+_lib = None
+
+_SIGS = {
+    "rbc_last_error": (ctypes.c_char_p, []),
+    "rbc_version": (_int, []),
+    "rbc_set_device": (_int, [_int]),
+    "rbc_set_metric": (_int, [_int]),
+    "rbc_get_metric": (_int, []),
+    "rbc_brute_k": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _u32, _vp, _vp, _u32, _int,
+                           _vp]),
+    "rbc_brute_search": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _vp, _vp, _int, _vp]),
+    "rbc_brute_range_count": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _vp, _vp, _int,
+                                     _vp]),
+    "rbc_brute_cpu": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _vp]),
+    "rbc_build": (_int, [_vp, _u32, _u32, _u32, _u32, _u32, _vp, _u64, _int,
+                         ctypes.POINTER(RBCStruct)]),
+    "rbc_query": (_int, [_vp, _u32, _u32, ctypes.POINTER(RBCStruct), _vp, _vp, _int, _vp]),
+    "rbc_kquery": (_int, [_vp, _u32, _u32, ctypes.POINTER(RBCStruct), _vp, _vp, _u32, _int, _vp]),
+    "rbc_destroy": (_int, [ctypes.POINTER(RBCStruct)]),
+    "rbc_exact_build": (_int, [_vp, _u32, _u32, _u32, _u32, _int, _vp, _u64, _int,
+                               ctypes.POINTER(_vp)]),
+    "rbc_exact_kquery": (_int, [_vp, _vp, _u32, _u32, _u32, _int, _vp, _vp, _u32, _int, _vp,
+                                ctypes.POINTER(QueryStats)]),
+    "rbc_exact_get_info": (_int, [_vp, ctypes.POINTER(ExactInfo)]),
+    "rbc_exact_get_lists": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rbc_exact_destroy": (_int, [_vp]),
+}
+
+
+def lib():
+    """Loads librbc_hip.so; raises if it is missing (build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RBCError(-1, f"{LIB_PATH} is not built")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != 0:
+        raise RBCError(status, lib().rbc_last_error().decode())
+
+
+def _host(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, a.ctypes.data
+
+
+def _ptr(a):
+    return a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+
+
+def set_metric(metric):
+    _check(lib().rbc_set_metric(metric))
+
+
+# ------------------------------------------------------------------ brute force --
+
+def bruteK(x, q, k=KMAX, metric=L2):
+    """bruteK (brute.cu:65-88) on host arrays: (ids [m,k] uint32, dists [m,k] float32)."""
+    x, xp = _host(x, np.float32)
+    q, qp = _host(q, np.float32)
+    m, d = q.shape
+    ids = np.empty((m, k), np.uint32)
+    ds = np.empty((m, k), np.float32)
+    _check(lib().rbc_brute_k(xp, x.shape[0], x.shape[1], qp, m, q.shape[1], d, metric, k,
+                             ids.ctypes.data, ds.ctypes.data, k, MEM_HOST, None))
+    return ids, ds
+
+
+def bruteSearch(x, q, metric=L2):
+    x, xp = _host(x, np.float32)
+    q, qp = _host(q, np.float32)
+    m, d = q.shape
+    ids = np.empty(m, np.uint32)
+    ds = np.empty(m, np.float32)
+    _check(lib().rbc_brute_search(xp, x.shape[0], x.shape[1], qp, m, q.shape[1], d, metric,
+                                  ids.ctypes.data, ds.ctypes.data, MEM_HOST, None))
+    return ids, ds
+
+
+def bruteRangeCount(x, q, ranges, metric=L2):
+    x, xp = _host(x, np.float32)
+    q, qp = _host(q, np.float32)
+    r, rp = _host(ranges, np.float32)
+    m, d = q.shape
+    cnt = np.empty(m, np.uint32)
+    _check(lib().rbc_brute_range_count(xp, x.shape[0], x.shape[1], qp, m, q.shape[1], d, metric,
+                                       rp, cnt.ctypes.data, MEM_HOST, None))
+    return cnt
+
+
+def bruteCPU(x, q, metric=L2):
+    x, xp = _host(x, np.float32)
+    q, qp = _host(q, np.float32)
+    ids = np.empty(q.shape[0], np.uint32)
+    _check(lib().rbc_brute_cpu(xp, x.shape[0], x.shape[1], qp, q.shape[0], q.shape[1], q.shape[1],
+                               metric, ids.ctypes.data))
+    return ids
+
+
+# ------------------------------------------------------------- one-shot RBC ----
+
+class OneShotRBC:
+    """buildRBC / queryRBC / kqueryRBC / destroyRBC (rbc.cu) over an rbcStruct."""
+
+    def __init__(self, x, num_reps, s=None, rep_ids=None, seed=0):
+        x, xp = _host(x, np.float32)
+        self.n, self.d = x.shape
+        self.s = num_reps if s is None else s
+        self.rbcs = RBCStruct()
+        rp = None
+        if rep_ids is not None:
+            self._reps, rp = _host(rep_ids, np.uint32)
+        _check(lib().rbc_build(xp, self.n, self.d, self.d, num_reps, self.s, rp, seed, MEM_HOST,
+                               ctypes.byref(self.rbcs)))
+        self.num_reps = num_reps
+
+    def group_counts(self):
+        n = self.num_reps
+        return np.ctypeslib.as_array(ctypes.cast(self.rbcs.groupCount,
+                                                 ctypes.POINTER(ctypes.c_uint32)), (n,)).copy()
+
+    def query(self, q):
+        q, qp = _host(q, np.float32)
+        m = q.shape[0]
+        ids = np.empty(m, np.uint32)
+        ds = np.empty(m, np.float32)
+        _check(lib().rbc_query(qp, m, q.shape[1], ctypes.byref(self.rbcs), ids.ctypes.data,
+                               ds.ctypes.data, MEM_HOST, None))
+        return ids, ds
+
+    def kquery(self, q):
+        q, qp = _host(q, np.float32)
+        m = q.shape[0]
+        ids = np.empty((m, KMAX), np.uint32)
+        ds = np.empty((m, KMAX), np.float32)
+        _check(lib().rbc_kquery(qp, m, q.shape[1], ctypes.byref(self.rbcs), ids.ctypes.data,
+                                ds.ctypes.data, KMAX, MEM_HOST, None))
+        return ids, ds
+
+    def close(self):
+        if self.rbcs.dx.mat:
+            _check(lib().rbc_destroy(ctypes.byref(self.rbcs)))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- exact RBC ----
+
+class ExactRBC:
+    """Exact k-NN index (rbc_exact_*).  x may be a host numpy array or a device tensor."""
+
+    def __init__(self, x, num_reps, metric=L2, rep_ids=None, seed=0, device=False):
+        self.h = _vp()
+        if device:
+            n, d = x.shape
+            xp, ld, mem = _ptr(x), x.stride(0), MEM_DEVICE
+        else:
+            x, xp = _host(x, np.float32)
+            n, d = x.shape
+            ld, mem = d, MEM_HOST
+        rp = None
+        if rep_ids is not None:
+            self._reps, rp = _host(rep_ids, np.uint32)
+        _check(lib().rbc_exact_build(xp, n, d, ld, num_reps, metric, rp, seed, mem,
+                                     ctypes.byref(self.h)))
+        self.n, self.d, self.num_reps, self.metric = n, d, num_reps, metric
+
+    def kquery(self, q, k, precision=PREC_FP32, stats=False):
+        q, qp = _host(q, np.float32)
+        m = q.shape[0]
+        ids = np.empty((m, k), np.uint32)
+        ds = np.empty((m, k), np.float32)
+        st = QueryStats()
+        _check(lib().rbc_exact_kquery(self.h, qp, m, q.shape[1], k, precision, ids.ctypes.data,
+                                      ds.ctypes.data, k, MEM_HOST, None, ctypes.byref(st)))
+        return (ids, ds, st.as_dict()) if stats else (ids, ds)
+
+    def kquery_device(self, q, k, ids, ds, precision=PREC_FP32, stream=None):
+        """q, ids, ds are device tensors ([m,d] float32, [m,k] int32/uint32, [m,k] float32)."""
+        st = QueryStats()
+        _check(lib().rbc_exact_kquery(self.h, _ptr(q), q.shape[0], q.stride(0), k, precision,
+                                      _ptr(ids), _ptr(ds), ids.stride(0), MEM_DEVICE, stream,
+                                      ctypes.byref(st)))
+        return st.as_dict()
+
+    def info(self):
+        inf = ExactInfo()
+        _check(lib().rbc_exact_get_info(self.h, ctypes.byref(inf)))
+        return {n: getattr(inf, n) for n, _ in inf._fields_}
+
+    def lists(self):
+        R, n = self.num_reps, self.n
+        reps = np.empty(R, np.uint32)
+        off = np.empty(R + 1, np.uint32)
+        perm = np.empty(n, np.uint32)
+        owner = np.empty(n, np.uint32)
+        radii = np.empty(R, np.float32)
+        _check(lib().rbc_exact_get_lists(self.h, reps.ctypes.data, off.ctypes.data,
+                                         perm.ctypes.data, owner.ctypes.data, radii.ctypes.data))
+        return {"rep_ids": reps, "offsets": off, "perm": perm, "owner": owner, "radii": radii}
+
+    def close(self):
+        if self.h:
+            _check(lib().rbc_exact_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

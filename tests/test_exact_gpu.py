@@ -1,0 +1,81 @@
+"""GPU parity of the exact RBC (build and k-query) against the CPU oracle.
+
+Indices must equal the FP64 brute-force oracle's (ties to the lowest index) and the
+returned distances must be within 1e-6 relative of FP64."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import datagen, rbc
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-6  # distances vs FP64
+
+
+def check_against_fp64(x, q, ids, ds, k, metric=0):
+    oi, od = oracle.brute_k_f64(x, q, k, metric)
+    bad = np.nonzero((ids != oi).any(1))[0]
+    assert bad.size == 0, f"{bad.size} queries differ, first {bad[:5]}: {ids[bad[0]]} vs {oi[bad[0]]}"
+    rel = np.abs(ds.astype(np.float64) - od) / np.maximum(od, 1e-300)
+    rel[od == 0] = np.abs(ds[od == 0])
+    assert rel.max() <= REL_TOL, f"max relative error {rel.max()}"
+
+
+@pytest.mark.parametrize("gen,n,m,d,R,k,metric", [
+    ("uniform", 20000, 512, 16, 141, 1, 0),
+    ("gmm", 30000, 400, 64, 173, 32, 0),
+    ("gmm", 30000, 400, 64, 173, 32, 1),
+    ("lowrank", 40000, 300, 128, 200, 10, 0),
+    ("uniform", 5000, 257, 33, 70, 7, 1),
+])
+def test_exact_fp32_matches_fp64_oracle(gen, n, m, d, R, k, metric):
+    g = getattr(datagen, gen)
+    x, q = g(n, d, 0), g(m, d, 1)
+    idx = rbc.ExactRBC(x, R, metric=metric, seed=3)
+    ids, ds, st = idx.kquery(q, k, precision=rbc.PREC_FP32, stats=True)
+    check_against_fp64(x, q, ids, ds, k, metric)
+    assert st["pairs_scan"] + st["pairs_probe"] > 0
+
+
+def test_exact_build_lists_match_oracle():
+    x = datagen.gmm(20000, 32, 0)
+    reps = np.sort(np.random.default_rng(5).choice(20000, 100, replace=False)).astype(np.uint32)
+    idx = rbc.ExactRBC(x, 100, rep_ids=reps)
+    L = idx.lists()
+    np.testing.assert_array_equal(L["rep_ids"], reps)
+    own = oracle.exact_owners(x, reps)
+    np.testing.assert_array_equal(L["owner"], own)
+    np.testing.assert_array_equal(np.diff(L["offsets"]), np.bincount(own, minlength=100))
+    # Lists are the points of each owner in ascending original index.
+    np.testing.assert_array_equal(L["perm"], np.argsort(own, kind="stable"))
+    np.testing.assert_array_equal(L["radii"], oracle.exact_radii(x, reps, own))
+
+
+def test_exact_duplicates_and_ties():
+    base = datagen.uniform(3000, 16, 0)
+    x = np.concatenate([base, base, base[:1000]])
+    q = np.concatenate([base[:50], datagen.uniform(50, 16, 1)])  # queries equal to points
+    idx = rbc.ExactRBC(x, 60, seed=1)
+    ids, ds = idx.kquery(q, 8)
+    check_against_fp64(x, q, ids, ds, 8)
+
+
+@pytest.mark.parametrize("n,R,m,k", [(40, 1, 5, 3), (40, 40, 5, 40), (100, 7, 1, 64), (65, 9, 130, 1)])
+def test_exact_edge_shapes(n, R, m, k):
+    x = datagen.gmm(n, 24, 0)
+    q = datagen.gmm(m, 24, 1)
+    idx = rbc.ExactRBC(x, R, seed=2)
+    ids, ds = idx.kquery(q, k)
+    check_against_fp64(x, q, ids, ds, k)
+
+
+def test_exact_rejects_bad_arguments():
+    x = datagen.uniform(100, 8, 0)
+    with pytest.raises(rbc.RBCError):
+        rbc.ExactRBC(x, 101)
+    with pytest.raises(rbc.RBCError):
+        rbc.ExactRBC(x, 10, rep_ids=np.array([1, 1, 2, 3, 4, 5, 6, 7, 8, 9]))
+    idx = rbc.ExactRBC(x, 10)
+    with pytest.raises(rbc.RBCError):
+        idx.kquery(datagen.uniform(3, 8, 1), 65)
