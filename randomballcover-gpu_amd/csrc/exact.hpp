@@ -7,11 +7,12 @@
 #include "launch.hpp"
 
 struct rbc_exact {
-  uint32_t n = 0, d = 0, dpad = 0, ld = 0, R = 0;
+  uint32_t n = 0, d = 0, dpad = 0, ld = 0, ldb = 0, R = 0;
   int metric = RBC_L2;
   // Database permuted into contiguous ownership lists (row-major, ld floats per row).
   rbc::DevBuf<float> xp;
-  // BF16 copy of xp (ldb = ld bf16 per row) and per-row FP64 squared norms of it.
+  // BF16 copy of xp (ldb = d rounded up to 32 bf16 per row, zero padded) and the
+  // squared norms of its rows.
   rbc::DevBuf<uint16_t> xb;
   rbc::DevBuf<float> xb_norm;   // ||bf16(x)||^2, rounded to nearest
   rbc::DevBuf<float> list_qerr; // per list: max ||x - bf16(x)|| of its points (rounded up)

@@ -49,8 +49,8 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_floa
 // norm and the BF16 quantisation error ||x - bf16(x)||.
 __global__ __launch_bounds__(256) void permute_rows(const float *x, size_t ldx, const uint32_t *perm,
                                                     const uint32_t *owner, const float *reps, uint32_t n,
-                                                    uint32_t d, uint32_t ld, int metric, float *xp,
-                                                    uint16_t *xb, double *own64, float *bnorm,
+                                                    uint32_t d, uint32_t ld, uint32_t ldb, int metric,
+                                                    float *xp, uint16_t *xb, double *own64, float *bnorm,
                                                     float *qerr) {
   uint32_t pos = blockIdx.x * 4 + threadIdx.x / kWave;
   if (pos >= n) return;
@@ -59,11 +59,12 @@ __global__ __launch_bounds__(256) void permute_rows(const float *x, size_t ldx, 
   const float *src = x + size_t(id) * ldx;
   const float *rp = reps + size_t(owner[id]) * ld;
   double nb = 0.0, qe = 0.0;
-  for (uint32_t c = l; c < ld; c += kWave) {
+  for (uint32_t c = l; c < ldb || c < ld; c += kWave) {
     float v = c < d ? src[c] : 0.f;
-    xp[size_t(pos) * ld + c] = v;
+    if (c < ld) xp[size_t(pos) * ld + c] = v;
+    if (c >= ldb) continue;
     uint16_t h = f32_to_bf16_rne(v);
-    xb[size_t(pos) * ld + c] = h;
+    xb[size_t(pos) * ldb + c] = h;
     double hv = bf16_to_f32(h);
     nb += hv * hv;
     double e = (double)v - hv;
@@ -124,15 +125,15 @@ __global__ __launch_bounds__(256) void list_stats(const uint32_t *off, const dou
 }
 
 // BF16 copies of the reps with their squared norms and quantisation errors.
-__global__ void reps_bf16(const float *reps, uint32_t R, uint32_t d, uint32_t ld, uint16_t *rb,
-                          float *rbn, float *rqe) {
+__global__ void reps_bf16(const float *reps, uint32_t R, uint32_t d, uint32_t ld, uint32_t ldb,
+                          uint16_t *rb, float *rbn, float *rqe) {
   uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= R) return;
   double nb = 0.0, qe = 0.0;
-  for (uint32_t c = 0; c < ld; ++c) {
-    float v = reps[size_t(j) * ld + c];
+  for (uint32_t c = 0; c < ldb; ++c) {
+    float v = c < d ? reps[size_t(j) * ld + c] : 0.f;
     uint16_t h = f32_to_bf16_rne(v);
-    rb[size_t(j) * ld + c] = h;
+    rb[size_t(j) * ldb + c] = h;
     double hv = bf16_to_f32(h);
     nb += hv * hv;
     double e = (double)v - hv;
@@ -208,6 +209,7 @@ int rbc_exact_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32
   h->d = d;
   h->dpad = (d + 3) & ~3u;
   h->ld = pad16(d);
+  h->ldb = (d + 31) & ~31u;
   h->R = num_reps;
   h->metric = metric;
   const uint32_t R = num_reps, ld = h->ld;
@@ -242,14 +244,14 @@ int rbc_exact_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32
 
   // Permuted FP32 + BF16 copies and per-row FP64 data.
   RBC_TRY(h->xp.alloc(size_t(n) * ld));
-  RBC_TRY(h->xb.alloc(size_t(n) * ld));
+  RBC_TRY(h->xb.alloc(size_t(n) * h->ldb));
   RBC_TRY(h->xb_norm.alloc(n));
   DevBuf<double> own64;
   DevBuf<float> qerr;
   RBC_TRY(own64.alloc(n));
   RBC_TRY(qerr.alloc(n));
   hipLaunchKernelGGL(permute_rows, dim3(cdiv(n, 4)), dim3(256), 0, st, X.p, X.ld, h->perm.get(),
-                     h->owner.get(), h->reps.get(), n, d, ld, metric, h->xp.get(), h->xb.get(),
+                     h->owner.get(), h->reps.get(), n, d, ld, h->ldb, metric, h->xp.get(), h->xb.get(),
                      own64.get(), h->xb_norm.get(), qerr.get());
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(h->radius.alloc(R));
@@ -258,10 +260,10 @@ int rbc_exact_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32
   hipLaunchKernelGGL(list_stats, dim3(R), dim3(256), 0, st, h->list_off.get(), own64.get(), qerr.get(),
                      h->xb_norm.get(), h->radius.get(), h->list_qerr.get(), h->list_nmax.get());
   RBC_HIP_TRY(hipGetLastError());
-  RBC_TRY(h->reps_b.alloc(size_t(R) * ld));
+  RBC_TRY(h->reps_b.alloc(size_t(R) * h->ldb));
   RBC_TRY(h->reps_bnorm.alloc(R));
   RBC_TRY(h->reps_qerr.alloc(R));
-  hipLaunchKernelGGL(reps_bf16, dim3(cdiv(R, 64)), dim3(64), 0, st, h->reps.get(), R, d, ld,
+  hipLaunchKernelGGL(reps_bf16, dim3(cdiv(R, 64)), dim3(64), 0, st, h->reps.get(), R, d, ld, h->ldb,
                      h->reps_b.get(), h->reps_bnorm.get(), h->reps_qerr.get());
   RBC_HIP_TRY(hipGetLastError());
 

@@ -343,9 +343,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   // 1. q -> reps
   RBC_TRY(ws.get(20, size_t(m) * R * 4, &p));
   float *Dqr = static_cast<float *>(p);
-  if (precision == RBC_PREC_BF16) {
-    RBC_TRY(bf16_qreps(h, qd, ldq, m, Dqr, st));
-  } else {
+  {
     std::vector<ScanItem> iv;
     for (uint32_t q0 = 0; q0 < m; q0 += kTQ) iv.push_back(ScanItem{0, R, q0, std::min((uint32_t)kTQ, m - q0), 0});
     DevBuf<ScanItem> items;
@@ -379,11 +377,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   float *probe_d = static_cast<float *>(p);
   RBC_TRY(ws.get(26, size_t(m) * Kp * 4, &p));
   uint32_t *probe_i = static_cast<uint32_t *>(p);
-  double pairs_probe = 0;
-  for (uint32_t j = 0; j < R; ++j) (void)j;
-  if (precision == RBC_PREC_BF16) {
-    RBC_TRY(bf16_probe(h, qd, ldq, qorder, pitems.get(), npitems, Kp, probe_d, probe_i, st));
-  } else {
+  {
     TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, qorder, d, h->dpad};
     EpiTopK epi{Kp, probe_d, probe_i, Kp, 0};
     RBC_TRY(launch_tiles_m(h->metric, src, pitems.get(), npitems, epi, st));
@@ -395,15 +389,9 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   double *gam = static_cast<double *>(p);
   RBC_TRY(ws.get(29, size_t(m) * 4, &p));
   uint32_t *cert = static_cast<uint32_t *>(p);
-  const double e_filter = precision == RBC_PREC_BF16 ? 0.0 : e;
-  if (precision == RBC_PREC_BF16) {
-    RBC_TRY(bf16_bounds(h, qd, ldq, m, k, Kp, probe_d, rep_kth, near, tau, gam, cert, st));
-  } else {
-    hipLaunchKernelGGL(probe_bounds, dim3(cdiv(m, 256)), dim3(256), 0, st, probe_d, Kp, k, rep_kth, near,
-                       h->list_len.get(), m, e, h->metric, tau, gam, cert);
-    RBC_HIP_TRY(hipGetLastError());
-  }
-  (void)e_filter;
+  hipLaunchKernelGGL(probe_bounds, dim3(cdiv(m, 256)), dim3(256), 0, st, probe_d, Kp, k, rep_kth, near,
+                     h->list_len.get(), m, e, h->metric, tau, gam, cert);
+  RBC_HIP_TRY(hipGetLastError());
   tm.mark();
   // 4. plan
   const uint32_t W = cdiv(R, 32);
@@ -415,9 +403,8 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint32_t *cursor = static_cast<uint32_t *>(p);
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
-  const double e_plan = precision == RBC_PREC_BF16 ? bf16_plan_err(h) : e;
   hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, Dqr, m, R, near, gam, cert,
-                     h->radius.get(), h->rr.get(), e_plan, h->metric, W, bits, lcnt);
+                     h->radius.get(), h->rr.get(), e, h->metric, W, bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));
   RBC_TRY(ws.get(33, 4, &p));
@@ -431,7 +418,20 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_HIP_TRY(hipGetLastError());
   DevBuf<ScanItem> sitems;
   uint32_t nsitems = 0;
-  RBC_TRY(make_bucket_items(lcnt, h->list_off.get(), h->list_len.get(), R, sitems, &nsitems, ws, 16, st));
+  const bool bf16 = precision == RBC_PREC_BF16;
+  RBC_TRY(make_bucket_items(lcnt, h->list_off.get(), h->list_len.get(), R, sitems, &nsitems, ws, 16, st,
+                            bf16 ? 128u : (uint32_t)kTQ));
+  uint16_t *qb = nullptr;
+  float *qn = nullptr, *qe = nullptr;
+  if (bf16) {
+    RBC_TRY(ws.get(40, size_t(m) * h->ldb * 2, &p));
+    qb = static_cast<uint16_t *>(p);
+    RBC_TRY(ws.get(41, size_t(m) * 4, &p));
+    qn = static_cast<float *>(p);
+    RBC_TRY(ws.get(42, size_t(m) * 4, &p));
+    qe = static_cast<float *>(p);
+    RBC_TRY(bf16_prepare_queries(h, qd, ldq, m, qb, qn, qe, st));
+  }
   tm.mark();
   // 5. scan with candidate emission; re-run with a larger buffer on overflow.
   uint32_t cap = std::max<uint32_t>(256, 4 * k);
@@ -447,7 +447,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     cand_i = static_cast<uint32_t *>(p);
     RBC_TRY(fill_u32(cand_n, 0, m, st));
     if (precision == RBC_PREC_BF16) {
-      RBC_TRY(bf16_scan(h, qd, ldq, qlist, sitems.get(), nsitems, tau, cand_d, cand_i, cand_n, cap, st));
+      RBC_TRY(bf16_scan(h, qb, qn, qe, gam, qlist, sitems.get(), nsitems, cand_d, cand_i, cand_n, cap, st));
     } else {
       TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, qlist, d, h->dpad};
       EpiEmit epi{tau, cand_d, cand_i, cand_n, cap};
@@ -500,18 +500,22 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   if (stats) {
     unsigned long long hs[4];
     RBC_HIP_TRY(hipMemcpy(hs, sdev, sizeof(hs), hipMemcpyDeviceToHost));
-    // Pairs evaluated by the probe and the scan, from the host copies of the items.
-    std::vector<ScanItem> hp(npitems), hsi(nsitems);
-    if (npitems) RBC_HIP_TRY(hipMemcpy(hp.data(), pitems.get(), npitems * sizeof(ScanItem), hipMemcpyDeviceToHost));
-    if (nsitems) RBC_HIP_TRY(hipMemcpy(hsi.data(), sitems.get(), nsitems * sizeof(ScanItem), hipMemcpyDeviceToHost));
+    // Pairs evaluated by the probe and the scan, from the per-list query counts.
+    std::vector<uint32_t> hq(R + 1), hl(R + 1);
+    RBC_HIP_TRY(hipMemcpy(hq.data(), qoff, (R + 1) * 4, hipMemcpyDeviceToHost));
+    RBC_HIP_TRY(hipMemcpy(hl.data(), lcnt, (R + 1) * 4, hipMemcpyDeviceToHost));
     double pp = 0, ps = 0, bytes = 0;
-    const double bx = precision == RBC_PREC_BF16 ? 2.0 : 4.0;
-    for (auto &it : hp) pp += double(it.x_count) * it.q_count;
-    for (auto &it : hsi) {
-      ps += double(it.x_count) * it.q_count;
-      bytes += bx * d * (double(it.x_count) + it.q_count);
+    // Algorithmic bytes of the scan: each item reads its list once and its queries once
+    // (BF16 rows of ldb, or FP32 rows of d); emitted candidates are not counted.
+    const double row_bytes = bf16 ? 2.0 * h->ldb : 4.0 * d;
+    const double tile = bf16 ? 128.0 : (double)kTQ;
+    for (uint32_t j = 0; j < R; ++j) {
+      const double len = h->h_list_off[j + 1] - h->h_list_off[j];
+      const double cq = hl[j + 1] - hl[j];
+      pp += double(hq[j + 1] - hq[j]) * len;
+      ps += cq * len;
+      bytes += row_bytes * (std::ceil(cq / tile) * len + cq);
     }
-    pairs_probe = pp;
     stats->pairs_probe = pp;
     stats->pairs_scan = ps;
     stats->pairs_rerank = (double)hs[1];
@@ -527,7 +531,6 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     stats->ms_select = tm.ms(4, 5);
     stats->ms_total = tm.ms(0, 5);
   }
-  (void)pairs_probe;
   return RBC_OK;
 }
 

@@ -27,15 +27,10 @@ struct SelectArgs {
 int launch_select(const SelectArgs &a, int metric, hipStream_t st);
 
 // BF16 MFMA stages (exact_bf16.hip).
-int bf16_qreps(rbc_exact *h, const float *q, size_t ldq, uint32_t m, float *Dqr, hipStream_t st);
-int bf16_probe(rbc_exact *h, const float *q, size_t ldq, const uint32_t *qorder, const ScanItem *items,
-               uint32_t nitems, uint32_t Kp, float *probe_d, uint32_t *probe_i, hipStream_t st);
-int bf16_bounds(rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint32_t k, uint32_t Kp,
-                const float *probe_d, const float *rep_kth, const uint32_t *near, float *tau,
-                double *gam, uint32_t *cert, hipStream_t st);
-int bf16_scan(rbc_exact *h, const float *q, size_t ldq, const uint32_t *qlist, const ScanItem *items,
-              uint32_t nitems, const float *tau, float *cand_d, uint32_t *cand_i, uint32_t *cand_n,
-              uint32_t cap, hipStream_t st);
-double bf16_plan_err(const rbc_exact *h);
+int bf16_prepare_queries(rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint16_t *qb, float *qn,
+                         float *qe, hipStream_t st);
+int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe, const double *gam,
+              const uint32_t *qlist, const ScanItem *items, uint32_t nitems, float *cand_d,
+              uint32_t *cand_i, uint32_t *cand_n, uint32_t cap, hipStream_t st);
 
 }  // namespace rbc

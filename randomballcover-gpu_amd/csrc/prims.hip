@@ -140,23 +140,23 @@ __global__ void gather_offsets(const uint32_t *hist_off, uint32_t nb_buckets, ui
 }
 
 // ---- items ----
-__global__ void bucket_tiles(const uint32_t *qoff, uint32_t nb, uint32_t *ntiles) {
+__global__ void bucket_tiles(const uint32_t *qoff, uint32_t nb, uint32_t tile, uint32_t *ntiles) {
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb) ntiles[b] = (qoff[b + 1] - qoff[b] + kTQ - 1) / kTQ;
+  if (b < nb) ntiles[b] = (qoff[b + 1] - qoff[b] + tile - 1) / tile;
 }
 
 __global__ void bucket_items(const uint32_t *qoff, const uint32_t *xoff, const uint32_t *xlen,
-                             uint32_t nb, const uint32_t *toff, ScanItem *items) {
+                             uint32_t nb, uint32_t tile, const uint32_t *toff, ScanItem *items) {
   uint32_t b = blockIdx.x;
   if (b >= nb) return;
   uint32_t q0 = qoff[b], cnt = qoff[b + 1] - q0, t0 = toff[b];
-  uint32_t nt = (cnt + kTQ - 1) / kTQ;
+  uint32_t nt = (cnt + tile - 1) / tile;
   for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) {
     ScanItem it;
     it.x_begin = xoff[b];
     it.x_count = xlen[b];
-    it.q_begin = q0 + t * kTQ;
-    it.q_count = min((uint32_t)kTQ, cnt - t * kTQ);
+    it.q_begin = q0 + t * tile;
+    it.q_count = min(tile, cnt - t * tile);
     it.aux = b;
     items[t0 + t] = it;
   }
@@ -215,11 +215,11 @@ int counting_sort(const uint32_t *key, size_t n, uint32_t nbuckets, uint32_t *or
 
 int make_bucket_items(const uint32_t *qoff, const uint32_t *xoff, const uint32_t *xlen,
                       uint32_t nbuckets, DevBuf<ScanItem> &items_buf, uint32_t *nitems,
-                      Workspace &ws, size_t slot, hipStream_t st) {
+                      Workspace &ws, size_t slot, hipStream_t st, uint32_t tile) {
   void *tv;
   RBC_TRY(ws.get(slot, (size_t(nbuckets) + 1) * sizeof(uint32_t), &tv));
   uint32_t *toff = static_cast<uint32_t *>(tv);
-  hipLaunchKernelGGL(bucket_tiles, dim3(cdiv(nbuckets, 256)), dim3(256), 0, st, qoff, nbuckets, toff);
+  hipLaunchKernelGGL(bucket_tiles, dim3(cdiv(nbuckets, 256)), dim3(256), 0, st, qoff, nbuckets, tile, toff);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(toff, toff, nbuckets, ws, slot + 1, st));
   uint32_t total = 0;
@@ -228,7 +228,7 @@ int make_bucket_items(const uint32_t *qoff, const uint32_t *xoff, const uint32_t
   *nitems = total;
   if (total == 0) return RBC_OK;
   RBC_TRY(items_buf.ensure(total));
-  hipLaunchKernelGGL(bucket_items, dim3(nbuckets), dim3(64), 0, st, qoff, xoff, xlen, nbuckets, toff,
+  hipLaunchKernelGGL(bucket_items, dim3(nbuckets), dim3(64), 0, st, qoff, xoff, xlen, nbuckets, tile, toff,
                      items_buf.get());
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;

@@ -39,13 +39,13 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, Workspace &w
 int counting_sort(const uint32_t *key, size_t n, uint32_t nbuckets, uint32_t *order,
                   uint32_t *offsets, Workspace &ws, size_t slot, hipStream_t st);
 
-// For every bucket b with cnt_b = qoff[b+1]-qoff[b] queries, emits ceil(cnt_b/64) items
-// {x_begin = xoff[b], x_count = xlen[b], q_begin = qoff[b] + 64 t, q_count, aux = b}.
+// For every bucket b with cnt_b = qoff[b+1]-qoff[b] queries, emits ceil(cnt_b/tile) items
+// {x_begin = xoff[b], x_count = xlen[b], q_begin = qoff[b] + tile t, q_count, aux = b}.
 // Returns the number of items on the host (one synchronisation).  Items are written
 // bucket-major.  Uses workspace slots [slot, slot+5) and items_buf.
 int make_bucket_items(const uint32_t *qoff, const uint32_t *xoff, const uint32_t *xlen,
                       uint32_t nbuckets, DevBuf<ScanItem> &items_buf, uint32_t *nitems,
-                      Workspace &ws, size_t slot, hipStream_t st);
+                      Workspace &ws, size_t slot, hipStream_t st, uint32_t tile = kTQ);
 
 // Items covering queries [0, m) in tiles of 64 against points [0, n) split into
 // segments of at most seg_len points; aux = segment index.

@@ -79,3 +79,29 @@ def test_exact_rejects_bad_arguments():
     idx = rbc.ExactRBC(x, 10)
     with pytest.raises(rbc.RBCError):
         idx.kquery(datagen.uniform(3, 8, 1), 65)
+
+
+@pytest.mark.parametrize("gen,n,m,d,R,k", [
+    ("lowrank", 40000, 300, 128, 200, 10),
+    ("gmm", 30000, 300, 64, 173, 32),
+    ("uniform", 20000, 300, 16, 141, 1),
+    ("gmm_dirichlet", 30000, 300, 96, 180, 10),
+])
+def test_exact_bf16_matches_fp64_oracle(gen, n, m, d, R, k):
+    g = getattr(datagen, gen)
+    x, q = g(n, d, 0), g(m, d, 1)
+    idx = rbc.ExactRBC(x, R, seed=4)
+    ids, ds, st = idx.kquery(q, k, precision=rbc.PREC_BF16, stats=True)
+    check_against_fp64(x, q, ids, ds, k)
+    fi, fd = idx.kquery(q, k, precision=rbc.PREC_FP32)
+    np.testing.assert_array_equal(ids, fi)
+    np.testing.assert_array_equal(ds, fd)
+
+
+def test_exact_bf16_duplicates():
+    base = datagen.lowrank(4000, 128, 0)
+    x = np.concatenate([base, base[::-1]])
+    q = np.concatenate([base[:40], datagen.lowrank(40, 128, 1)])
+    idx = rbc.ExactRBC(x, 64, seed=1)
+    ids, ds = idx.kquery(q, 10, precision=rbc.PREC_BF16)
+    check_against_fp64(x, q, ids, ds, 10)
