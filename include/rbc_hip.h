@@ -159,6 +159,37 @@ int rbc_exact_get_lists(const rbc_exact *h, uint32_t *rep_ids, uint32_t *list_of
 
 int rbc_exact_destroy(rbc_exact *h);
 
+/* ------------------------------------------------------- learned list proposal -- */
+
+/* A small MLP d_in -> h1 -> h2 -> n_out (GELU, FP16 weights and activations, FP32
+ * accumulation) trained on the GPU by the library's own kernels.  For an index it maps
+ * a query to the ownership list most likely to hold its nearest neighbour (n_out =
+ * num_reps).  No reference counterpart. */
+typedef struct rbc_mlp rbc_mlp;
+
+int rbc_mlp_create(uint32_t d_in, uint32_t h1, uint32_t h2, uint32_t n_out, uint64_t seed,
+                   rbc_mlp **out);
+int rbc_mlp_destroy(rbc_mlp *m);
+int rbc_mlp_num_params(const rbc_mlp *m, size_t *count);
+/* FP32 master parameters, concatenated: W1[h1][d_in], b1[h1], W2[h2][h1], b2[h2],
+ * W3[n_out][h2], b3[n_out] (host buffers). */
+int rbc_mlp_get_params(const rbc_mlp *m, float *params);
+int rbc_mlp_set_params(rbc_mlp *m, const float *params);
+/* Mean softmax cross-entropy of one batch (host x, labels) and its gradient w.r.t. the
+ * parameters (host, same layout as the parameters); no update. */
+int rbc_mlp_loss_grad(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx,
+                      const uint32_t *labels, float *loss, float *grads);
+/* Logits (host, rows x n_out) of host inputs. */
+int rbc_mlp_logits(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, float *logits);
+/* Adam training (beta 0.9/0.999) for `epochs` passes over shuffled batches; epoch_loss
+ * (may be NULL) receives the loss of each epoch's last batch. */
+int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels,
+                  int mem, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                  float *epoch_loss);
+/* Use (or, with NULL, stop using) the network's top-1 list as a second first-probe of
+ * the exact k-query.  The network is not owned by the index. */
+int rbc_exact_set_proposal(rbc_exact *h, rbc_mlp *mlp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,11 @@ struct rbc_exact {
   rbc::DevBuf<float> rr;          // R x R FP32 rep-rep distances (metric value)
   std::vector<uint32_t> h_list_off;
   rbc::Workspace ws;
+  // Candidate-buffer capacity per query, grown to 1.25x the largest count seen, so that
+  // repeated queries on one index do not re-run the scan.
+  uint32_t cand_cap = 0;
+  // Learned list proposal (not owned); NULL = probe only the nearest rep's list.
+  struct rbc_mlp *proposal = nullptr;
 };
 
 namespace rbc {
@@ -49,6 +54,10 @@ inline uint32_t tree_width(uint32_t d) {
   while (P < d) P <<= 1;
   return P;
 }
+
+// Top-1 list of the proposal network for each query row.
+int mlp_propose(struct rbc_mlp *m, const float *q, size_t ldq, uint32_t rows, uint32_t *out, hipStream_t st);
+int mlp_dims(const struct rbc_mlp *m, uint32_t *d_in, uint32_t *n_out);
 
 int nn_device(const float *x, size_t ldx, uint32_t n, const uint32_t *xid, const float *q,
               size_t ldq, uint32_t m, uint32_t d, int metric, float *od, uint32_t *oi,

@@ -93,25 +93,33 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
   const uint32_t item = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
   if (item >= a.nitems) return;
   const ScanItem it = a.items[item];
-  __shared__ float thr[kBQ];
+  __shared__ float thr[kBQ];   // t_i: emit when nx - 2 q~.x~ <= t_i
+  __shared__ float sh[kBQ];    // t_i / 2 plus a margin covering FP32 rounding (pre-filter)
   __shared__ uint32_t qrow[kBQ];
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const uint32_t list = it.aux;
   if (t < kBQ) {
     uint32_t row = (uint32_t)t < it.q_count ? a.qids[it.q_begin + t] : kDummy;
     qrow[t] = row;
-    float th = -INFINITY;
+    float th = -INFINITY, hs = -INFINITY;
     if (row != kDummy) {
       double G = a.gam[row];
       double eta = (double)a.qe[row] + (double)a.lqe[list];
       double nq = (double)a.qn[row];
-      double E = a.c_gemm * (nq + (double)a.lnm[list]) + 1e-30;
+      double nxm = (double)a.lnm[list];
+      double E = a.c_gemm * (nq + nxm) + 1e-30;
       double T = (G + eta) * (G + eta) + E;
       double tt = (T - nq) * (1.0 + 1e-9) + 1e-30;
       th = tt >= (double)FLT_MAX ? INFINITY : (float)tt;
       if ((double)th < tt) th = nextafterf(th, INFINITY);
+      // Pre-filter: nx - 2 acc <= t  implies  acc + (t/2 + margin) >= nx/2 in FP32 when
+      // the margin exceeds the rounding of both forms (|acc| <= (nq + nx)/2).
+      double hh = 0.5 * tt + 0x1p-20 * (fabs(tt) + 2.0 * (nq + nxm)) + 1e-30;
+      hs = hh >= (double)FLT_MAX ? INFINITY : (float)hh;
+      if ((double)hs < hh) hs = nextafterf(hs, INFINITY);
     }
     thr[t] = th;
+    sh[t] = hs;
   }
   __syncthreads();
 
@@ -162,25 +170,174 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
 #pragma unroll
       for (int qb = 0; qb < 8; ++qb)
         acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][s], B[s], acc[qb], 0, 0, 0);
-    // Lane holds C[qb*16 + (l/16)*4 + r][point pl].
+    // Lane holds C[qb*16 + (l/16)*4 + r][point pl].  Branch-free pre-filter: one add
+    // and one max per element; the exact test runs only when some lane may pass.
+    float mx = -INFINITY;
 #pragma unroll
     for (int qb = 0; qb < 8; ++qb) {
-      const int qi0 = qb * 16 + (l / 16) * 4;
-      const float4 th = *reinterpret_cast<const float4 *>(&thr[qi0]);
-      const float tv[4] = {th.x, th.y, th.z, th.w};
+      const float4 s4 = *reinterpret_cast<const float4 *>(&sh[qb * 16 + (l / 16) * 4]);
+      mx = fmaxf(mx, fmaxf(fmaxf(acc[qb][0] + s4.x, acc[qb][1] + s4.y),
+                           fmaxf(acc[qb][2] + s4.z, acc[qb][3] + s4.w)));
+    }
+    const bool hit = pv && mx >= 0.5f * nx;
+    if (__any(hit)) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float v = __builtin_fmaf(-2.f, acc[qb][r], nx);
-        if (pv && v <= tv[r]) {
-          uint32_t row = qrow[qi0 + r];
-          uint32_t slot = atomicAdd(&a.cand_n[row], 1u);
-          if (slot < a.cap) {
-            a.cand_d[size_t(row) * a.cap + slot] = v + a.qn[row];
-            a.cand_i[size_t(row) * a.cap + slot] = a.perm[x0 + p];
+      for (int qb = 0; qb < 8; ++qb) {
+        const int qi0 = qb * 16 + (l / 16) * 4;
+        const float4 th = *reinterpret_cast<const float4 *>(&thr[qi0]);
+        const float tv[4] = {th.x, th.y, th.z, th.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float v = __builtin_fmaf(-2.f, acc[qb][r], nx);
+          if (hit && v <= tv[r]) {
+            uint32_t row = qrow[qi0 + r];
+            uint32_t slot = atomicAdd(&a.cand_n[row], 1u);
+            if (slot < a.cap) {
+              a.cand_d[size_t(row) * a.cap + slot] = v + a.qn[row];
+              a.cand_i[size_t(row) * a.cap + slot] = a.perm[x0 + p];
+            }
           }
         }
       }
     }
+  }
+}
+
+
+// ---- BF16 probe -----------------------------------------------------------------------
+// Same tiling as the scan; every query keeps its K' (<= 32) smallest approximate
+// distances in LDS.  Candidates under the running K'-th are staged per step; one thread
+// per query then folds them in by insertion.
+constexpr int kPK = 32;
+
+struct Bf16ProbeArgs {
+  const ScanItem *items;
+  uint32_t nitems;
+  const uint32_t *qids;
+  const uint16_t *qb;
+  const float *qn;
+  const uint16_t *xb;
+  const float *xn;
+  const uint32_t *perm;
+  uint32_t ldb, Kp;
+  uint32_t *out_i;   // [m][Kp] original ids, ascending approximate distance
+  float *out_kth;    // [m] approximate D~ of the K'-th (FLT_MAX if the list is shorter)
+};
+
+template <int KS>
+__global__ __launch_bounds__(256) void bf16_probe_kernel(Bf16ProbeArgs a) {
+  const ScanItem it = a.items[blockIdx.x];
+  __shared__ uint32_t qrow[kBQ];
+  __shared__ float qnrm[kBQ];
+  __shared__ float thd[kBQ];
+  __shared__ uint32_t thi[kBQ];
+  __shared__ uint32_t cn[kBQ];
+  __shared__ float td[kBQ][kPK];
+  __shared__ uint32_t ti[kBQ][kPK];
+  __shared__ float cd[kBQ][kWave];
+  __shared__ uint32_t ci[kBQ][kWave];
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  if (t < kBQ) {
+    uint32_t row = (uint32_t)t < it.q_count ? a.qids[it.q_begin + t] : kDummy;
+    qrow[t] = row;
+    qnrm[t] = row != kDummy ? a.qn[row] : 0.f;
+    thd[t] = FLT_MAX;
+    thi[t] = kDummy;
+    cn[t] = 0;
+  }
+  for (int i = t; i < kBQ * kPK; i += 256) {
+    td[i / kPK][i % kPK] = FLT_MAX;
+    ti[i / kPK][i % kPK] = kDummy;
+  }
+  __syncthreads();
+  bf16x8 A[8][KS];
+  const uint32_t koff = (l / 16) * 8;
+#pragma unroll
+  for (int qb = 0; qb < 8; ++qb) {
+    uint32_t row = qrow[qb * 16 + (l % 16)];
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+      A[qb][s] = row != kDummy ? load8(a.qb + size_t(row) * a.ldb + s * 32 + koff) : bf16x8{};
+  }
+  const uint32_t x0 = it.x_begin, np = it.x_count;
+  const uint32_t pl = w * 16 + (l % 16);
+  bf16x8 Bn[KS];
+  {
+    const bool pv = pl < np;
+    const uint16_t *src = a.xb + size_t(x0 + (pv ? pl : 0)) * a.ldb + koff;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) Bn[s] = pv ? load8(src + s * 32) : bf16x8{};
+  }
+  for (uint32_t p0 = 0; p0 < np; p0 += kBP) {
+    const uint32_t p = p0 + pl;
+    const bool pv = p < np;
+    bf16x8 B[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) B[s] = Bn[s];
+    {
+      const uint32_t pn = p + kBP;
+      const bool nv = pn < np;
+      const uint16_t *src = a.xb + size_t(x0 + (nv ? pn : 0)) * a.ldb + koff;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) Bn[s] = nv ? load8(src + s * 32) : bf16x8{};
+    }
+    const float nx = pv ? a.xn[x0 + p] : 0.f;
+    const uint32_t id = pv ? a.perm[x0 + p] : kDummy;
+    f32x4 acc[8];
+#pragma unroll
+    for (int qb = 0; qb < 8; ++qb) acc[qb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+      for (int qb = 0; qb < 8; ++qb)
+        acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][s], B[s], acc[qb], 0, 0, 0);
+#pragma unroll
+    for (int qb = 0; qb < 8; ++qb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int qi = qb * 16 + (l / 16) * 4 + r;
+        float v = __builtin_fmaf(-2.f, acc[qb][r], nx) + qnrm[qi];
+        if (pv && pair_less(v, id, thd[qi], thi[qi])) {
+          uint32_t slot = atomicAdd(&cn[qi], 1u);
+          cd[qi][slot] = v;
+          ci[qi][slot] = id;
+        }
+      }
+    }
+    __syncthreads();
+    if (t < kBQ) {
+      const uint32_t n = cn[t];
+      if (n) {
+        float kd = thd[t];
+        uint32_t ki = thi[t];
+        for (uint32_t c = 0; c < n; ++c) {
+          const float cv = cd[t][c];
+          const uint32_t cid = ci[t][c];
+          if (!pair_less(cv, cid, kd, ki)) continue;
+          int pos = (int)a.Kp - 1;
+          while (pos > 0 && pair_less(cv, cid, td[t][pos - 1], ti[t][pos - 1])) {
+            td[t][pos] = td[t][pos - 1];
+            ti[t][pos] = ti[t][pos - 1];
+            --pos;
+          }
+          td[t][pos] = cv;
+          ti[t][pos] = cid;
+          kd = td[t][a.Kp - 1];
+          ki = ti[t][a.Kp - 1];
+        }
+        thd[t] = kd;
+        thi[t] = ki;
+        cn[t] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = t; i < kBQ * kPK; i += 256) {
+    const int qi = i / kPK, j = i % kPK;
+    const uint32_t row = qrow[qi];
+    if (row == kDummy || j >= (int)a.Kp) continue;
+    a.out_i[size_t(row) * a.Kp + j] = ti[qi][j];
+    if (j == (int)a.Kp - 1) a.out_kth[row] = td[qi][j];
   }
 }
 
@@ -212,8 +369,7 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
   a.lqe = h->list_qerr.get();
   a.lnm = h->list_nmax.get();
   a.ldb = h->ldb;
-  const double up = 0x1p-23, K = 32.0 * KS;
-  a.c_gemm = K * up / (1.0 - K * up) + 6.0 * up;
+  a.c_gemm = bf16_gemm_err(h);
   a.cand_d = cand_d;
   a.cand_i = cand_i;
   a.cand_n = cand_n;
@@ -230,6 +386,49 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
   }
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
+}
+
+}  // namespace rbc
+
+namespace rbc {
+
+int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t *qorder,
+               const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_kth,
+               hipStream_t st) {
+  if (nitems == 0) return RBC_OK;
+  if (Kp > (uint32_t)kPK) {
+    set_error("BF16 probe keeps at most %d candidates (K'=%u)", kPK, Kp);
+    return RBC_ERR_LIMIT;
+  }
+  Bf16ProbeArgs a;
+  a.items = items;
+  a.nitems = nitems;
+  a.qids = qorder;
+  a.qb = qb;
+  a.qn = qn;
+  a.xb = h->xb.get();
+  a.xn = h->xb_norm.get();
+  a.perm = h->perm.get();
+  a.ldb = h->ldb;
+  a.Kp = Kp;
+  a.out_i = out_i;
+  a.out_kth = out_kth;
+  switch (h->ldb / 32) {
+    case 1: hipLaunchKernelGGL(bf16_probe_kernel<1>, dim3(nitems), dim3(256), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(bf16_probe_kernel<2>, dim3(nitems), dim3(256), 0, st, a); break;
+    case 3: hipLaunchKernelGGL(bf16_probe_kernel<3>, dim3(nitems), dim3(256), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(bf16_probe_kernel<4>, dim3(nitems), dim3(256), 0, st, a); break;
+    default:
+      set_error("BF16 filter supports d <= 128 (d=%u)", h->d);
+      return RBC_ERR_LIMIT;
+  }
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+double bf16_gemm_err(const rbc_exact *h) {
+  const double up = 0x1p-23, K = (double)h->ldb;
+  return K * up / (1.0 - K * up) + 6.0 * up;
 }
 
 }  // namespace rbc

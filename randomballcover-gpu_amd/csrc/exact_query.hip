@@ -60,26 +60,84 @@ __global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uin
 }
 
 // ---- 3b. bounds from the probe ----------------------------------------------------
-// gamma_hi: upper bound on the true k-th NN distance value (squared for L2);
-// tau: candidate cut-off on the computed FP32 values; Gam: gamma_hi as a metric
-// distance (sqrt for L2) for the pruning tests; cert: whether the probe's K' kept every
-// point of the probed list that could pass tau.
-__global__ void probe_bounds(const float *probe_d, uint32_t Kp, uint32_t k, const float *rep_kth,
-                             const uint32_t *near, const uint32_t *list_len, uint32_t m, double e_rel,
-                             int metric, float *tau, double *gam, uint32_t *cert) {
+// gamma_hi: upper bound on the true k-th NN distance value (squared for L2), from the
+// k-th smallest FP32 value over k distinct points (the reps, or the probed lists' best);
+// tau: candidate cut-off on the computed FP32 values; gam: gamma_hi as a metric distance
+// (sqrt for L2) for the pruning tests; cert bit s: probe slot s kept every point of its
+// list that could pass tau (bit set also when the slot is unused).
+struct BoundArgs {
+  const float *pd0, *pd1;        // probe values per slot (ascending), [m][Kp]
+  const float *kth0, *kth1;      // BF16 probe: approximate D~ of the K'-th per slot
+  uint32_t Kp, k, m, R;
+  const float *rep_kth;          // k-th smallest FP32 rep distance (FLT_MAX if R < k)
+  const uint32_t *near, *key1, *list_len;
+  double e_probe, e_rep;         // relative error of the probe values and of rep_kth
+  int metric;
+  bool bf16;
+  const float *qn, *qe, *lqe, *lnm;  // BF16 error terms
+  double c_gemm;
+  float *tau;
+  double *gam;
+  uint32_t *cert;
+};
+
+__global__ void probe_bounds(BoundArgs b) {
   uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= m) return;
-  double g = (double)rep_kth[q];
-  double gp = (double)probe_d[size_t(q) * Kp + k - 1];
-  if (gp < g) g = gp;
-  double ghi = g >= (double)FLT_MAX ? INFINITY : g / (1.0 - e_rel) * (1.0 + 1e-12);
-  double t = ghi * (1.0 + e_rel) * (1.0 + 1e-12);
+  if (q >= b.m) return;
+  const float *a0 = b.pd0 + size_t(q) * b.Kp;
+  const bool two = b.pd1 != nullptr && b.key1[q] < b.R;
+  const float *a1 = two ? b.pd1 + size_t(q) * b.Kp : nullptr;
+  // k-th smallest of the union of the two ascending slot lists.
+  uint32_t i0 = 0, i1 = 0;
+  float kth = FLT_MAX;
+  for (uint32_t t = 0; t < b.k; ++t) {
+    float va = i0 < b.Kp ? a0[i0] : FLT_MAX;
+    float vb = (two && i1 < b.Kp) ? a1[i1] : FLT_MAX;
+    if (va <= vb) {
+      kth = va;
+      ++i0;
+    } else {
+      kth = vb;
+      ++i1;
+    }
+  }
+  double gp = kth >= FLT_MAX ? INFINITY : (double)kth / (1.0 - b.e_probe);
+  double gr = b.rep_kth[q] >= FLT_MAX ? INFINITY : (double)b.rep_kth[q] / (1.0 - b.e_rep);
+  double ghi = fmin(gp, gr) * (1.0 + 1e-12);
+  double t = ghi * (1.0 + b.e_rep) * (1.0 + 1e-12);
   float tf = t >= (double)FLT_MAX ? INFINITY : (float)t;
   if ((double)tf < t) tf = nextafterf(tf, INFINITY);
-  tau[q] = tf;
-  gam[q] = metric == 0 ? sqrt(ghi) * (1.0 + 1e-12) : ghi;
-  uint32_t len = list_len[near[q]];
-  cert[q] = (len <= Kp || probe_d[size_t(q) * Kp + Kp - 1] > tf) ? 1u : 0u;
+  b.tau[q] = tf;
+  const double G = b.metric == 0 ? sqrt(ghi) * (1.0 + 1e-12) : ghi;
+  b.gam[q] = G;
+  uint32_t c0, c1 = 1u;
+  if (!b.bf16) {
+    c0 = (b.list_len[b.near[q]] <= b.Kp || a0[b.Kp - 1] > tf) ? 1u : 0u;
+    if (two) c1 = (b.list_len[b.key1[q]] <= b.Kp || a1[b.Kp - 1] > tf) ? 1u : 0u;
+  } else {
+    // Every point of the list not kept has D~ >= the K'-th kept D~; its true distance is
+    // at least sqrt(D~ - E) - |q - q~| - max_list |x - x~|.
+    auto complete = [&](uint32_t list, float kthv) {
+      if (b.list_len[list] <= b.Kp) return 1u;
+      if (kthv >= FLT_MAX) return 0u;
+      double E = b.c_gemm * ((double)b.qn[q] + (double)b.lnm[list]) + 1e-30;
+      double lb = sqrt(fmax(0.0, (double)kthv - E)) - (double)b.qe[q] - (double)b.lqe[list];
+      return lb * (1.0 - 1e-12) > G ? 1u : 0u;
+    };
+    c0 = complete(b.near[q], b.kth0[q]);
+    if (two) c1 = complete(b.key1[q], b.kth1[q]);
+  }
+  b.cert[q] = c0 | (c1 << 1);
+}
+
+// Second-probe key: the proposal's list when it differs from the nearest rep's, else R
+// (no second probe).
+__global__ void second_probe_key(const uint32_t *prop, const uint32_t *near, uint32_t m, uint32_t R,
+                                 uint32_t *key1) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  uint32_t p = prop[q];
+  key1[q] = (p < R && p != near[q]) ? p : R;
 }
 
 // ---- 4. plan ------------------------------------------------------------------------
@@ -109,8 +167,9 @@ __device__ __forceinline__ double list_lower_bound(double Dqj, double Dq0, doubl
 constexpr uint32_t kPlanQ = 32;
 
 __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, uint32_t R,
-                                                  const uint32_t *near, const double *gam,
-                                                  const uint32_t *cert, const float *radius,
+                                                  const uint32_t *near, const uint32_t *key1,
+                                                  const double *gam, const uint32_t *cert,
+                                                  const float *radius,
                                                   const float *rr, double e, int metric,
                                                   uint32_t W, uint32_t *bits, uint32_t *list_cnt) {
   const uint32_t q0 = blockIdx.x * kPlanQ;
@@ -122,10 +181,14 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
     for (uint32_t qi = 0; qi < kPlanQ && q0 + qi < m; ++qi) {
       const uint32_t q = q0 + qi;
       const uint32_t j0 = near[q];
+      const uint32_t j1 = key1 ? key1[q] : R;
+      const uint32_t cb = cert[q];
       bool keep = false;
-      if (j < R && (j != j0 || cert[q] == 0)) {
+      if (j < R) {
         if (j == j0) {
-          keep = true;
+          keep = (cb & 1u) == 0;  // probed; rescan only if the probe was not complete
+        } else if (j == j1) {
+          keep = (cb & 2u) == 0;
         } else {
           double Dqj = (double)Dqr[size_t(q) * R + j];
           double Dq0 = (double)Dqr[size_t(q) * R + j0];
@@ -229,16 +292,20 @@ __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
   }
   float tv = FLT_MAX;
   uint32_t ti = kDummy;
-  const bool use_probe = a.cert == nullptr || a.cert[q] != 0;
-  const uint32_t np = use_probe ? a.Kp : 0;
+  const uint32_t cb = a.cert[q];
+  const uint32_t np0 = (cb & 1u) ? a.Kp : 0;
+  const bool two = a.probe_i1 != nullptr && a.key1[q] < a.R && (cb & 2u);
+  const uint32_t np = np0 + (two ? a.Kp : 0);
   const uint32_t nc = a.cand_n ? min(a.cand_n[q], a.cap) : 0;
   const uint32_t total = np + nc;
   uint32_t reranked = 0;
   for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
     uint32_t e = b0 + l;
     uint32_t myid = kDummy;
-    if (e < np)
+    if (e < np0)
       myid = a.probe_i[size_t(q) * a.Kp + e];
+    else if (e < np)
+      myid = a.probe_i1[size_t(q) * a.Kp + (e - np0)];
     else if (e < total)
       myid = a.cand_i[size_t(q) * a.cap + (e - np)];
     float myv = FLT_MAX;
@@ -283,6 +350,54 @@ __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
     a.out_d[size_t(q) * a.ldo + l] = tv;
     a.out_i[size_t(q) * a.ldo + l] = ti;
   }
+}
+
+// Exact FP32 (tree-order) distances of a probe's K' candidates, re-sorted by
+// (distance, id); one wave per query.
+template <int METRIC, int NPL>
+__global__ __launch_bounds__(256) void probe_rerank(const float *Q, size_t ldq, const float *xp, size_t ldx,
+                                                    const uint32_t *inv, uint32_t m, uint32_t d, uint32_t Kp,
+                                                    uint32_t *pi, float *pd) {
+  const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (q >= m) return;
+  const int l = lane_id();
+  const float *qrow = Q + size_t(q) * ldq;
+  float qv[NPL];
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    uint32_t c = l + kWave * j;
+    qv[j] = c < d ? qrow[c] : 0.f;
+  }
+  uint32_t myid = l < (int)Kp ? pi[size_t(q) * Kp + l] : kDummy;
+  float myv = FLT_MAX;
+  for (uint32_t cc = 0; cc < Kp; ++cc) {
+    uint32_t id = __shfl(myid, cc, kWave);
+    if (id == kDummy) continue;
+    float v = tree_dist_row<METRIC, NPL>(xp + size_t(inv[id]) * ldx, qv, d);
+    if (l == (int)cc) myv = v;
+  }
+  wave_bitonic_sort(myv, myid);
+  if (l < (int)Kp) {
+    pi[size_t(q) * Kp + l] = myid;
+    pd[size_t(q) * Kp + l] = myv;
+  }
+}
+
+int launch_probe_rerank(const float *Q, size_t ldq, const rbc_exact *h, uint32_t m, uint32_t Kp, uint32_t *pi,
+                        float *pd, hipStream_t st) {
+  uint32_t P = tree_width(h->d);
+  dim3 g(cdiv(m, 4)), b(256);
+  if (P <= 64)
+    hipLaunchKernelGGL((probe_rerank<0, 1>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
+                       h->d, Kp, pi, pd);
+  else if (P == 128)
+    hipLaunchKernelGGL((probe_rerank<0, 2>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
+                       h->d, Kp, pi, pd);
+  else
+    hipLaunchKernelGGL((probe_rerank<0, 4>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
+                       h->d, Kp, pi, pd);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
 }
 
 int launch_select(const SelectArgs &a, int metric, hipStream_t st) {
@@ -364,23 +479,72 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_HIP_TRY(hipGetLastError());
   if (R < k) RBC_TRY(fill_f32(rep_kth, FLT_MAX, m, st));
   tm.mark();
-  // 3. probe: group queries by nearest rep, scan that list.
-  RBC_TRY(ws.get(23, size_t(m) * 4, &p));
-  uint32_t *qorder = static_cast<uint32_t *>(p);
-  RBC_TRY(ws.get(24, (size_t(R) + 1) * 4, &p));
-  uint32_t *qoff = static_cast<uint32_t *>(p);
-  RBC_TRY(counting_sort(near, m, R, qorder, qoff, ws, 0, st));
+  // 3. probe: group queries by nearest rep and scan that list (FP32 tiles keeping the K'
+  // best, or BF16 MFMA tiles keeping the K' best approximate distances, re-ranked in FP32).
+  const bool bf16 = precision == RBC_PREC_BF16;
+  // The BF16 probe keeps at most 32 candidates, so it serves k <= 32.
+  const bool bprobe = bf16 && k <= 32;
+  const uint32_t KpUse = bprobe ? std::min<uint32_t>(32, k + 8) : Kp;
+  uint16_t *qb = nullptr;
+  float *qn = nullptr, *qe = nullptr;
+  if (bf16) {
+    RBC_TRY(ws.get(40, size_t(m) * h->ldb * 2, &p));
+    qb = static_cast<uint16_t *>(p);
+    RBC_TRY(ws.get(41, size_t(m) * 4, &p));
+    qn = static_cast<float *>(p);
+    RBC_TRY(ws.get(42, size_t(m) * 4, &p));
+    qe = static_cast<float *>(p);
+    RBC_TRY(bf16_prepare_queries(h, qd, ldq, m, qb, qn, qe, st));
+  }
   DevBuf<ScanItem> pitems;
   uint32_t npitems = 0;
-  RBC_TRY(make_bucket_items(qoff, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st));
-  RBC_TRY(ws.get(25, size_t(m) * Kp * 4, &p));
-  float *probe_d = static_cast<float *>(p);
-  RBC_TRY(ws.get(26, size_t(m) * Kp * 4, &p));
-  uint32_t *probe_i = static_cast<uint32_t *>(p);
-  {
-    TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, qorder, d, h->dpad};
-    EpiTopK epi{Kp, probe_d, probe_i, Kp, 0};
-    RBC_TRY(launch_tiles_m(h->metric, src, pitems.get(), npitems, epi, st));
+  // Runs one probe slot for the queries whose key (list) is < R.
+  auto run_probe = [&](const uint32_t *keys, size_t slot_order, size_t slot_off, size_t slot_d, size_t slot_i,
+                       size_t slot_kth, float **pd_out, uint32_t **pi_out, float **kth_out) -> int {
+    void *pp;
+    RBC_TRY(ws.get(slot_order, size_t(m) * 4, &pp));
+    uint32_t *order = static_cast<uint32_t *>(pp);
+    RBC_TRY(ws.get(slot_off, (size_t(R) + 1) * 4, &pp));
+    uint32_t *off = static_cast<uint32_t *>(pp);
+    RBC_TRY(counting_sort(keys, m, R, order, off, ws, 0, st));
+    RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st,
+                              bprobe ? 128u : (uint32_t)kTQ));
+    RBC_TRY(ws.get(slot_d, size_t(m) * KpUse * 4, &pp));
+    float *pd = static_cast<float *>(pp);
+    RBC_TRY(ws.get(slot_i, size_t(m) * KpUse * 4, &pp));
+    uint32_t *pi = static_cast<uint32_t *>(pp);
+    RBC_TRY(fill_f32(pd, FLT_MAX, size_t(m) * KpUse, st));
+    RBC_TRY(fill_u32(pi, kDummy, size_t(m) * KpUse, st));
+    if (bprobe) {
+      RBC_TRY(ws.get(slot_kth, size_t(m) * 4, &pp));
+      float *kth = static_cast<float *>(pp);
+      RBC_TRY(fill_f32(kth, FLT_MAX, m, st));
+      RBC_TRY(bf16_probe(h, qb, qn, order, pitems.get(), npitems, KpUse, pi, kth, st));
+      RBC_TRY(launch_probe_rerank(qd, ldq, h, m, KpUse, pi, pd, st));
+      *kth_out = kth;
+    } else {
+      TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, order, d, h->dpad};
+      EpiTopK epi{KpUse, pd, pi, KpUse, 0};
+      RBC_TRY(launch_tiles_m(h->metric, src, pitems.get(), npitems, epi, st));
+    }
+    *pd_out = pd;
+    *pi_out = pi;
+    return RBC_OK;
+  };
+  float *probe_d = nullptr, *probe_d1 = nullptr, *kth0 = nullptr, *kth1 = nullptr;
+  uint32_t *probe_i = nullptr, *probe_i1 = nullptr, *key1 = nullptr;
+  RBC_TRY(run_probe(near, 23, 24, 25, 26, 49, &probe_d, &probe_i, &kth0));
+  // 3a. second probe: the proposal network's top-1 list where it differs from the
+  // nearest rep's.
+  if (h->proposal) {
+    RBC_TRY(ws.get(43, size_t(m) * 4, &p));
+    uint32_t *prop = static_cast<uint32_t *>(p);
+    RBC_TRY(mlp_propose(h->proposal, qd, ldq, m, prop, st));
+    RBC_TRY(ws.get(44, size_t(m) * 4, &p));
+    key1 = static_cast<uint32_t *>(p);
+    hipLaunchKernelGGL(second_probe_key, dim3(cdiv(m, 256)), dim3(256), 0, st, prop, near, m, R, key1);
+    RBC_HIP_TRY(hipGetLastError());
+    RBC_TRY(run_probe(key1, 45, 46, 47, 48, 50, &probe_d1, &probe_i1, &kth1));
   }
   // 3b. bounds
   RBC_TRY(ws.get(27, size_t(m) * 4, &p));
@@ -389,9 +553,35 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   double *gam = static_cast<double *>(p);
   RBC_TRY(ws.get(29, size_t(m) * 4, &p));
   uint32_t *cert = static_cast<uint32_t *>(p);
-  hipLaunchKernelGGL(probe_bounds, dim3(cdiv(m, 256)), dim3(256), 0, st, probe_d, Kp, k, rep_kth, near,
-                     h->list_len.get(), m, e, h->metric, tau, gam, cert);
-  RBC_HIP_TRY(hipGetLastError());
+  {
+    BoundArgs ba{};
+    ba.pd0 = probe_d;
+    ba.pd1 = probe_d1;
+    ba.kth0 = kth0;
+    ba.kth1 = kth1;
+    ba.Kp = KpUse;
+    ba.k = k;
+    ba.m = m;
+    ba.R = R;
+    ba.rep_kth = rep_kth;
+    ba.near = near;
+    ba.key1 = key1;
+    ba.list_len = h->list_len.get();
+    ba.e_probe = bprobe ? tree_rel_err(tree_width(d)) : e;
+    ba.e_rep = e;
+    ba.metric = h->metric;
+    ba.bf16 = bprobe;
+    ba.qn = qn;
+    ba.qe = qe;
+    ba.lqe = h->list_qerr.get();
+    ba.lnm = h->list_nmax.get();
+    ba.c_gemm = bprobe ? bf16_gemm_err(h) : 0.0;
+    ba.tau = tau;
+    ba.gam = gam;
+    ba.cert = cert;
+    hipLaunchKernelGGL(probe_bounds, dim3(cdiv(m, 256)), dim3(256), 0, st, ba);
+    RBC_HIP_TRY(hipGetLastError());
+  }
   tm.mark();
   // 4. plan
   const uint32_t W = cdiv(R, 32);
@@ -403,7 +593,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint32_t *cursor = static_cast<uint32_t *>(p);
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
-  hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, Dqr, m, R, near, gam, cert,
+  hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, Dqr, m, R, near, key1, gam, cert,
                      h->radius.get(), h->rr.get(), e, h->metric, W, bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));
@@ -418,23 +608,11 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_HIP_TRY(hipGetLastError());
   DevBuf<ScanItem> sitems;
   uint32_t nsitems = 0;
-  const bool bf16 = precision == RBC_PREC_BF16;
   RBC_TRY(make_bucket_items(lcnt, h->list_off.get(), h->list_len.get(), R, sitems, &nsitems, ws, 16, st,
                             bf16 ? 128u : (uint32_t)kTQ));
-  uint16_t *qb = nullptr;
-  float *qn = nullptr, *qe = nullptr;
-  if (bf16) {
-    RBC_TRY(ws.get(40, size_t(m) * h->ldb * 2, &p));
-    qb = static_cast<uint16_t *>(p);
-    RBC_TRY(ws.get(41, size_t(m) * 4, &p));
-    qn = static_cast<float *>(p);
-    RBC_TRY(ws.get(42, size_t(m) * 4, &p));
-    qe = static_cast<float *>(p);
-    RBC_TRY(bf16_prepare_queries(h, qd, ldq, m, qb, qn, qe, st));
-  }
   tm.mark();
   // 5. scan with candidate emission; re-run with a larger buffer on overflow.
-  uint32_t cap = std::max<uint32_t>(256, 4 * k);
+  uint32_t cap = std::max<uint32_t>({256u, 4 * k, h->cand_cap});
   RBC_TRY(ws.get(36, size_t(m) * 4, &p));
   uint32_t *cand_n = static_cast<uint32_t *>(p);
   float *cand_d = nullptr;
@@ -462,6 +640,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
       mx = std::max(mx, v);
       if (v > cap) fallbacks += 1;
     }
+    h->cand_cap = std::max(h->cand_cap, mx + mx / 4);
     if (mx <= cap) break;
     if (attempt == 1) {
       set_error("candidate buffer overflow persisted (%u > %u)", mx, cap);
@@ -483,8 +662,11 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   sa.m = m;
   sa.d = d;
   sa.k = k;
-  sa.Kp = Kp;
+  sa.Kp = KpUse;
   sa.probe_i = probe_i;
+  sa.probe_i1 = probe_i1;
+  sa.key1 = key1;
+  sa.R = R;
   sa.cert = cert;
   sa.cand_i = cand_i;
   sa.cand_n = cand_n;
@@ -501,8 +683,15 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     unsigned long long hs[4];
     RBC_HIP_TRY(hipMemcpy(hs, sdev, sizeof(hs), hipMemcpyDeviceToHost));
     // Pairs evaluated by the probe and the scan, from the per-list query counts.
-    std::vector<uint32_t> hq(R + 1), hl(R + 1);
-    RBC_HIP_TRY(hipMemcpy(hq.data(), qoff, (R + 1) * 4, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> hq(R + 1), hl(R + 1), hq1(R + 1, 0);
+    void *pq0;
+    RBC_TRY(ws.get(24, (size_t(R) + 1) * 4, &pq0));
+    RBC_HIP_TRY(hipMemcpy(hq.data(), pq0, (R + 1) * 4, hipMemcpyDeviceToHost));
+    if (h->proposal) {
+      void *pq1;
+      RBC_TRY(ws.get(46, (size_t(R) + 1) * 4, &pq1));
+      RBC_HIP_TRY(hipMemcpy(hq1.data(), pq1, (R + 1) * 4, hipMemcpyDeviceToHost));
+    }
     RBC_HIP_TRY(hipMemcpy(hl.data(), lcnt, (R + 1) * 4, hipMemcpyDeviceToHost));
     double pp = 0, ps = 0, bytes = 0;
     // Algorithmic bytes of the scan: each item reads its list once and its queries once
@@ -512,7 +701,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     for (uint32_t j = 0; j < R; ++j) {
       const double len = h->h_list_off[j + 1] - h->h_list_off[j];
       const double cq = hl[j + 1] - hl[j];
-      pp += double(hq[j + 1] - hq[j]) * len;
+      pp += double(hq[j + 1] - hq[j]) * len + double(hq1[j + 1] - hq1[j]) * len;
       ps += cq * len;
       bytes += row_bytes * (std::ceil(cq / tile) * len + cq);
     }
@@ -537,6 +726,18 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
 }  // namespace rbc
 
 using namespace rbc;
+
+extern "C" int rbc_exact_set_proposal(rbc_exact *h, rbc_mlp *mlp) {
+  RBC_ARG_CHECK(h, "NULL handle");
+  if (mlp) {
+    uint32_t din, nout;
+    mlp_dims(mlp, &din, &nout);
+    RBC_ARG_CHECK(din == h->d && nout == h->R, "proposal network shape %u -> %u does not match d=%u, R=%u",
+                  din, nout, h->d, h->R);
+  }
+  h->proposal = mlp;
+  return RBC_OK;
+}
 
 extern "C" int rbc_exact_kquery(rbc_exact *h, const float *q, uint32_t m, uint32_t ldq, uint32_t k,
                                 int precision, uint32_t *nn_ids, float *nn_dists, uint32_t ldo, int mem,

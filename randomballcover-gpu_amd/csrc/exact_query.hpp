@@ -12,8 +12,11 @@ struct SelectArgs {
   size_t ldx;
   const uint32_t *inv;
   uint32_t m, d, k, Kp;
-  const uint32_t *probe_i;  // [m][Kp] probe candidates (original ids)
-  const uint32_t *cert;     // per query: probe candidates complete (else ignored)
+  const uint32_t *probe_i;   // [m][Kp] first-probe candidates (original ids)
+  const uint32_t *probe_i1;  // [m][Kp] second-probe candidates, or NULL
+  const uint32_t *key1;      // second-probe list per query (R = none)
+  uint32_t R;
+  const uint32_t *cert;      // per query: bit s = probe slot s complete (else ignored)
   const uint32_t *cand_i;   // [m][cap] scan candidates
   const uint32_t *cand_n;
   uint32_t cap;
@@ -29,6 +32,11 @@ int launch_select(const SelectArgs &a, int metric, hipStream_t st);
 // BF16 MFMA stages (exact_bf16.hip).
 int bf16_prepare_queries(rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint16_t *qb, float *qn,
                          float *qe, hipStream_t st);
+int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t *qorder,
+               const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_kth,
+               hipStream_t st);
+// Relative error constant of the BF16 GEMM distance: gamma_K + 6u', u' = 2^-23.
+double bf16_gemm_err(const rbc_exact *h);
 int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe, const double *gam,
               const uint32_t *qlist, const ScanItem *items, uint32_t nitems, float *cand_d,
               uint32_t *cand_i, uint32_t *cand_n, uint32_t cap, hipStream_t st);

@@ -1,0 +1,520 @@
+// Learned list proposal: a small MLP (d -> H1 -> H2 -> R, GELU, FP16 weights and
+// activations, FP32 accumulation) trained on the GPU with the engine's own kernels.
+//
+// It predicts which ownership list holds a query's nearest neighbour.  The exact
+// query uses its top-1 list as an extra first probe: a better probe tightens the
+// bound on the k-th NN distance, so the plan prunes more lists and the scan emits
+// fewer candidates.  It only changes which lists are probed first -- exactness is
+// untouched (pruning and the re-rank do not depend on it).
+//
+// Kernels: a tiled FP16 GEMM on v_mfma_f32_16x16x32_f16 (64x64 tiles, 4 waves of 32x32,
+// LDS-staged operands, generic strides so one kernel serves X W^T, dY^T X and dY W),
+// with bias and GELU fused in its epilogue; softmax cross-entropy; GELU backward;
+// column sums; Adam on FP32 master weights with FP16 copies.
+#include <cmath>
+#include <random>
+#include <vector>
+
+#include "exact.hpp"
+
+struct rbc_mlp;
+
+namespace rbc {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+struct GemmArgs {
+  const void *A;
+  int64_t a_rs, a_cs;  // A(i,k) = A[i*a_rs + k*a_cs]
+  const void *B;
+  int64_t b_rs, b_cs;  // B(k,j) = B[k*b_rs + j*b_cs]
+  float *C;            // FP32 result (pre-activation), C[i*ldc + j]; may be NULL
+  int64_t ldc;
+  _Float16 *H;         // FP16 copy of act(result), H[i*ldh + j]; may be NULL
+  int64_t ldh;
+  const float *bias;   // [N] or NULL
+  int M, N, K;
+  int gelu;            // apply GELU for H
+  float alpha;         // result = alpha * A B + bias (0 means 1)
+};
+
+constexpr int kGT = 64, kGK = 32, kLdsPad = 40;
+
+template <typename T>
+__device__ __forceinline__ _Float16 to_h(T v) {
+  return (_Float16)v;
+}
+
+__device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
+}
+
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
+  __shared__ _Float16 As[kGT][kLdsPad];
+  __shared__ _Float16 Bs[kGT][kLdsPad];  // Bs[j][k]
+  const int t = threadIdx.x, w = t / 64, l = t % 64;
+  const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
+  const TA *A = static_cast<const TA *>(g.A);
+  const TB *B = static_cast<const TB *>(g.B);
+  f32x4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int lr = t / 4, lk = (t % 4) * 8;
+  for (int k0 = 0; k0 < g.K; k0 += kGK) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      int i = i0 + lr, k = k0 + lk + e, j = j0 + lr;
+      As[lr][lk + e] = (i < g.M && k < g.K) ? to_h(A[i * g.a_rs + k * g.a_cs]) : (_Float16)0.f;
+      Bs[lr][lk + e] = (j < g.N && k < g.K) ? to_h(B[k * g.b_rs + j * g.b_cs]) : (_Float16)0.f;
+    }
+    __syncthreads();
+    const int rb = (w / 2) * 32, cb = (w % 2) * 32;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+      f16x8 av = *reinterpret_cast<const f16x8 *>(&As[rb + bi * 16 + l % 16][(l / 16) * 8]);
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        f16x8 bv = *reinterpret_cast<const f16x8 *>(&Bs[cb + bj * 16 + l % 16][(l / 16) * 8]);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[bi][bj], 0, 0, 0);
+      }
+    }
+  }
+  const int rb = (w / 2) * 32, cb = (w % 2) * 32;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int i = i0 + rb + bi * 16 + (l / 16) * 4 + r, j = j0 + cb + bj * 16 + l % 16;
+        if (i >= g.M || j >= g.N) continue;
+        float v = (g.alpha != 0.f ? g.alpha : 1.f) * acc[bi][bj][r] + (g.bias ? g.bias[j] : 0.f);
+        if (g.C) g.C[i * g.ldc + j] = v;
+        if (g.H) g.H[i * g.ldh + j] = (_Float16)(g.gelu ? gelu_f(v) : v);
+      }
+}
+
+template <typename TA, typename TB>
+int gemm(const GemmArgs &g, hipStream_t st) {
+  dim3 grid(cdiv(g.N, kGT), cdiv(g.M, kGT));
+  hipLaunchKernelGGL((gemm_f16<TA, TB>), grid, dim3(256), 0, st, g);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+// Softmax cross-entropy over rows of logits [B][N]: dZ = softmax - onehot as FP16 (the
+// 1/B of the mean is applied to the weight gradients in FP32, so that FP16 does not have
+// to hold values of order 1/(B N)); the mean loss accumulates into *loss.  One wave per
+// row.
+__global__ __launch_bounds__(256) void softmax_ce(const float *Z, const uint32_t *labels, int B, int N,
+                                                  _Float16 *dZ, float *loss) {
+  const int row = blockIdx.x * 4 + threadIdx.x / 64, l = threadIdx.x % 64;
+  if (row >= B) return;
+  const float *z = Z + int64_t(row) * N;
+  float mx = -INFINITY;
+  for (int j = l; j < N; j += 64) mx = fmaxf(mx, z[j]);
+  for (int s = 32; s > 0; s >>= 1) mx = fmaxf(mx, __shfl_xor(mx, s, 64));
+  float se = 0.f;
+  for (int j = l; j < N; j += 64) se += __expf(z[j] - mx);
+  for (int s = 32; s > 0; s >>= 1) se += __shfl_xor(se, s, 64);
+  const uint32_t lab = labels[row];
+  const float inv = 1.f / (float)B;
+  for (int j = l; j < N; j += 64) {
+    float p = __expf(z[j] - mx) / se;
+    dZ[int64_t(row) * N + j] = (_Float16)(p - (j == (int)lab ? 1.f : 0.f));
+  }
+  if (l == 0 && loss) atomicAdd(loss, (logf(se) + mx - z[lab]) * inv);
+}
+
+// dZ = dH * gelu'(Z), FP16 out.
+__global__ void gelu_backward(const float *dH, const float *Z, int64_t n, _Float16 *dZ) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dZ[i] = (_Float16)(dH[i] * gelu_grad(Z[i]));
+}
+
+// db[j] = scale * sum_i dZ[i][j].
+__global__ void col_sums(const _Float16 *dZ, int B, int N, float scale, float *db) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  float s = 0.f;
+  for (int i = 0; i < B; ++i) s += (float)dZ[int64_t(i) * N + j];
+  db[j] = scale * s;
+}
+
+__global__ void adam_step(float *w, _Float16 *w16, const float *g, float *m1, float *m2, int64_t n, float lr,
+                          float b1, float b2, float eps, float c1, float c2) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float a = b1 * m1[i] + (1.f - b1) * gi;
+  float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  float v = w[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  w[i] = v;
+  if (w16) w16[i] = (_Float16)v;
+}
+
+__global__ void to_half(const float *x, int64_t n, _Float16 *y) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (_Float16)x[i];
+}
+
+// Gathers rows idx[0..B) of X (FP32, ldx) into an FP16 [B][d] batch.
+__global__ void gather_half(const float *X, int64_t ldx, const uint32_t *idx, int B, int d, _Float16 *out) {
+  int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= int64_t(B) * d) return;
+  int r = int(e / d), c = int(e % d);
+  out[e] = (_Float16)X[int64_t(idx[r]) * ldx + c];
+}
+
+__global__ void gather_u32(const uint32_t *src, const uint32_t *idx, int B, uint32_t *out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < B) out[i] = src[idx[i]];
+}
+
+// Row argmax (lowest index on ties), one wave per row.
+__global__ __launch_bounds__(256) void row_argmax(const float *Z, int B, int N, uint32_t *out) {
+  const int row = blockIdx.x * 4 + threadIdx.x / 64, l = threadIdx.x % 64;
+  if (row >= B) return;
+  float bv = -INFINITY;
+  uint32_t bi = 0xFFFFFFFFu;
+  for (int j = l; j < N; j += 64) {
+    float v = Z[int64_t(row) * N + j];
+    if (v > bv) {
+      bv = v;
+      bi = j;
+    }
+  }
+  for (int s = 32; s > 0; s >>= 1) {
+    float ov = __shfl_xor(bv, s, 64);
+    uint32_t oi = __shfl_xor(bi, s, 64);
+    if (ov > bv || (ov == bv && oi < bi)) {
+      bv = ov;
+      bi = oi;
+    }
+  }
+  if (l == 0) out[row] = bi;
+}
+
+}  // namespace
+
+}  // namespace rbc
+
+// Parameters: W1 [H1][D], b1 [H1], W2 [H2][H1], b2 [H2], W3 [C][H2], b3 [C] in one FP32
+// buffer (this order) with an FP16 copy.
+struct rbc_mlp {
+  uint32_t D = 0, H1 = 0, H2 = 0, C = 0;
+  size_t np = 0;
+  size_t off[7];
+  rbc::DevBuf<float> w, g, m1, m2;
+  rbc::DevBuf<_Float16> w16;
+  uint64_t step = 0;
+  // Activations for a batch of up to `cap` rows.
+  uint32_t cap = 0;
+  rbc::DevBuf<_Float16> x16, h1, h2, dz3, dz2, dz1;
+  rbc::DevBuf<float> z1, z2, z3, dh2, dh1, loss;
+  rbc::DevBuf<uint32_t> lab, idx;
+
+  int ensure(uint32_t B) {
+    if (B <= cap) return RBC_OK;
+    cap = B;
+    RBC_TRY(x16.alloc(size_t(B) * D));
+    RBC_TRY(h1.alloc(size_t(B) * H1));
+    RBC_TRY(h2.alloc(size_t(B) * H2));
+    RBC_TRY(dz3.alloc(size_t(B) * C));
+    RBC_TRY(dz2.alloc(size_t(B) * H2));
+    RBC_TRY(dz1.alloc(size_t(B) * H1));
+    RBC_TRY(z1.alloc(size_t(B) * H1));
+    RBC_TRY(z2.alloc(size_t(B) * H2));
+    RBC_TRY(z3.alloc(size_t(B) * C));
+    RBC_TRY(dh2.alloc(size_t(B) * H2));
+    RBC_TRY(dh1.alloc(size_t(B) * H1));
+    RBC_TRY(lab.alloc(B));
+    RBC_TRY(idx.alloc(B));
+    RBC_TRY(loss.alloc(1));
+    return RBC_OK;
+  }
+  float *W(int i) { return w.get() + off[2 * i]; }
+  float *Bv(int i) { return w.get() + off[2 * i + 1]; }
+  _Float16 *W16(int i) { return w16.get() + off[2 * i]; }
+  float *GW(int i) { return g.get() + off[2 * i]; }
+  float *GB(int i) { return g.get() + off[2 * i + 1]; }
+};
+
+namespace rbc {
+
+namespace {
+
+// Forward pass on the FP16 batch in m->x16 (B rows); logits to m->z3 (or `logits`).
+int mlp_forward(rbc_mlp *m, uint32_t B, float *logits, hipStream_t st) {
+  GemmArgs g{};
+  g.A = m->x16.get(); g.a_rs = m->D; g.a_cs = 1;
+  g.B = m->W16(0); g.b_rs = 1; g.b_cs = m->D;
+  g.C = m->z1.get(); g.ldc = m->H1; g.H = m->h1.get(); g.ldh = m->H1; g.bias = m->Bv(0);
+  g.M = B; g.N = m->H1; g.K = m->D; g.gelu = 1;
+  RBC_TRY((gemm<_Float16, _Float16>(g, st)));
+  g = GemmArgs{};
+  g.A = m->h1.get(); g.a_rs = m->H1; g.a_cs = 1;
+  g.B = m->W16(1); g.b_rs = 1; g.b_cs = m->H1;
+  g.C = m->z2.get(); g.ldc = m->H2; g.H = m->h2.get(); g.ldh = m->H2; g.bias = m->Bv(1);
+  g.M = B; g.N = m->H2; g.K = m->H1; g.gelu = 1;
+  RBC_TRY((gemm<_Float16, _Float16>(g, st)));
+  g = GemmArgs{};
+  g.A = m->h2.get(); g.a_rs = m->H2; g.a_cs = 1;
+  g.B = m->W16(2); g.b_rs = 1; g.b_cs = m->H2;
+  g.C = logits ? logits : m->z3.get(); g.ldc = m->C; g.bias = m->Bv(2);
+  g.M = B; g.N = m->C; g.K = m->H2; g.gelu = 0;
+  RBC_TRY((gemm<_Float16, _Float16>(g, st)));
+  return RBC_OK;
+}
+
+// Backward pass for the batch (labels in m->lab); gradients to m->g.
+int mlp_backward(rbc_mlp *m, uint32_t B, hipStream_t st) {
+  RBC_TRY(fill_f32(m->loss.get(), 0.f, 1, st));
+  hipLaunchKernelGGL(softmax_ce, dim3(cdiv(B, 4)), dim3(256), 0, st, m->z3.get(), m->lab.get(), (int)B,
+                     (int)m->C, m->dz3.get(), m->loss.get());
+  RBC_HIP_TRY(hipGetLastError());
+  const _Float16 *dz[3] = {m->dz1.get(), m->dz2.get(), m->dz3.get()};
+  const _Float16 *in[3] = {m->x16.get(), m->h1.get(), m->h2.get()};
+  const uint32_t outd[3] = {m->H1, m->H2, m->C}, ind[3] = {m->D, m->H1, m->H2};
+  float *dh[3] = {nullptr, m->dh1.get(), m->dh2.get()};
+  const float *zs[3] = {nullptr, m->z1.get(), m->z2.get()};
+  _Float16 *dzo[3] = {nullptr, m->dz1.get(), m->dz2.get()};
+  for (int layer = 2; layer >= 0; --layer) {
+    // dW = dZ^T X  ([out][in]), db = column sums of dZ.
+    GemmArgs g{};
+    g.A = dz[layer]; g.a_rs = 1; g.a_cs = outd[layer];
+    g.B = in[layer]; g.b_rs = ind[layer]; g.b_cs = 1;
+    g.C = m->GW(layer); g.ldc = ind[layer];
+    g.M = outd[layer]; g.N = ind[layer]; g.K = B;
+    g.alpha = 1.f / (float)B;
+    RBC_TRY((gemm<_Float16, _Float16>(g, st)));
+    hipLaunchKernelGGL(col_sums, dim3(cdiv(outd[layer], 256)), dim3(256), 0, st, dz[layer], (int)B,
+                       (int)outd[layer], 1.f / (float)B, m->GB(layer));
+    RBC_HIP_TRY(hipGetLastError());
+    if (layer == 0) break;
+    // dH_{layer} = dZ W ([B][in]) (per-row gradients, without the 1/B), then
+    // dZ_{layer-1} = dH * gelu'(Z_{layer}).
+    g = GemmArgs{};
+    g.A = dz[layer]; g.a_rs = outd[layer]; g.a_cs = 1;
+    g.B = m->W16(layer); g.b_rs = ind[layer]; g.b_cs = 1;
+    g.C = dh[layer]; g.ldc = ind[layer];
+    g.M = B; g.N = ind[layer]; g.K = outd[layer];
+    RBC_TRY((gemm<_Float16, _Float16>(g, st)));
+    int64_t n = int64_t(B) * ind[layer];
+    hipLaunchKernelGGL(gelu_backward, dim3((uint32_t)cdivz(n, 256)), dim3(256), 0, st, dh[layer], zs[layer],
+                       n, dzo[layer]);
+    RBC_HIP_TRY(hipGetLastError());
+  }
+  return RBC_OK;
+}
+
+}  // namespace
+
+// Logits -> top-1 list per row.
+int mlp_propose(rbc_mlp *m, const float *q, size_t ldq, uint32_t rows, uint32_t *out, hipStream_t st) {
+  const uint32_t chunk = 16384;
+  RBC_TRY(m->ensure(std::min(rows, chunk)));
+  DevBuf<uint32_t> iota;
+  RBC_TRY(iota.alloc(chunk));
+  std::vector<uint32_t> hi(chunk);
+  for (uint32_t i = 0; i < chunk; ++i) hi[i] = i;
+  RBC_HIP_TRY(hipMemcpyAsync(iota.get(), hi.data(), chunk * 4, hipMemcpyHostToDevice, st));
+  for (uint32_t r0 = 0; r0 < rows; r0 += chunk) {
+    uint32_t B = std::min(chunk, rows - r0);
+    int64_t ne = int64_t(B) * m->D;
+    hipLaunchKernelGGL(gather_half, dim3((uint32_t)cdivz(ne, 256)), dim3(256), 0, st, q + size_t(r0) * ldq,
+                       (int64_t)ldq, iota.get(), (int)B, (int)m->D, m->x16.get());
+    RBC_HIP_TRY(hipGetLastError());
+    RBC_TRY(mlp_forward(m, B, nullptr, st));
+    hipLaunchKernelGGL(row_argmax, dim3(cdiv(B, 4)), dim3(256), 0, st, m->z3.get(), (int)B, (int)m->C, out + r0);
+    RBC_HIP_TRY(hipGetLastError());
+  }
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  return RBC_OK;
+}
+
+int mlp_dims(const rbc_mlp *m, uint32_t *d_in, uint32_t *n_out) {
+  *d_in = m->D;
+  *n_out = m->C;
+  return RBC_OK;
+}
+
+}  // namespace rbc
+
+using namespace rbc;
+
+extern "C" {
+
+int rbc_mlp_create(uint32_t d_in, uint32_t h1, uint32_t h2, uint32_t n_out, uint64_t seed, rbc_mlp **out) {
+  RBC_ARG_CHECK(out, "NULL output");
+  RBC_ARG_CHECK(d_in && h1 && h2 && n_out, "zero layer width");
+  auto m = new rbc_mlp();
+  m->D = d_in;
+  m->H1 = h1;
+  m->H2 = h2;
+  m->C = n_out;
+  size_t sz[6] = {size_t(h1) * d_in, h1, size_t(h2) * h1, h2, size_t(n_out) * h2, n_out};
+  size_t o = 0;
+  for (int i = 0; i < 6; ++i) {
+    m->off[i] = o;
+    o += sz[i];
+  }
+  m->off[6] = o;
+  m->np = o;
+  std::vector<float> hw(o, 0.f);
+  std::mt19937_64 rng(seed);
+  const uint32_t fan_in[3] = {d_in, h1, h2};
+  for (int layer = 0; layer < 3; ++layer) {
+    float a = 1.0f / std::sqrt((float)fan_in[layer]);
+    std::uniform_real_distribution<float> U(-a, a);
+    for (size_t i = m->off[2 * layer]; i < m->off[2 * layer + 1]; ++i) hw[i] = U(rng);
+  }
+  int st;
+  if ((st = m->w.alloc(o)) || (st = m->g.alloc(o)) || (st = m->m1.alloc(o)) || (st = m->m2.alloc(o)) ||
+      (st = m->w16.alloc(o))) {
+    delete m;
+    return st;
+  }
+  if (hipMemcpy(m->w.get(), hw.data(), o * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    delete m;
+    set_error("parameter upload failed");
+    return RBC_ERR_HIP;
+  }
+  (void)fill_f32(m->m1.get(), 0.f, o, nullptr);
+  (void)fill_f32(m->m2.get(), 0.f, o, nullptr);
+  hipLaunchKernelGGL(to_half, dim3((uint32_t)cdivz(o, 256)), dim3(256), 0, nullptr, m->w.get(), (int64_t)o,
+                     m->w16.get());
+  if (hipDeviceSynchronize() != hipSuccess) {
+    delete m;
+    set_error("MLP init failed");
+    return RBC_ERR_HIP;
+  }
+  *out = m;
+  return RBC_OK;
+}
+
+int rbc_mlp_destroy(rbc_mlp *m) {
+  delete m;
+  return RBC_OK;
+}
+
+int rbc_mlp_num_params(const rbc_mlp *m, size_t *count) {
+  RBC_ARG_CHECK(m && count, "NULL argument");
+  *count = m->np;
+  return RBC_OK;
+}
+
+int rbc_mlp_get_params(const rbc_mlp *m, float *params) {
+  RBC_ARG_CHECK(m && params, "NULL argument");
+  RBC_HIP_TRY(hipMemcpy(params, m->w.get(), m->np * 4, hipMemcpyDeviceToHost));
+  return RBC_OK;
+}
+
+int rbc_mlp_set_params(rbc_mlp *m, const float *params) {
+  RBC_ARG_CHECK(m && params, "NULL argument");
+  RBC_HIP_TRY(hipMemcpy(m->w.get(), params, m->np * 4, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(to_half, dim3((uint32_t)cdivz(m->np, 256)), dim3(256), 0, nullptr, m->w.get(),
+                     (int64_t)m->np, m->w16.get());
+  RBC_HIP_TRY(hipDeviceSynchronize());
+  return RBC_OK;
+}
+
+int rbc_mlp_loss_grad(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels,
+                      float *loss, float *grads) {
+  RBC_ARG_CHECK(m && x && labels && rows, "bad argument");
+  RBC_ARG_CHECK(ldx >= m->D, "ldx < input width");
+  hipStream_t st = nullptr;
+  RBC_TRY(m->ensure(rows));
+  DevBuf<float> xd;
+  RBC_TRY(xd.alloc(size_t(rows) * ldx));
+  RBC_HIP_TRY(hipMemcpy(xd.get(), x, size_t(rows) * ldx * 4, hipMemcpyHostToDevice));
+  RBC_HIP_TRY(hipMemcpy(m->lab.get(), labels, size_t(rows) * 4, hipMemcpyHostToDevice));
+  std::vector<uint32_t> hi(rows);
+  for (uint32_t i = 0; i < rows; ++i) hi[i] = i;
+  RBC_HIP_TRY(hipMemcpy(m->idx.get(), hi.data(), size_t(rows) * 4, hipMemcpyHostToDevice));
+  int64_t ne = int64_t(rows) * m->D;
+  hipLaunchKernelGGL(gather_half, dim3((uint32_t)cdivz(ne, 256)), dim3(256), 0, st, xd.get(), (int64_t)ldx,
+                     m->idx.get(), (int)rows, (int)m->D, m->x16.get());
+  RBC_TRY(mlp_forward(m, rows, nullptr, st));
+  RBC_TRY(mlp_backward(m, rows, st));
+  RBC_HIP_TRY(hipDeviceSynchronize());
+  if (loss) RBC_HIP_TRY(hipMemcpy(loss, m->loss.get(), 4, hipMemcpyDeviceToHost));
+  if (grads) RBC_HIP_TRY(hipMemcpy(grads, m->g.get(), m->np * 4, hipMemcpyDeviceToHost));
+  return RBC_OK;
+}
+
+int rbc_mlp_logits(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, float *logits) {
+  RBC_ARG_CHECK(m && x && logits && rows, "bad argument");
+  RBC_TRY(m->ensure(rows));
+  DevBuf<float> xd, zd;
+  RBC_TRY(xd.alloc(size_t(rows) * ldx));
+  RBC_TRY(zd.alloc(size_t(rows) * m->C));
+  RBC_HIP_TRY(hipMemcpy(xd.get(), x, size_t(rows) * ldx * 4, hipMemcpyHostToDevice));
+  std::vector<uint32_t> hi(rows);
+  for (uint32_t i = 0; i < rows; ++i) hi[i] = i;
+  RBC_HIP_TRY(hipMemcpy(m->idx.get(), hi.data(), size_t(rows) * 4, hipMemcpyHostToDevice));
+  int64_t ne = int64_t(rows) * m->D;
+  hipLaunchKernelGGL(gather_half, dim3((uint32_t)cdivz(ne, 256)), dim3(256), 0, nullptr, xd.get(), (int64_t)ldx,
+                     m->idx.get(), (int)rows, (int)m->D, m->x16.get());
+  RBC_TRY(mlp_forward(m, rows, zd.get(), nullptr));
+  RBC_HIP_TRY(hipDeviceSynchronize());
+  RBC_HIP_TRY(hipMemcpy(logits, zd.get(), size_t(rows) * m->C * 4, hipMemcpyDeviceToHost));
+  return RBC_OK;
+}
+
+int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels, int mem,
+                  uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *epoch_loss) {
+  RBC_ARG_CHECK(m && x && labels && rows && batch, "bad argument");
+  RBC_ARG_CHECK(ldx >= m->D, "ldx < input width");
+  hipStream_t st = nullptr;
+  RBC_TRY(m->ensure(batch));
+  DevMat X;
+  RBC_TRY(X.stage(x, rows, m->D, ldx, mem, st));
+  DevBuf<uint32_t> labd, perm;
+  RBC_TRY(labd.alloc(rows));
+  RBC_TRY(perm.alloc(rows));
+  RBC_TRY(copy_in(labd.get(), labels, size_t(rows) * 4, mem, st));
+  std::vector<uint32_t> hp(rows);
+  for (uint32_t i = 0; i < rows; ++i) hp[i] = i;
+  std::mt19937_64 rng(seed);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (uint32_t ep = 0; ep < epochs; ++ep) {
+    std::shuffle(hp.begin(), hp.end(), rng);
+    RBC_HIP_TRY(hipMemcpyAsync(perm.get(), hp.data(), size_t(rows) * 4, hipMemcpyHostToDevice, st));
+    uint32_t nb = 0;
+    for (uint32_t r0 = 0; r0 + batch <= rows; r0 += batch, ++nb) {
+      const uint32_t B = batch;
+      int64_t ne = int64_t(B) * m->D;
+      hipLaunchKernelGGL(gather_half, dim3((uint32_t)cdivz(ne, 256)), dim3(256), 0, st, X.p, (int64_t)X.ld,
+                         perm.get() + r0, (int)B, (int)m->D, m->x16.get());
+      hipLaunchKernelGGL(gather_u32, dim3(cdiv(B, 256)), dim3(256), 0, st, labd.get(), perm.get() + r0, (int)B,
+                         m->lab.get());
+      RBC_HIP_TRY(hipGetLastError());
+      RBC_TRY(mlp_forward(m, B, nullptr, st));
+      RBC_TRY(mlp_backward(m, B, st));
+      ++m->step;
+      float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+      hipLaunchKernelGGL(adam_step, dim3((uint32_t)cdivz(m->np, 256)), dim3(256), 0, st, m->w.get(), m->w16.get(),
+                         m->g.get(), m->m1.get(), m->m2.get(), (int64_t)m->np, lr, b1, b2, eps, c1, c2);
+      RBC_HIP_TRY(hipGetLastError());
+    }
+    if (epoch_loss) {
+      RBC_HIP_TRY(hipStreamSynchronize(st));
+      float l = 0.f;
+      RBC_HIP_TRY(hipMemcpy(&l, m->loss.get(), 4, hipMemcpyDeviceToHost));
+      epoch_loss[ep] = nb ? l : 0.f;  // loss of the epoch's last batch
+    }
+  }
+  RBC_HIP_TRY(hipDeviceSynchronize());
+  return RBC_OK;
+}
+
+}  // extern "C"

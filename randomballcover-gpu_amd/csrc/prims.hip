@@ -133,10 +133,10 @@ __global__ __launch_bounds__(kSortNT) void sort_scatter(const uint32_t *key, siz
 }
 
 __global__ void gather_offsets(const uint32_t *hist_off, uint32_t nb_buckets, uint32_t nblocks,
-                               uint32_t *offsets, size_t n) {
+                               uint32_t *offsets) {
   uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nb_buckets) offsets[b] = hist_off[size_t(b) * nblocks];
-  if (b == nb_buckets) offsets[b] = (uint32_t)n;
+  if (b == nb_buckets) offsets[b] = hist_off[size_t(nb_buckets) * nblocks];  // items with a valid key
 }
 
 // ---- items ----
@@ -208,7 +208,7 @@ int counting_sort(const uint32_t *key, size_t n, uint32_t nbuckets, uint32_t *or
                      order);
   RBC_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(gather_offsets, dim3(cdiv(nbuckets + 1, 256)), dim3(256), 0, st, hist, nbuckets,
-                     nblocks, offsets, n);
+                     nblocks, offsets);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }

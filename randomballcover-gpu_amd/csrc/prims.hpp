@@ -35,7 +35,8 @@ int exclusive_scan_u32(const uint32_t *in, uint32_t *out, size_t n, Workspace &w
 
 // Stable counting sort of items 0..n-1 by key[i] in [0, nbuckets): order[pos] = i with
 // items of a bucket in ascending i; offsets[b] = first position of bucket b
-// (nbuckets+1 entries).  Uses workspace slots [slot, slot+6).
+// (nbuckets+1 entries; offsets[nbuckets] = number of items with a valid key -- items
+// whose key is >= nbuckets are left out).  Uses workspace slots [slot, slot+6).
 int counting_sort(const uint32_t *key, size_t n, uint32_t nbuckets, uint32_t *order,
                   uint32_t *offsets, Workspace &ws, size_t slot, hipStream_t st);
 

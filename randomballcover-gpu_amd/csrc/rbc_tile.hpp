@@ -143,8 +143,8 @@ __device__ void tile_run(const TileSrc &src, const ScanItem &it, Epi &epi, float
 // ---------------------------------------------------------------- epilogues ----
 
 // Keeps the K (<= 64) smallest (distance, id) pairs of every query of the tile in LDS;
-// candidates under the running threshold are staged and folded in with a wave-level
-// bitonic merge.  At the end the lists are written to out[(aux*slot_stride + qrow)*ldo].
+// candidates under the running threshold are staged per step and folded in by one
+// thread per query (insertion into the sorted list).  At the end the lists are written to out[(aux*slot_stride + qrow)*ldo].
 struct EpiTopK {
   uint32_t K;
   float *out_d;
@@ -205,25 +205,34 @@ struct EpiTopK {
       }
     }
     __syncthreads();
-    const int w = threadIdx.x / kWave, ln = lane_id();
-    for (int qi = w; qi < kTQ; qi += kNT / kWave) {
-      uint32_t n = l.cnt[qi];
-      if (n == 0) continue;
-      float cv = ln < (int)n ? l.cd[qi * 64 + ln] : FLT_MAX;
-      uint32_t cid = ln < (int)n ? l.ci[qi * 64 + ln] : kDummy;
-      wave_bitonic_sort(cv, cid);
-      float tv = l.td[qi * 64 + ln];
-      uint32_t tid = l.ti[qi * 64 + ln];
-      wave_merge_lower(tv, tid, cv, cid);
-      if (ln < (int)K) {
-        l.td[qi * 64 + ln] = tv;
-        l.ti[qi * 64 + ln] = tid;
+    // One thread per query folds its staged candidates into the sorted list by insertion.
+    if (threadIdx.x < kTQ) {
+      const int qi = threadIdx.x;
+      const uint32_t n = l.cnt[qi];
+      if (n) {
+        float kd = l.thd[qi];
+        uint32_t ki = l.thi[qi];
+        float *td = l.td + qi * 64;
+        uint32_t *ti = l.ti + qi * 64;
+        for (uint32_t c = 0; c < n; ++c) {
+          const float cv = l.cd[qi * 64 + c];
+          const uint32_t cid = l.ci[qi * 64 + c];
+          if (!pair_less(cv, cid, kd, ki)) continue;
+          int pos = (int)K - 1;
+          while (pos > 0 && pair_less(cv, cid, td[pos - 1], ti[pos - 1])) {
+            td[pos] = td[pos - 1];
+            ti[pos] = ti[pos - 1];
+            --pos;
+          }
+          td[pos] = cv;
+          ti[pos] = cid;
+          kd = td[K - 1];
+          ki = ti[K - 1];
+        }
+        l.thd[qi] = kd;
+        l.thi[qi] = ki;
+        l.cnt[qi] = 0;
       }
-      if (ln == (int)K - 1) {
-        l.thd[qi] = tv;
-        l.thi[qi] = tid;
-      }
-      if (ln == 0) l.cnt[qi] = 0;
     }
     __syncthreads();
   }

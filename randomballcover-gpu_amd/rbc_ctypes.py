@@ -88,6 +88,15 @@ _SIGS = {
     "rbc_exact_get_info": (_int, [_vp, ctypes.POINTER(ExactInfo)]),
     "rbc_exact_get_lists": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rbc_exact_destroy": (_int, [_vp]),
+    "rbc_exact_set_proposal": (_int, [_vp, _vp]),
+    "rbc_mlp_create": (_int, [_u32, _u32, _u32, _u32, _u64, ctypes.POINTER(_vp)]),
+    "rbc_mlp_destroy": (_int, [_vp]),
+    "rbc_mlp_num_params": (_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "rbc_mlp_get_params": (_int, [_vp, _vp]),
+    "rbc_mlp_set_params": (_int, [_vp, _vp]),
+    "rbc_mlp_loss_grad": (_int, [_vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float), _vp]),
+    "rbc_mlp_logits": (_int, [_vp, _vp, _u32, _u32, _vp]),
+    "rbc_mlp_train": (_int, [_vp, _vp, _u32, _u32, _vp, _int, _u32, _u32, ctypes.c_float, _u64, _vp]),
 }
 
 
@@ -260,6 +269,10 @@ class ExactRBC:
                                       ctypes.byref(st)))
         return st.as_dict()
 
+    def set_proposal(self, mlp):
+        _check(lib().rbc_exact_set_proposal(self.h, mlp.h if mlp is not None else None))
+        self._proposal = mlp
+
     def info(self):
         inf = ExactInfo()
         _check(lib().rbc_exact_get_info(self.h, ctypes.byref(inf)))
@@ -279,6 +292,63 @@ class ExactRBC:
     def close(self):
         if self.h:
             _check(lib().rbc_exact_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ proposal network ----
+
+class ProposalMLP:
+    """rbc_mlp_*: d_in -> h1 -> h2 -> n_out, GELU, FP16 weights, trained on the GPU."""
+
+    def __init__(self, d_in, n_out, h1=256, h2=256, seed=0):
+        self.h = _vp()
+        _check(lib().rbc_mlp_create(d_in, h1, h2, n_out, seed, ctypes.byref(self.h)))
+        self.dims = (d_in, h1, h2, n_out)
+        n = ctypes.c_size_t()
+        _check(lib().rbc_mlp_num_params(self.h, ctypes.byref(n)))
+        self.num_params = n.value
+
+    def get_params(self):
+        out = np.empty(self.num_params, np.float32)
+        _check(lib().rbc_mlp_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        p, pp = _host(p, np.float32)
+        _check(lib().rbc_mlp_set_params(self.h, pp))
+
+    def loss_grad(self, x, labels):
+        x, xp = _host(x, np.float32)
+        lab, lp = _host(labels, np.uint32)
+        loss = ctypes.c_float()
+        g = np.empty(self.num_params, np.float32)
+        _check(lib().rbc_mlp_loss_grad(self.h, xp, x.shape[0], x.shape[1], lp, ctypes.byref(loss),
+                                       g.ctypes.data))
+        return loss.value, g
+
+    def logits(self, x):
+        x, xp = _host(x, np.float32)
+        out = np.empty((x.shape[0], self.dims[3]), np.float32)
+        _check(lib().rbc_mlp_logits(self.h, xp, x.shape[0], x.shape[1], out.ctypes.data))
+        return out
+
+    def train(self, x, labels, epochs=5, batch=1024, lr=1e-3, seed=0):
+        x, xp = _host(x, np.float32)
+        lab, lp = _host(labels, np.uint32)
+        losses = np.zeros(epochs, np.float32)
+        _check(lib().rbc_mlp_train(self.h, xp, x.shape[0], x.shape[1], lp, MEM_HOST, epochs, batch,
+                                   lr, seed, losses.ctypes.data))
+        return losses
+
+    def close(self):
+        if self.h:
+            _check(lib().rbc_mlp_destroy(self.h))
             self.h = _vp()
 
     def __del__(self):
