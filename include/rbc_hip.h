@@ -190,6 +190,39 @@ int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const
  * the exact k-query.  The network is not owned by the index. */
 int rbc_exact_set_proposal(rbc_exact *h, rbc_mlp *mlp);
 
+/* ------------------------------------------------------------ batched exact search -- */
+
+/* Many independent exact indexes (each its own database, d <= 128, num_reps <= 1024),
+ * queried together: one persistent kernel walks every index's query tiles, captured in
+ * a hipGraph that is replayed while the batch shape and (device) buffers stay the same.
+ * The query-tile size is autotuned on the first call for a batch shape.  No reference
+ * counterpart. */
+typedef struct rbc_batched rbc_batched;
+
+typedef struct {
+  double ms_graph;          /* hipEvent time of the graph launch */
+  uint32_t tile_queries;    /* autotuned queries per tile */
+  uint32_t graph_replayed;  /* 1 if the call replayed a captured graph */
+  uint32_t uncertified;     /* queries re-run through the single-index path */
+  uint32_t pad;
+  double tune_ms[3];        /* autotuner timings for 16, 32, 64 queries per tile (first call) */
+  double guard_fp64;        /* candidates ordered in FP64 by the near-tie guard */
+} rbc_batched_stats;
+
+/* count indexes; x[i] has n[i] rows of d[i] floats with leading dimension ldx[i]; index
+ * i uses num_reps[i] representatives drawn from seed + i. */
+int rbc_batched_build(uint32_t count, const float *const *x, const uint32_t *n, const uint32_t *d,
+                      const uint32_t *ldx, const uint32_t *num_reps, int metric, uint64_t seed,
+                      int mem, rbc_batched **out);
+/* k-NN (k <= 56) of m[i] queries q[i] (leading dimension ldq[i]) against index i, for all
+ * i; results as rbc_exact_kquery into nn_ids[i], nn_dists[i] (leading dimension ldo[i]).
+ * Runs on the handle's own stream (`stream` is not used). */
+int rbc_batched_kquery(rbc_batched *h, const float *const *q, const uint32_t *m,
+                       const uint32_t *ldq, uint32_t k, uint32_t *const *nn_ids,
+                       float *const *nn_dists, const uint32_t *ldo, int mem, void *stream,
+                       rbc_batched_stats *stats);
+int rbc_batched_destroy(rbc_batched *h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,655 @@
+// Batched exact k-NN: many independent exact-RBC indexes, each with its own query batch,
+// answered by one persistent kernel launch (captured in a hipGraph and replayed).
+//
+// Work unit = a tile of up to QT queries of one index.  Persistent workgroups pull tiles
+// from a global counter across all indexes.  For its tile a workgroup
+//   1. computes every query's FP32 distance to every representative of the index (kept
+//      in a per-workgroup scratch row block),
+//   2. bounds each list from below for the whole tile (the minimum over the tile's
+//      queries of the triangle/Voronoi lower bound) and sorts the lists by it in LDS,
+//   3. scans lists in that order with the FP32 tile engine, keeping every query's best K'
+//      in LDS, skipping a list when no query of the tile can still find a point in it
+//      and stopping when the next list's bound exceeds every query's current k-th
+//      distance (the bound tightens as lists are scanned),
+//   4. writes each query's K' candidates and whether they are provably complete.
+// A second kernel re-ranks the candidates exactly (tree-ordered FP32, FP64 near-tie
+// guard) and writes each index's results.  Queries whose K' could not be proven
+// complete are re-run through the single-index pipeline.
+//
+// The tile size QT is picked by a small autotuner: the first query call on a batch shape
+// times QT = 16, 32 and 64 and keeps the fastest.
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "exact_device.hpp"
+#include "exact_query.hpp"
+
+struct rbc_batched;
+
+namespace rbc {
+
+namespace {
+
+struct BInst {
+  const float *xp;
+  const float *reps;
+  const uint32_t *list_off;
+  const uint32_t *perm;
+  const uint32_t *inv;
+  const float *radius;
+  const float *rr;
+  uint32_t n, d, dpad, ld, R, P;  // P: tree width (64, 128)
+  const float *q;                 // this index's queries (ldq), m of them
+  uint32_t ldq, m, qbase;         // qbase: first global query number
+  float *out_d;
+  uint32_t *out_i;
+  uint32_t ldo;
+};
+
+struct BTile {
+  uint32_t inst, q0, qn;
+};
+
+struct BatchArgs {
+  const BInst *inst;
+  const BTile *tiles;
+  uint32_t ntiles;
+  uint32_t *next;     // work counter
+  float *scratch;     // per workgroup: kTQ x Rmax
+  uint32_t Rmax, Rpad;
+  uint32_t k, Kp;
+  double e;           // FP32 FMA-order relative error
+  int metric;
+  uint32_t *cand_i;   // [mtot][Kp]
+  float *cand_d;      // [mtot][Kp]
+  uint32_t *cert;     // [mtot]
+  uint32_t *qinst;    // [mtot] index of each global query
+};
+
+// Dense distances of the tile's queries to a block of reps into the scratch rows.
+struct EpiScratch {
+  float *out;
+  uint32_t ld;
+  uint32_t col0;
+  static __host__ __device__ size_t lds_bytes() { return 0; }
+  __device__ void begin(const TileCtx &) {}
+  __device__ void step(const TileCtx &c, float (&acc)[4][4], uint32_t p0) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        uint32_t p = p0 + 4 * c.pg + b;
+        if (c.pid_s[4 * c.pg + b] != kDummy) out[(4 * c.qg + a) * ld + col0 + p] = acc[a][b];
+      }
+  }
+  __device__ void end(const TileCtx &) {}
+};
+
+__host__ __device__ inline size_t batch_lds_bytes(uint32_t dpad, uint32_t Rpad) {
+  return tile_lds_base_bytes(dpad) + EpiTopK::lds_bytes() + size_t(Rpad) * 8 + kTQ * 16 + 64;
+}
+
+template <int METRIC>
+__global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_t dpad_max) {
+  extern __shared__ float smem[];
+  const int t = threadIdx.x;
+  // LDS: [tile engine + TopK state][tl: Rpad floats][ord: Rpad u32][j0, d0, gam per query][misc]
+  unsigned char *base = reinterpret_cast<unsigned char *>(smem);
+  float *tl = reinterpret_cast<float *>(base + tile_lds_base_bytes(dpad_max) + EpiTopK::lds_bytes());
+  uint32_t *ord = reinterpret_cast<uint32_t *>(tl + a.Rpad);
+  uint32_t *j0s = ord + a.Rpad;
+  float *d0s = reinterpret_cast<float *>(j0s + kTQ);
+  float *gam = d0s + kTQ;
+  float *gtmp = gam + kTQ;  // [kTQ] scratch
+  uint32_t *misc = reinterpret_cast<uint32_t *>(gtmp + kTQ);  // [0] tile, [1] need flag, [2] gmax bits
+  float *scr = a.scratch + size_t(blockIdx.x) * kTQ * a.Rmax;
+
+  for (;;) {
+    __syncthreads();
+    if (t == 0) misc[0] = atomicAdd(a.next, 1u);
+    __syncthreads();
+    const uint32_t tile = misc[0];
+    if (tile >= a.ntiles) break;
+    const BTile T = a.tiles[tile];
+    const BInst I = a.inst[T.inst];
+    ScanItem it{0, 0, T.q0, T.qn, T.inst};
+    // 1. queries -> LDS; distances to all reps -> scratch.
+    TileSrc qsrc{I.xp, I.ld, nullptr, I.perm, I.q, I.ldq, nullptr, I.d, I.dpad};
+    TileCtx ctx = tile_setup(qsrc, it, smem);
+    {
+      TileSrc rsrc{I.reps, I.ld, nullptr, nullptr, I.q, I.ldq, nullptr, I.d, I.dpad};
+      EpiScratch es{scr, a.Rmax, 0};
+      tile_steps<METRIC>(rsrc, 0, I.R, es, ctx, smem);
+    }
+    __syncthreads();
+    // Nearest rep per query (lowest index on ties).
+    if (t < kTQ) {
+      float bv = FLT_MAX;
+      uint32_t bj = 0;
+      if ((uint32_t)t < T.qn)
+        for (uint32_t j = 0; j < I.R; ++j) {
+          float v = scr[t * a.Rmax + j];
+          if (v < bv) {
+            bv = v;
+            bj = j;
+          }
+        }
+      j0s[t] = bj;
+      d0s[t] = bv;
+      gam[t] = INFINITY;
+    }
+    __syncthreads();
+    // 2. Tile lower bound of every list, then sort ascending.
+    for (uint32_t j = t; j < a.Rpad; j += kNT) {
+      float b = INFINITY;
+      if (j < I.R) {
+        double psi = (double)I.radius[j];
+        double mn = INFINITY;
+        for (uint32_t qi = 0; qi < T.qn; ++qi) {
+          double lb = list_lower_bound((double)scr[qi * a.Rmax + j], (double)d0s[qi],
+                                       (double)I.rr[size_t(j0s[qi]) * I.R + j], psi, a.e, j == j0s[qi], METRIC);
+          mn = fmin(mn, lb);
+        }
+        b = (float)mn;
+        if ((double)b > mn) b = nextafterf(b, -INFINITY);
+      }
+      tl[j] = b;
+      ord[j] = j;
+    }
+    for (uint32_t kk = 2; kk <= a.Rpad; kk <<= 1)
+      for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+        __syncthreads();
+        for (uint32_t i = t; i < a.Rpad; i += kNT) {
+          uint32_t p = i ^ jj;
+          if (p > i) {
+            bool up = (i & kk) == 0;
+            float vi = tl[i], vp = tl[p];
+            bool sw = up ? (vp < vi || (vp == vi && ord[p] < ord[i])) : (vi < vp || (vi == vp && ord[i] < ord[p]));
+            if (sw) {
+              tl[i] = vp;
+              tl[p] = vi;
+              uint32_t o = ord[i];
+              ord[i] = ord[p];
+              ord[p] = o;
+            }
+          }
+        }
+      }
+    __syncthreads();
+    // 3. Scan lists in bound order with per-query best K'.
+    EpiTopK epi{a.Kp, nullptr, nullptr, 0, 0};
+    epi.begin(ctx);
+    if (t == 0) misc[2] = __float_as_uint(INFINITY);
+    __syncthreads();
+    for (uint32_t idx = 0; idx < I.R; ++idx) {
+      __syncthreads();  // everyone has read the previous iteration's flag
+      const float gmax = __uint_as_float(misc[2]);
+      if (tl[idx] > gmax) break;  // every later list is out of every query's reach
+      const uint32_t j = ord[idx];
+      if (t == 0) misc[1] = 0;
+      __syncthreads();
+      if ((uint32_t)t < T.qn) {
+        double lb = list_lower_bound((double)scr[t * a.Rmax + j], (double)d0s[t],
+                                     (double)I.rr[size_t(j0s[t]) * I.R + j], (double)I.radius[j], a.e,
+                                     j == j0s[t], METRIC);
+        if (lb <= (double)gam[t]) misc[1] = 1;
+      }
+      __syncthreads();
+      if (!misc[1]) continue;
+      tile_steps<METRIC>(qsrc, I.list_off[j], I.list_off[j + 1] - I.list_off[j], epi, ctx, smem);
+      // Refresh each query's bound from its current k-th value.
+      EpiTopK::L L = epi.lay(ctx);
+      if (t < kTQ) {
+        float g = INFINITY;
+        if ((uint32_t)t < T.qn) {
+          float v = L.td[t * 64 + (a.k - 1)];
+          if (v < FLT_MAX) {
+            double ghi = (double)v / (1.0 - a.e) * (1.0 + 1e-12);
+            double gd = METRIC == 0 ? sqrt(ghi) * (1.0 + 1e-12) : ghi;
+            g = (float)gd;
+            if ((double)g < gd) g = nextafterf(g, INFINITY);
+          }
+          gam[t] = g;
+        }
+        gtmp[t] = (uint32_t)t < T.qn ? g : 0.f;
+      }
+      __syncthreads();
+      if (t == 0) {
+        float mx = 0.f;
+        for (uint32_t qi = 0; qi < T.qn; ++qi) mx = fmaxf(mx, gtmp[qi]);
+        misc[2] = __float_as_uint(mx);
+      }
+      __syncthreads();
+    }
+    // 4. Candidates and completeness.
+    EpiTopK::L L = epi.lay(ctx);
+    for (uint32_t i = t; i < kTQ * a.Kp; i += kNT) {
+      const uint32_t qi = i / a.Kp, jj = i % a.Kp;
+      if (qi >= T.qn) continue;
+      const size_t g = size_t(I.qbase + T.q0 + qi);
+      a.cand_i[g * a.Kp + jj] = L.ti[qi * 64 + jj];
+      a.cand_d[g * a.Kp + jj] = L.td[qi * 64 + jj];
+      if (jj == 0) {
+        const float vk = L.td[qi * 64 + a.k - 1], vK = L.td[qi * 64 + a.Kp - 1];
+        // Points left out have FP32 value >= vK; they cannot beat the k-th kept when
+        // vK / (1 + e) > vk / (1 - e).
+        bool ok = vK >= FLT_MAX || (double)vK / (1.0 + a.e) > (double)vk / (1.0 - a.e) * (1.0 + 1e-12);
+        a.cert[g] = ok ? 1u : 0u;
+      }
+    }
+  }
+}
+
+struct BSelArgs {
+  const BInst *inst;
+  const uint32_t *qinst;
+  uint32_t mtot, k, Kp, P;
+  const uint32_t *cand_i;
+  const uint32_t *cert;
+  unsigned long long *stats;
+};
+
+template <int METRIC, int NPL>
+__global__ __launch_bounds__(256) void batched_select(BSelArgs a) {
+  const uint32_t g = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (g >= a.mtot) return;
+  const BInst &I = a.inst[a.qinst[g]];
+  if (I.P != a.P || !a.cert[g]) return;
+  const int l = lane_id();
+  const uint32_t q = g - I.qbase;
+  const float *qrow = I.q + size_t(q) * I.ldq;
+  float qv[NPL];
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    uint32_t c = l + kWave * j;
+    qv[j] = c < I.d ? qrow[c] : 0.f;
+  }
+  float tv = FLT_MAX;
+  uint32_t ti = kDummy;
+  auto cand = [&](uint32_t e) -> uint32_t { return a.cand_i[size_t(g) * a.Kp + e]; };
+  rerank_candidates<METRIC, NPL>(tv, ti, a.Kp, cand, I.xp, I.ld, I.inv, qv, I.d);
+  const double P = (double)a.P;
+  const double e_tree = ((P <= 64 ? 6.0 : 7.0) + 4.0) * 0x1p-24 * 1.01;
+  near_tie_guard<METRIC>(tv, ti, a.Kp, I.xp, I.ld, I.inv, qrow, I.d, a.k, e_tree, a.stats);
+  if (l < (int)a.k) {
+    I.out_d[size_t(q) * I.ldo + l] = tv;
+    I.out_i[size_t(q) * I.ldo + l] = ti;
+  }
+}
+
+__global__ void reset_counter(uint32_t *c, unsigned long long *stats) {
+  c[0] = 0;
+  stats[0] = stats[1] = stats[2] = stats[3] = 0;
+}
+
+}  // namespace
+
+int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, uint32_t k, int precision,
+                       float *od, uint32_t *oi, uint32_t ldo, hipStream_t st, rbc_query_stats *stats);
+
+}  // namespace rbc
+
+struct rbc_batched {
+  std::vector<rbc_exact *> idx;
+  int metric = RBC_L2;
+  uint32_t Rmax = 0, Rpad = 0, dpad_max = 0;
+  hipStream_t stream = nullptr;
+  // Per-call state, reused while the shapes and buffers stay the same.
+  struct Plan {
+    std::vector<uint32_t> m;
+    std::vector<const void *> qptr, optr;
+    uint32_t k = 0, qt = 0;
+    hipGraphExec_t exec = nullptr;
+  } plan;
+  rbc::DevBuf<rbc::BInst> dinst;
+  rbc::DevBuf<rbc::BTile> dtiles;
+  rbc::DevBuf<uint32_t> counter, cand_i, cert, qinst;
+  rbc::DevBuf<float> cand_d, scratch, qstage;
+  rbc::DevBuf<unsigned long long> stats;
+  uint32_t ntiles = 0, mtot = 0, grid = 0;
+  std::map<std::vector<uint32_t>, uint32_t> tuned;  // batch shape -> tile size
+  ~rbc_batched() {
+    if (plan.exec) (void)hipGraphExecDestroy(plan.exec);
+    for (auto *p : idx) rbc_exact_destroy(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace rbc {
+
+namespace {
+
+int batch_tiles(rbc_batched *b, const std::vector<uint32_t> &m, uint32_t qt) {
+  std::vector<BTile> tiles;
+  for (uint32_t i = 0; i < m.size(); ++i)
+    for (uint32_t q0 = 0; q0 < m[i]; q0 += qt) tiles.push_back(BTile{i, q0, std::min(qt, m[i] - q0)});
+  // Largest indexes first: their tiles take longest.
+  std::stable_sort(tiles.begin(), tiles.end(), [&](const BTile &x, const BTile &y) {
+    return b->idx[x.inst]->n > b->idx[y.inst]->n;
+  });
+  b->ntiles = (uint32_t)tiles.size();
+  RBC_TRY(b->dtiles.ensure(std::max<size_t>(1, tiles.size())));
+  if (!tiles.empty())
+    RBC_HIP_TRY(hipMemcpy(b->dtiles.get(), tiles.data(), tiles.size() * sizeof(BTile), hipMemcpyHostToDevice));
+  return RBC_OK;
+}
+
+int batch_launch(rbc_batched *b, uint32_t k, uint32_t Kp, hipStream_t st) {
+  hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, b->counter.get(), b->stats.get());
+  BatchArgs a;
+  a.inst = b->dinst.get();
+  a.tiles = b->dtiles.get();
+  a.ntiles = b->ntiles;
+  a.next = b->counter.get();
+  a.scratch = b->scratch.get();
+  a.Rmax = b->Rmax;
+  a.Rpad = b->Rpad;
+  a.k = k;
+  a.Kp = Kp;
+  a.e = fma_rel_err(b->dpad_max);
+  a.metric = b->metric;
+  a.cand_i = b->cand_i.get();
+  a.cand_d = b->cand_d.get();
+  a.cert = b->cert.get();
+  a.qinst = b->qinst.get();
+  const size_t lds = batch_lds_bytes(b->dpad_max, b->Rpad);
+  if (b->metric == RBC_L1)
+    hipLaunchKernelGGL(batched_exact_kernel<1>, dim3(b->grid), dim3(kNT), lds, st, a, b->dpad_max);
+  else
+    hipLaunchKernelGGL(batched_exact_kernel<0>, dim3(b->grid), dim3(kNT), lds, st, a, b->dpad_max);
+  RBC_HIP_TRY(hipGetLastError());
+  BSelArgs s{b->dinst.get(), b->qinst.get(), b->mtot, k, Kp, 64, b->cand_i.get(), b->cert.get(), b->stats.get()};
+  dim3 g(cdiv(b->mtot, 4));
+  for (uint32_t P : {64u, 128u}) {
+    s.P = P;
+    if (b->metric == RBC_L1) {
+      if (P == 64) hipLaunchKernelGGL((batched_select<1, 1>), g, dim3(256), 0, st, s);
+      else hipLaunchKernelGGL((batched_select<1, 2>), g, dim3(256), 0, st, s);
+    } else {
+      if (P == 64) hipLaunchKernelGGL((batched_select<0, 1>), g, dim3(256), 0, st, s);
+      else hipLaunchKernelGGL((batched_select<0, 2>), g, dim3(256), 0, st, s);
+    }
+    RBC_HIP_TRY(hipGetLastError());
+  }
+  return RBC_OK;
+}
+
+}  // namespace
+
+}  // namespace rbc
+
+using namespace rbc;
+
+extern "C" {
+
+int rbc_batched_build(uint32_t count, const float *const *x, const uint32_t *n, const uint32_t *d,
+                      const uint32_t *ldx, const uint32_t *num_reps, int metric, uint64_t seed, int mem,
+                      rbc_batched **out) {
+  RBC_ARG_CHECK(out && x && n && d && ldx && num_reps && count, "bad argument");
+  *out = nullptr;
+  auto b = new rbc_batched();
+  b->metric = metric;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (d[i] > 128) {
+      delete b;
+      set_error("batched index %u: d=%u above the batched limit of 128", i, d[i]);
+      return RBC_ERR_LIMIT;
+    }
+    rbc_exact *h = nullptr;
+    int s = rbc_exact_build(x[i], n[i], d[i], ldx[i], num_reps[i], metric, nullptr, seed + i, mem, &h);
+    if (s != RBC_OK) {
+      delete b;
+      return s;
+    }
+    b->idx.push_back(h);
+    b->Rmax = std::max(b->Rmax, h->R);
+    b->dpad_max = std::max(b->dpad_max, h->dpad);
+  }
+  b->Rpad = 1;
+  while (b->Rpad < b->Rmax) b->Rpad <<= 1;
+  if (b->Rpad > 1024) {
+    delete b;
+    set_error("batched indexes support at most 1024 representatives (got %u)", b->Rmax);
+    return RBC_ERR_LIMIT;
+  }
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    set_error("device query failed");
+    return RBC_ERR_HIP;
+  }
+  const size_t lds = batch_lds_bytes(b->dpad_max, b->Rpad);
+  if ((int)lds > max_lds_bytes()) {
+    delete b;
+    set_error("batched kernel needs %zu bytes of LDS", lds);
+    return RBC_ERR_LIMIT;
+  }
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batched_exact_kernel<0>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batched_exact_kernel<1>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
+  b->grid = (uint32_t)prop.multiProcessorCount;  // one resident workgroup per CU
+  int s;
+  if ((s = b->scratch.alloc(size_t(b->grid) * kTQ * b->Rmax)) || (s = b->counter.alloc(1)) ||
+      (s = b->stats.alloc(4))) {
+    delete b;
+    return s;
+  }
+  *out = b;
+  return RBC_OK;
+}
+
+int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m, const uint32_t *ldq,
+                       uint32_t k, uint32_t *const *nn_ids, float *const *nn_dists, const uint32_t *ldo,
+                       int mem, void *stream, rbc_batched_stats *stats) {
+  RBC_ARG_CHECK(b && q && m && ldq && nn_ids && nn_dists && ldo, "bad argument");
+  RBC_ARG_CHECK(k >= 1 && k <= 56, "k=%u outside [1, 56]", k);
+  (void)stream;
+  hipStream_t st = b->stream;
+  const uint32_t count = (uint32_t)b->idx.size();
+  const uint32_t Kp = std::min<uint32_t>(64, k + 8);
+  std::vector<uint32_t> mv(m, m + count);
+  for (uint32_t i = 0; i < count; ++i) {
+    RBC_ARG_CHECK(k <= b->idx[i]->n, "k=%u exceeds index %u's size", k, i);
+    RBC_ARG_CHECK(ldq[i] >= b->idx[i]->d && ldo[i] >= k, "bad leading dimension for index %u", i);
+  }
+  // Stage queries (device memory with ld % 4 == 0 is used in place).
+  std::vector<size_t> qoff(count + 1, 0), ooff(count + 1, 0);
+  for (uint32_t i = 0; i < count; ++i) {
+    qoff[i + 1] = qoff[i] + size_t(m[i]) * pad16(b->idx[i]->d);
+    ooff[i + 1] = ooff[i] + size_t(m[i]) * k;
+  }
+  const bool in_place = mem == RBC_MEM_DEVICE;
+  for (uint32_t i = 0; in_place && i < count; ++i)
+    RBC_ARG_CHECK(ldq[i] % 4 == 0, "device queries need a leading dimension that is a multiple of 4");
+  DevBuf<float> od;
+  DevBuf<uint32_t> oi;
+  if (!in_place) {
+    RBC_TRY(b->qstage.ensure(std::max<size_t>(1, qoff[count])));
+    RBC_TRY(od.alloc(std::max<size_t>(1, ooff[count])));
+    RBC_TRY(oi.alloc(std::max<size_t>(1, ooff[count])));
+    for (uint32_t i = 0; i < count; ++i)
+      if (m[i])
+        RBC_HIP_TRY(hipMemcpy2DAsync(b->qstage.get() + qoff[i], pad16(b->idx[i]->d) * 4, q[i], size_t(ldq[i]) * 4,
+                                     size_t(b->idx[i]->d) * 4, m[i], hipMemcpyHostToDevice, st));
+  }
+  std::vector<BInst> hi(count);
+  uint32_t mtot = 0;
+  std::vector<uint32_t> qin;
+  for (uint32_t i = 0; i < count; ++i) {
+    rbc_exact *h = b->idx[i];
+    BInst &I = hi[i];
+    I.xp = h->xp.get();
+    I.reps = h->reps.get();
+    I.list_off = h->list_off.get();
+    I.perm = h->perm.get();
+    I.inv = h->inv.get();
+    I.radius = h->radius.get();
+    I.rr = h->rr.get();
+    I.n = h->n;
+    I.d = h->d;
+    I.dpad = h->dpad;
+    I.ld = h->ld;
+    I.R = h->R;
+    I.P = tree_width(h->d);
+    I.q = in_place ? q[i] : b->qstage.get() + qoff[i];
+    I.ldq = in_place ? ldq[i] : pad16(h->d);
+    I.m = m[i];
+    I.qbase = mtot;
+    I.out_d = in_place ? nn_dists[i] : od.get() + ooff[i];
+    I.out_i = in_place ? nn_ids[i] : oi.get() + ooff[i];
+    I.ldo = in_place ? ldo[i] : k;
+    mtot += m[i];
+    qin.insert(qin.end(), m[i], i);
+  }
+  if (mtot == 0) return RBC_OK;
+  RBC_TRY(b->dinst.ensure(count));
+  RBC_HIP_TRY(hipMemcpyAsync(b->dinst.get(), hi.data(), count * sizeof(BInst), hipMemcpyHostToDevice, st));
+  b->mtot = mtot;
+  RBC_TRY(b->qinst.ensure(mtot));
+  RBC_TRY(b->cand_i.ensure(size_t(mtot) * Kp));
+  RBC_TRY(b->cand_d.ensure(size_t(mtot) * Kp));
+  RBC_TRY(b->cert.ensure(mtot));
+  RBC_HIP_TRY(hipMemcpyAsync(b->qinst.get(), qin.data(), mtot * 4, hipMemcpyHostToDevice, st));
+
+  // Autotune the tile size once per batch shape.
+  std::vector<uint32_t> key(mv);
+  key.push_back(k);
+  uint32_t qt = 0;
+  double tune_ms[3] = {0, 0, 0};
+  auto found = b->tuned.find(key);
+  if (found == b->tuned.end()) {
+    const uint32_t cand_qt[3] = {16, 32, 64};
+    float best = 1e30f;
+    hipEvent_t e0, e1;
+    RBC_HIP_TRY(hipEventCreate(&e0));
+    RBC_HIP_TRY(hipEventCreate(&e1));
+    for (int c = 0; c < 3; ++c) {
+      RBC_TRY(batch_tiles(b, mv, cand_qt[c]));
+      RBC_HIP_TRY(hipEventRecord(e0, st));
+      RBC_TRY(batch_launch(b, k, Kp, st));
+      RBC_HIP_TRY(hipEventRecord(e1, st));
+      RBC_HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.f;
+      RBC_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      tune_ms[c] = ms;
+      if (ms < best) {
+        best = ms;
+        qt = cand_qt[c];
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->tuned[key] = qt;
+  } else {
+    qt = found->second;
+  }
+  // Graph: reset counter + persistent kernel + re-rank; replayed while nothing changes.
+  std::vector<const void *> qp(q, q + count), op;
+  for (uint32_t i = 0; i < count; ++i) op.push_back(in_place ? (const void *)nn_ids[i] : (const void *)oi.get());
+  bool replay = b->plan.exec && b->plan.m == mv && b->plan.qptr == qp && b->plan.optr == op && b->plan.k == k &&
+                b->plan.qt == qt && in_place;
+  if (!replay) {
+    if (b->plan.exec) {
+      (void)hipGraphExecDestroy(b->plan.exec);
+      b->plan.exec = nullptr;
+    }
+    RBC_TRY(batch_tiles(b, mv, qt));
+    hipGraph_t graph;
+    RBC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int s = batch_launch(b, k, Kp, st);
+    hipError_t ce = hipStreamEndCapture(st, &graph);
+    if (s != RBC_OK) return s;
+    RBC_HIP_TRY(ce);
+    RBC_HIP_TRY(hipGraphInstantiate(&b->plan.exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    b->plan.m = mv;
+    b->plan.qptr = qp;
+    b->plan.optr = op;
+    b->plan.k = k;
+    b->plan.qt = qt;
+  }
+  hipEvent_t t0, t1;
+  RBC_HIP_TRY(hipEventCreate(&t0));
+  RBC_HIP_TRY(hipEventCreate(&t1));
+  RBC_HIP_TRY(hipEventRecord(t0, st));
+  RBC_HIP_TRY(hipGraphLaunch(b->plan.exec, st));
+  RBC_HIP_TRY(hipEventRecord(t1, st));
+  RBC_HIP_TRY(hipEventSynchronize(t1));
+  float ms = 0.f;
+  RBC_HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (!in_place) {
+    // The graph wrote to this call's staging buffers: keep no replay across calls.
+    (void)hipGraphExecDestroy(b->plan.exec);
+    b->plan.exec = nullptr;
+  }
+  unsigned long long hs[4];
+  RBC_HIP_TRY(hipMemcpy(hs, b->stats.get(), sizeof(hs), hipMemcpyDeviceToHost));
+  if (hs[2]) {
+    set_error("%llu queries have more than 64 candidates within the FP32 error bound of each other", hs[2]);
+    return RBC_ERR_LIMIT;
+  }
+  // Queries whose K' is not provably complete: re-run them through the single-index path.
+  std::vector<uint32_t> hc(mtot);
+  RBC_HIP_TRY(hipMemcpy(hc.data(), b->cert.get(), mtot * 4, hipMemcpyDeviceToHost));
+  uint32_t uncert = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    std::vector<uint32_t> rows;
+    for (uint32_t qi = 0; qi < m[i]; ++qi)
+      if (!hc[hi[i].qbase + qi]) rows.push_back(qi);
+    if (rows.empty()) continue;
+    uncert += (uint32_t)rows.size();
+    rbc_exact *h = b->idx[i];
+    const uint32_t ldh = pad16(h->d);
+    DevBuf<float> qs, fd;
+    DevBuf<uint32_t> fi;
+    RBC_TRY(qs.alloc(rows.size() * ldh));
+    RBC_TRY(fd.alloc(rows.size() * k));
+    RBC_TRY(fi.alloc(rows.size() * k));
+    for (size_t r = 0; r < rows.size(); ++r)
+      RBC_HIP_TRY(hipMemcpyAsync(qs.get() + r * ldh, hi[i].q + size_t(rows[r]) * hi[i].ldq, size_t(h->d) * 4,
+                                 hipMemcpyDeviceToDevice, st));
+    RBC_TRY(exact_query_common(h, qs.get(), ldh, (uint32_t)rows.size(), k, RBC_PREC_FP32, fd.get(), fi.get(), k, st,
+                               nullptr));
+    for (size_t r = 0; r < rows.size(); ++r) {
+      RBC_HIP_TRY(hipMemcpyAsync(hi[i].out_d + size_t(rows[r]) * hi[i].ldo, fd.get() + r * k, k * 4,
+                                 hipMemcpyDeviceToDevice, st));
+      RBC_HIP_TRY(hipMemcpyAsync(hi[i].out_i + size_t(rows[r]) * hi[i].ldo, fi.get() + r * k, k * 4,
+                                 hipMemcpyDeviceToDevice, st));
+    }
+    RBC_HIP_TRY(hipStreamSynchronize(st));
+  }
+  if (!in_place) {
+    for (uint32_t i = 0; i < count; ++i)
+      if (m[i]) {
+        RBC_HIP_TRY(hipMemcpy2DAsync(nn_dists[i], size_t(ldo[i]) * 4, od.get() + ooff[i], size_t(k) * 4, size_t(k) * 4,
+                                     m[i], hipMemcpyDeviceToHost, st));
+        RBC_HIP_TRY(hipMemcpy2DAsync(nn_ids[i], size_t(ldo[i]) * 4, oi.get() + ooff[i], size_t(k) * 4, size_t(k) * 4,
+                                     m[i], hipMemcpyDeviceToHost, st));
+      }
+  }
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  if (stats) {
+    stats->ms_graph = ms;
+    stats->tile_queries = qt;
+    stats->graph_replayed = replay ? 1u : 0u;
+    stats->uncertified = uncert;
+    stats->tune_ms[0] = tune_ms[0];
+    stats->tune_ms[1] = tune_ms[1];
+    stats->tune_ms[2] = tune_ms[2];
+    stats->guard_fp64 = (double)hs[0];
+  }
+  return RBC_OK;
+}
+
+int rbc_batched_destroy(rbc_batched *b) {
+  delete b;
+  return RBC_OK;
+}
+
+}  // extern "C"

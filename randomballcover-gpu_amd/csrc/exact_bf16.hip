@@ -1,20 +1,20 @@
 // BF16 MFMA distance filter of the exact query (precision RBC_PREC_BF16).
 //
-// The list scan computes D~ = |q~|^2 + |x~|^2 - 2 q~.x~ on BF16 copies (q~ = bf16(q),
-// x~ = bf16(x)) with v_mfma_f32_16x16x32_bf16 (FP32 accumulation) and emits every point
-// whose TRUE distance could still be within gamma (the probe's upper bound on the k-th
-// NN distance).  With eta = |q - q~| + max_list |x - x~| (FP64, rounded up) and
-// E = (gamma_K + 6u')(|q~|^2 + max_list |x~|^2) bounding the FP32 error of D~ (K the
-// padded dimension, u' = 2^-23 so that no rounding mode of the accumulator is assumed),
-//     |q - x| <= G  implies  D~ <= (G + eta)^2 + E,
-// so the emission test is conservative; the exact FP32 re-rank decides.
+// The list scan evaluates q~.x~ on BF16 copies (q~ = bf16(q), x~ = bf16(x)) with
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation) and emits every point whose TRUE
+// distance could still be within G, the probe's upper bound on the k-th NN distance.
+// With eta = |q - q~| + max_list |x - x~| (FP64, rounded up), |q - x| <= G implies
+// |q~ - x~| <= G + eta, which the kernel tests as q~.x~ + s >= |x~|^2/2 with s folded into
+// the accumulator's initial value and a margin of (gamma_K + 6u') times the absolute
+// terms (u' = 2^-23, so no rounding mode of the accumulator is assumed).  The test is
+// conservative; the exact FP32 re-rank decides.
 //
-// Work item = (one ownership list, up to 128 queries that need it).  A workgroup of four
-// waves keeps the 128 queries' BF16 rows in VGPRs as MFMA A operands; each wave streams
-// its own 16 points per 64-point step as B operands straight from HBM/L2 (one 16-byte
-// load per lane per 32 dimensions, prefetched one step ahead).  Items are list-major and
-// remapped so that each XCD takes a contiguous range: the items of a list run on one
-// XCD and re-read the list from that XCD's L2.
+// Scan work item = (one ownership list, up to 256 queries that need it).  The four waves
+// of a workgroup keep 64 queries each as MFMA A operands in VGPRs and share the list's
+// points, 32 per step, through a three-stage LDS ring filled two steps ahead.  Items are
+// list-major and remapped so that each XCD takes a contiguous range: the items of a list
+// run on one XCD and re-read the list from that XCD's L2.  The probe (top-K' of the
+// nearest list) uses 128-query items and per-wave point streams.
 #include "exact_query.hpp"
 
 namespace rbc {
@@ -24,8 +24,8 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
-constexpr int kBQ = 128;  // queries per item
-constexpr int kBP = 64;   // points per step (16 per wave)
+constexpr int kBQ = 128;  // probe: queries per item
+constexpr int kBP = 64;   // probe: points per step (16 per wave)
 constexpr int kXCD = 8;
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -86,115 +86,152 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
   return *reinterpret_cast<const bf16x8 *>(p);
 }
 
+// ---- BF16 list scan ------------------------------------------------------------------
+// Work item = (one list, up to 256 queries).  Wave w keeps queries 64w..64w+63 as MFMA A
+// operands in VGPRs; the 32 points of a step are shared by all four waves through a
+// three-stage LDS ring (rows padded to 272 bytes so that the 16 lanes reading 16 different
+// points at one k-offset hit distinct banks).  Each thread stages 32 bytes of a future step
+// in registers: loads for step s+2 are issued while step s computes, written to LDS during
+// step s+1.  One barrier per step.
+constexpr int kVQ = kBf16ScanQ;  // queries per item
+constexpr int kVP = 32;          // points per step
+constexpr int kRowB = 272;       // LDS bytes per point row (256 + 16 pad)
+constexpr int kStages = 3;
+
 template <int KS>
 __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
-  // XCD-aware remap: blocks b, b+8, b+16, ... (one XCD) take a contiguous item range.
   const uint32_t per = (a.nitems + kXCD - 1) / kXCD;
   const uint32_t item = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
   if (item >= a.nitems) return;
   const ScanItem it = a.items[item];
-  __shared__ float thr[kBQ];   // t_i: emit when nx - 2 q~.x~ <= t_i
-  __shared__ float sh[kBQ];    // t_i / 2 plus a margin covering FP32 rounding (pre-filter)
-  __shared__ uint32_t qrow[kBQ];
+  // Emission test per (query i, point x): MFMA(s_i + q~.x~) >= |x~|^2 / 2, with the
+  // accumulator initialised to s_i = (T_i - |q~|^2)/2 + margin, T_i = (G_i + eta)^2.  Any x
+  // with |q - x| <= G_i has |q~ - x~| <= G_i + eta, i.e. q~.x~ + (T_i - |q~|^2)/2 >= |x~|^2/2
+  // in exact arithmetic; the margin covers the FP32 accumulation (gamma_{K+1} of the
+  // absolute terms) and the rounding of the stored norms, so the test never drops it.
+  __shared__ float sh[kVQ];
+  __shared__ uint32_t qrow[kVQ];
+  __shared__ __attribute__((aligned(16))) unsigned char ring[kStages][kVP * kRowB];
+  __shared__ float hring[kStages][kVP];  // 0.5 * ||bf16(x)||^2 of the step's points (INF past the end)
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const uint32_t list = it.aux;
-  if (t < kBQ) {
+  {
     uint32_t row = (uint32_t)t < it.q_count ? a.qids[it.q_begin + t] : kDummy;
     qrow[t] = row;
-    float th = -INFINITY, hs = -INFINITY;
+    float hs = -INFINITY;
     if (row != kDummy) {
-      double G = a.gam[row];
-      double eta = (double)a.qe[row] + (double)a.lqe[list];
-      double nq = (double)a.qn[row];
-      double nxm = (double)a.lnm[list];
-      double E = a.c_gemm * (nq + nxm) + 1e-30;
-      double T = (G + eta) * (G + eta) + E;
-      double tt = (T - nq) * (1.0 + 1e-9) + 1e-30;
-      th = tt >= (double)FLT_MAX ? INFINITY : (float)tt;
-      if ((double)th < tt) th = nextafterf(th, INFINITY);
-      // Pre-filter: nx - 2 acc <= t  implies  acc + (t/2 + margin) >= nx/2 in FP32 when
-      // the margin exceeds the rounding of both forms (|acc| <= (nq + nx)/2).
-      double hh = 0.5 * tt + 0x1p-20 * (fabs(tt) + 2.0 * (nq + nxm)) + 1e-30;
+      const double G = a.gam[row];
+      const double eta = (double)a.qe[row] + (double)a.lqe[list];
+      const double nq = (double)a.qn[row];
+      const double nxm = (double)a.lnm[list];
+      const double T0 = (G + eta) * (G + eta);
+      const double s0 = 0.5 * (T0 - nq);
+      const double up = 0x1p-23;
+      const double margin = a.c_gemm * (fabs(s0) + 0.5 * (nq + nxm)) + up * (nq + nxm) + 1e-30;
+      const double hh = (s0 + margin) * (1.0 + 1e-9);
       hs = hh >= (double)FLT_MAX ? INFINITY : (float)hh;
       if ((double)hs < hh) hs = nextafterf(hs, INFINITY);
     }
-    thr[t] = th;
     sh[t] = hs;
   }
   __syncthreads();
-
-  // A operands: 8 blocks of 16 queries x KS steps of 32 dims.
-  bf16x8 A[8][KS];
+  // This wave's queries: A operands and the initial accumulators.
+  const bool wave_live = (uint32_t)(w * 64) < it.q_count;
+  bf16x8 A[4][KS];
+  f32x4 S[4];
   const uint32_t koff = (l / 16) * 8;
 #pragma unroll
-  for (int qb = 0; qb < 8; ++qb) {
-    uint32_t row = qrow[qb * 16 + (l % 16)];
-#pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      if (row != kDummy)
-        A[qb][s] = load8(a.qb + size_t(row) * a.ldb + s * 32 + koff);
-      else
-        A[qb][s] = bf16x8{};
-    }
-  }
-
-  const uint32_t x0 = it.x_begin, np = it.x_count;
-  const uint32_t pl = w * 16 + (l % 16);  // this lane's point within a step
-  bf16x8 Bn[KS];
-  {
-    const bool pv = pl < np;
-    const uint16_t *src = a.xb + size_t(x0 + (pv ? pl : 0)) * a.ldb + koff;
-#pragma unroll
-    for (int s = 0; s < KS; ++s) Bn[s] = pv ? load8(src + s * 32) : bf16x8{};
-  }
-  for (uint32_t p0 = 0; p0 < np; p0 += kBP) {
-    bf16x8 B[KS];
-#pragma unroll
-    for (int s = 0; s < KS; ++s) B[s] = Bn[s];
-    const uint32_t p = p0 + pl;
-    const bool pv = p < np;
-    // Prefetch the next step's operands.
-    {
-      const uint32_t pn = p + kBP;
-      const bool nv = pn < np;
-      const uint16_t *src = a.xb + size_t(x0 + (nv ? pn : 0)) * a.ldb + koff;
-#pragma unroll
-      for (int s = 0; s < KS; ++s) Bn[s] = nv ? load8(src + s * 32) : bf16x8{};
-    }
-    const float nx = pv ? a.xn[x0 + p] : 0.f;
-    f32x4 acc[8];
-#pragma unroll
-    for (int qb = 0; qb < 8; ++qb) acc[qb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int qb = 0; qb < 4; ++qb) {
+    const int q0 = w * 64 + qb * 16;
+    const uint32_t row = qrow[q0 + (l % 16)];
 #pragma unroll
     for (int s = 0; s < KS; ++s)
-#pragma unroll
-      for (int qb = 0; qb < 8; ++qb)
-        acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][s], B[s], acc[qb], 0, 0, 0);
-    // Lane holds C[qb*16 + (l/16)*4 + r][point pl].  Branch-free pre-filter: one add
-    // and one max per element; the exact test runs only when some lane may pass.
-    float mx = -INFINITY;
-#pragma unroll
-    for (int qb = 0; qb < 8; ++qb) {
-      const float4 s4 = *reinterpret_cast<const float4 *>(&sh[qb * 16 + (l / 16) * 4]);
-      mx = fmaxf(mx, fmaxf(fmaxf(acc[qb][0] + s4.x, acc[qb][1] + s4.y),
-                           fmaxf(acc[qb][2] + s4.z, acc[qb][3] + s4.w)));
+      A[qb][s] = row != kDummy ? load8(a.qb + size_t(row) * a.ldb + s * 32 + koff) : bf16x8{};
+    const float4 s4 = *reinterpret_cast<const float4 *>(&sh[q0 + (l / 16) * 4]);
+    S[qb] = f32x4{s4.x, s4.y, s4.z, s4.w};
+  }
+  const uint32_t x0 = it.x_begin, np = it.x_count;
+  const uint32_t nsteps = (np + kVP - 1) / kVP;
+  // Staging: thread t copies bytes [32*(t%8), 32*(t%8)+32) of point t/8 of a step
+  // (KS*64 bytes per row are live; the rest of the 256 is zero padding that is never read).
+  const uint32_t sp = t / 8, sc = (t % 8) * 32;  // point in step, byte offset in row
+  const bool stage_live = sc < (uint32_t)KS * 64;
+  // Threads 0..31 also stage the half squared norm of point t of the step.
+  auto gload = [&](uint32_t step, uint4 &r0, uint4 &r1, float &hn) {
+    const uint32_t p = step * kVP + sp;
+    if (stage_live && step < nsteps && p < np) {
+      const uint4 *src = reinterpret_cast<const uint4 *>(
+          reinterpret_cast<const unsigned char *>(a.xb + size_t(x0 + p) * a.ldb) + sc);
+      r0 = src[0];
+      r1 = src[1];
+    } else {
+      r0 = make_uint4(0, 0, 0, 0);
+      r1 = make_uint4(0, 0, 0, 0);
     }
-    const bool hit = pv && mx >= 0.5f * nx;
+    const uint32_t pn = step * kVP + t;
+    hn = (t < kVP && step < nsteps && pn < np) ? 0.5f * a.xn[x0 + pn] : INFINITY;
+  };
+  auto lstore = [&](int buf, const uint4 &r0, const uint4 &r1, float hn) {
+    if (t < kVP) hring[buf][t] = hn;
+    if (!stage_live) return;
+    uint4 *dst = reinterpret_cast<uint4 *>(&ring[buf][sp * kRowB + sc]);
+    dst[0] = r0;
+    dst[1] = r1;
+  };
+  uint4 st0, st1;
+  float sn;
+  gload(0, st0, st1, sn);
+  lstore(0, st0, st1, sn);
+  gload(1, st0, st1, sn);
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    __syncthreads();
+    // Stage s+1 goes to LDS, the loads for s+2 start.
+    lstore((s + 1) % kStages, st0, st1, sn);
+    gload(s + 2, st0, st1, sn);
+    if (!wave_live) continue;
+    const unsigned char *buf = ring[s % kStages];
+    // This lane's two points of the step: pb*16 + l%16.
+    const float hx[2] = {hring[s % kStages][l % 16], hring[s % kStages][16 + l % 16]};
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int pb = 0; pb < 2; ++pb) {
+      const unsigned char *rowp = buf + (pb * 16 + (l % 16)) * kRowB + koff * 2;
+      bf16x8 B[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) B[ks] = *reinterpret_cast<const bf16x8 *>(rowp + ks * 64);
+#pragma unroll
+      for (int qb = 0; qb < 4; ++qb) acc[qb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][0], B[0], S[qb], 0, 0, 0);
+#pragma unroll
+      for (int ks = 1; ks < KS; ++ks)
+#pragma unroll
+        for (int qb = 0; qb < 4; ++qb)
+          acc[qb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][ks], B[ks], acc[qb][pb], 0, 0, 0);
+    }
+    bool hit = false;
+#pragma unroll
+    for (int pb = 0; pb < 2; ++pb) {
+      float mx = fmaxf(fmaxf(acc[0][pb][0], acc[0][pb][1]), fmaxf(acc[0][pb][2], acc[0][pb][3]));
+#pragma unroll
+      for (int qb = 1; qb < 4; ++qb)
+        mx = fmaxf(mx, fmaxf(fmaxf(acc[qb][pb][0], acc[qb][pb][1]), fmaxf(acc[qb][pb][2], acc[qb][pb][3])));
+      hit = hit || mx >= hx[pb];
+    }
     if (__any(hit)) {
 #pragma unroll
-      for (int qb = 0; qb < 8; ++qb) {
-        const int qi0 = qb * 16 + (l / 16) * 4;
-        const float4 th = *reinterpret_cast<const float4 *>(&thr[qi0]);
-        const float tv[4] = {th.x, th.y, th.z, th.w};
+      for (int pb = 0; pb < 2; ++pb) {
+        const uint32_t p = s * kVP + pb * 16 + (l % 16);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          float v = __builtin_fmaf(-2.f, acc[qb][r], nx);
-          if (hit && v <= tv[r]) {
-            uint32_t row = qrow[qi0 + r];
-            uint32_t slot = atomicAdd(&a.cand_n[row], 1u);
-            if (slot < a.cap) {
-              a.cand_d[size_t(row) * a.cap + slot] = v + a.qn[row];
-              a.cand_i[size_t(row) * a.cap + slot] = a.perm[x0 + p];
+        for (int qb = 0; qb < 4; ++qb) {
+          const int qi0 = w * 64 + qb * 16 + (l / 16) * 4;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (acc[qb][pb][r] >= hx[pb]) {
+              const uint32_t row = qrow[qi0 + r];
+              const uint32_t slot = atomicAdd(&a.cand_n[row], 1u);
+              if (slot < a.cap) {
+                a.cand_d[size_t(row) * a.cap + slot] = a.qn[row] + 2.f * (hx[pb] - (acc[qb][pb][r] - sh[qi0 + r]));
+                a.cand_i[size_t(row) * a.cap + slot] = a.perm[x0 + p];
+              }
             }
           }
         }

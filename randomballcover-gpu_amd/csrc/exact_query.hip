@@ -25,6 +25,7 @@
 #include <cmath>
 
 #include "exact.hpp"
+#include "exact_device.hpp"
 #include "exact_query.hpp"
 
 namespace rbc {
@@ -44,8 +45,25 @@ __global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uin
   float thr = FLT_MAX;
   uint32_t thri = kDummy;
   const float *row = D + size_t(q) * R;
+  // The first 32 chunks are loaded up front so that their latencies overlap.
+  constexpr int kPre = 32;
+  float pre[kPre];
+#pragma unroll
+  for (int c = 0; c < kPre; ++c) {
+    const uint32_t j = c * kWave + l;
+    pre[c] = j < R ? row[j] : FLT_MAX;
+  }
   for (uint32_t c0 = 0; c0 < R; c0 += kWave) {
-    float v = c0 + l < R ? row[c0 + l] : FLT_MAX;
+    const uint32_t ci = c0 / kWave;
+    float v;
+    if (ci < (uint32_t)kPre) {
+      v = FLT_MAX;
+#pragma unroll
+      for (int c = 0; c < kPre; ++c)
+        if ((uint32_t)c == ci) v = pre[c];
+    } else {
+      v = c0 + l < R ? row[c0 + l] : FLT_MAX;
+    }
     uint32_t id = c0 + l < R ? c0 + l : kDummy;
     // Skip the chunk when nothing in it beats the current k-th.
     bool any = pair_less(v, id, thr, thri);
@@ -141,27 +159,6 @@ __global__ void second_probe_key(const uint32_t *prop, const uint32_t *near, uin
 }
 
 // ---- 4. plan ------------------------------------------------------------------------
-// Lower bound (metric distance) on d(q, x) for every x in list j.
-__device__ __forceinline__ double list_lower_bound(double Dqj, double Dq0, double rr_j0, double psi,
-                                                   double e, bool same, int metric) {
-  double lb;
-  if (metric == 0) {
-    double dlo = sqrt(fmax(Dqj * (1.0 - e), 0.0));
-    lb = dlo - psi;
-    if (!same && rr_j0 > 0.0) {
-      // Voronoi plane between r_j and the query's nearest rep r_0: every x owned by r_j
-      // satisfies |x-r_j|^2 <= |x-r_0|^2 + 2e psi^2 (the owner test ran in FP32), so
-      // d(q,x) >= (|q-r_j|^2 - |q-r_0|^2 - 2e psi^2) / (2 |r_j - r_0|).
-      double num = Dqj * (1.0 - e) - Dq0 * (1.0 + e) - 2.0 * e * psi * psi;
-      double den = 2.0 * sqrt(rr_j0 * (1.0 + e));
-      if (num > 0.0) lb = fmax(lb, num / den);
-    }
-  } else {
-    lb = Dqj * (1.0 - e) - psi;
-  }
-  return lb * (1.0 - 1e-12);
-}
-
 // Pass A: keep bits (m x W words) and per-list query counts.  Block = 256 threads over
 // kPlanQ queries; thread t covers lists t, t+256, ...
 constexpr uint32_t kPlanQ = 32;
@@ -229,55 +226,6 @@ __global__ __launch_bounds__(256) void plan_scatter(uint32_t m, uint32_t R, uint
 
 // ---- 6. select ----------------------------------------------------------------------
 
-// Sorts (run, key64, id) triples held one per lane ascending (run, key, id), carrying
-// a float payload.
-__device__ void wave_sort_runs(uint32_t &run, double &key, uint32_t &id, float &pay) {
-  const int l = lane_id();
-  for (int k = 2; k <= kWave; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      uint32_t pr = __shfl_xor(run, j, kWave);
-      double pk = __shfl_xor(key, j, kWave);
-      uint32_t pi = __shfl_xor(id, j, kWave);
-      float pp = __shfl_xor(pay, j, kWave);
-      bool p_less = pr < run || (pr == run && (pk < key || (pk == key && pi < id)));
-      bool m_less = run < pr || (run == pr && (key < pk || (key == pk && id < pi)));
-      bool keep_min = ((l & j) == 0) == ((l & k) == 0);
-      if (keep_min ? p_less : m_less) {
-        run = pr;
-        key = pk;
-        id = pi;
-        pay = pp;
-      }
-    }
-}
-
-template <int METRIC, int NPL>
-__device__ float tree_dist_row(const float *xrow, const float (&qv)[NPL], uint32_t d) {
-  const int l = lane_id();
-  float t[NPL];
-#pragma unroll
-  for (int j = 0; j < NPL; ++j) {
-    uint32_t c = l + kWave * j;
-    t[j] = c < d ? tree_term<METRIC>(xrow[c], qv[j]) : 0.f;
-  }
-  // Strides >= 64 inside the lane: t[j] covers coordinate l + 64 j.
-#pragma unroll
-  for (int s = NPL / 2; s >= 1; s >>= 1)
-#pragma unroll
-    for (int j = 0; j < s; ++j) t[j] = __fadd_rn(t[j], t[j + s]);
-  return wave_tree_sum(t[0]);
-}
-
-template <int METRIC>
-__device__ double fp64_dist(const float *xrow, const float *qrow, uint32_t d) {
-  double acc = 0.0;
-  for (uint32_t c = 0; c < d; ++c) {
-    double t = __dsub_rn((double)xrow[c], (double)qrow[c]);
-    acc = __dadd_rn(acc, METRIC == 0 ? __dmul_rn(t, t) : fabs(t));
-  }
-  return acc;
-}
-
 template <int METRIC, int NPL>
 __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
   const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
@@ -298,54 +246,13 @@ __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
   const uint32_t np = np0 + (two ? a.Kp : 0);
   const uint32_t nc = a.cand_n ? min(a.cand_n[q], a.cap) : 0;
   const uint32_t total = np + nc;
-  uint32_t reranked = 0;
-  for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
-    uint32_t e = b0 + l;
-    uint32_t myid = kDummy;
-    if (e < np0)
-      myid = a.probe_i[size_t(q) * a.Kp + e];
-    else if (e < np)
-      myid = a.probe_i1[size_t(q) * a.Kp + (e - np0)];
-    else if (e < total)
-      myid = a.cand_i[size_t(q) * a.cap + (e - np)];
-    float myv = FLT_MAX;
-    for (int cc = 0; cc < kWave && b0 + cc < total; ++cc) {
-      uint32_t id = __shfl(myid, cc, kWave);
-      if (id == kDummy) continue;
-      const float *xrow = a.xp + size_t(a.inv[id]) * a.ldx;
-      float v = tree_dist_row<METRIC, NPL>(xrow, qv, a.d);
-      if (l == cc) myv = v;
-      ++reranked;
-    }
-    wave_bitonic_sort(myv, myid);
-    wave_merge_lower(tv, ti, myv, myid);
-  }
-  // Near-tie guard: adjacent results whose error intervals overlap are ordered by their
-  // FP64 distances (computed exactly as the FP64 oracle does).
-  const float er = (float)a.e_tree;
-  float nv = __shfl_down(tv, 1, kWave);
-  bool ov = l < kWave - 1 && tv < FLT_MAX && nv < FLT_MAX && nv * (1.f - er) <= tv * (1.f + er);
-  bool ov_prev = __shfl_up(ov, 1, kWave) && l > 0;
-  uint64_t bmask = __ballot(!ov_prev);  // run starts
-  uint64_t upto = bmask & (l == 63 ? ~0ull : ((2ull << l) - 1));
-  int start = 63 - __clzll(upto);
-  uint32_t run = (uint32_t)__popcll(upto);
-  bool amb = (ov || ov_prev) && start < (int)a.k;
-  double key = (double)tv;
-  if (__any(amb)) {
-    if (amb) {
-      const float *xrow = a.xp + size_t(a.inv[ti]) * a.ldx;
-      key = fp64_dist<METRIC>(xrow, qrow, a.d);
-    }
-    uint32_t cnt = (uint32_t)__popcll(__ballot(amb));
-    if (l == 0) atomicAdd(&a.stats[0], (unsigned long long)cnt);
-    // A near-tie run that reaches the last kept slot could continue past it.
-    if (__shfl(amb ? 1 : 0, kWave - 1, kWave) && total > (uint32_t)kWave && l == 0)
-      atomicAdd(&a.stats[2], 1ull);
-    // Reported distances stay the FP32 re-ranked values, carried with their ids.
-    wave_sort_runs(run, key, ti, tv);
-  }
-  if (l == 0) atomicAdd(&a.stats[1], (unsigned long long)reranked);
+  auto cand = [&](uint32_t e) -> uint32_t {
+    if (e < np0) return a.probe_i[size_t(q) * a.Kp + e];
+    if (e < np) return a.probe_i1[size_t(q) * a.Kp + (e - np0)];
+    return a.cand_i[size_t(q) * a.cap + (e - np)];
+  };
+  rerank_candidates<METRIC, NPL>(tv, ti, total, cand, a.xp, a.ldx, a.inv, qv, a.d);
+  near_tie_guard<METRIC>(tv, ti, total, a.xp, a.ldx, a.inv, qrow, a.d, a.k, a.e_tree, a.stats);
   if (l < (int)a.k) {
     a.out_d[size_t(q) * a.ldo + l] = tv;
     a.out_i[size_t(q) * a.ldo + l] = ti;
@@ -609,7 +516,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   DevBuf<ScanItem> sitems;
   uint32_t nsitems = 0;
   RBC_TRY(make_bucket_items(lcnt, h->list_off.get(), h->list_len.get(), R, sitems, &nsitems, ws, 16, st,
-                            bf16 ? 128u : (uint32_t)kTQ));
+                            bf16 ? kBf16ScanQ : (uint32_t)kTQ));
   tm.mark();
   // 5. scan with candidate emission; re-run with a larger buffer on overflow.
   uint32_t cap = std::max<uint32_t>({256u, 4 * k, h->cand_cap});
@@ -617,7 +524,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint32_t *cand_n = static_cast<uint32_t *>(p);
   float *cand_d = nullptr;
   uint32_t *cand_i = nullptr;
-  double fallbacks = 0;
+  double fallbacks = 0, rerank_total = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
     RBC_TRY(ws.get(37, size_t(m) * cap * 4, &p));
     cand_d = static_cast<float *>(p);
@@ -636,8 +543,10 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     RBC_HIP_TRY(hipMemcpyAsync(hn.data(), cand_n, size_t(m) * 4, hipMemcpyDeviceToHost, st));
     RBC_HIP_TRY(hipStreamSynchronize(st));
     uint32_t mx = 0;
+    rerank_total = 0;
     for (uint32_t v : hn) {
       mx = std::max(mx, v);
+      rerank_total += v;
       if (v > cap) fallbacks += 1;
     }
     h->cand_cap = std::max(h->cand_cap, mx + mx / 4);
@@ -679,9 +588,17 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_TRY(launch_select(sa, h->metric, st));
   tm.mark();
   RBC_HIP_TRY(hipStreamSynchronize(st));
+  unsigned long long hs[4];
+  RBC_HIP_TRY(hipMemcpy(hs, sdev, sizeof(hs), hipMemcpyDeviceToHost));
+  if (hs[2]) {
+    // A run of mutually unresolvable FP32 distances reached the 64th kept candidate, so
+    // the order past it is not established; refuse rather than return a possibly
+    // inexact answer.
+    set_error("%llu queries have more than 64 candidates within the FP32 error bound of each other",
+              hs[2]);
+    return RBC_ERR_LIMIT;
+  }
   if (stats) {
-    unsigned long long hs[4];
-    RBC_HIP_TRY(hipMemcpy(hs, sdev, sizeof(hs), hipMemcpyDeviceToHost));
     // Pairs evaluated by the probe and the scan, from the per-list query counts.
     std::vector<uint32_t> hq(R + 1), hl(R + 1), hq1(R + 1, 0);
     void *pq0;
@@ -697,7 +614,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     // Algorithmic bytes of the scan: each item reads its list once and its queries once
     // (BF16 rows of ldb, or FP32 rows of d); emitted candidates are not counted.
     const double row_bytes = bf16 ? 2.0 * h->ldb : 4.0 * d;
-    const double tile = bf16 ? 128.0 : (double)kTQ;
+    const double tile = bf16 ? (double)kBf16ScanQ : (double)kTQ;
     for (uint32_t j = 0; j < R; ++j) {
       const double len = h->h_list_off[j + 1] - h->h_list_off[j];
       const double cq = hl[j + 1] - hl[j];
@@ -707,7 +624,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     }
     stats->pairs_probe = pp;
     stats->pairs_scan = ps;
-    stats->pairs_rerank = (double)hs[1];
+    stats->pairs_rerank = rerank_total;
     stats->items_scan = nsitems;
     stats->lists_scanned = total_pairs;
     stats->bytes_scan = bytes;

@@ -24,7 +24,7 @@ struct SelectArgs {
   float *out_d;
   uint32_t *out_i;
   uint32_t ldo;
-  unsigned long long *stats;  // [0] FP64-guarded slots, [1] re-ranked, [2] runs at the edge
+  unsigned long long *stats;  // [0] FP64-guarded slots, [2] near-tie runs reaching the last kept slot
 };
 
 int launch_select(const SelectArgs &a, int metric, hipStream_t st);
@@ -35,6 +35,8 @@ int bf16_prepare_queries(rbc_exact *h, const float *q, size_t ldq, uint32_t m, u
 int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t *qorder,
                const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_kth,
                hipStream_t st);
+// Queries per work item of the BF16 scan (its items must be cut at this size).
+constexpr uint32_t kBf16ScanQ = 256;
 // Relative error constant of the BF16 GEMM distance: gamma_K + 6u', u' = 2^-23.
 double bf16_gemm_err(const rbc_exact *h);
 int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe, const double *gam,

@@ -53,8 +53,8 @@ struct TileCtx {
   int qg, pg;  // this thread's 4 queries: 4*qg.., 4 points: 4*pg..
 };
 
-template <int METRIC, class Epi>
-__device__ void tile_run(const TileSrc &src, const ScanItem &it, Epi &epi, float *smem) {
+// Sets up the LDS carve-up of a tile and loads its queries (transposed).
+__device__ inline TileCtx tile_setup(const TileSrc &src, const ScanItem &it, float *smem) {
   const int t = threadIdx.x;
   float *qs = smem;
   float *xs = qs + size_t(src.dpad) * kTQ;
@@ -67,7 +67,7 @@ __device__ void tile_run(const TileSrc &src, const ScanItem &it, Epi &epi, float
   ctx.epi = reinterpret_cast<unsigned char *>(pid_s + kTP);
   ctx.qg = t & 15;
   ctx.pg = t >> 4;
-
+  __syncthreads();  // the previous user of this LDS is done
   if (t < kTQ) {
     uint32_t row = kDummy;
     if ((uint32_t)t < it.q_count) row = src.qids ? src.qids[it.q_begin + t] : it.q_begin + t;
@@ -87,9 +87,19 @@ __device__ void tile_run(const TileSrc &src, const ScanItem &it, Epi &epi, float
     qs[(c + 2) * kTQ + i] = c + 2 < src.d ? v.z : 0.f;
     qs[(c + 3) * kTQ + i] = c + 3 < src.d ? v.w : 0.f;
   }
-  epi.begin(ctx);
+  return ctx;
+}
 
-  for (uint32_t p0 = 0; p0 < it.x_count; p0 += kTP) {
+// Streams points [x_begin, x_begin + x_count) of src past the loaded queries, handing
+// every 64-point block of distances to epi.step.
+template <int METRIC, class Epi>
+__device__ void tile_steps(const TileSrc &src, uint32_t x_begin, uint32_t x_count, Epi &epi, TileCtx &ctx,
+                           float *smem) {
+  const int t = threadIdx.x;
+  float *qs = smem;
+  float *xs = qs + size_t(src.dpad) * kTQ;
+  uint32_t *pid_s = ctx.pid_s;
+  for (uint32_t p0 = 0; p0 < x_count; p0 += kTP) {
     float acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -102,8 +112,8 @@ __device__ void tile_run(const TileSrc &src, const ScanItem &it, Epi &epi, float
       if (c0 == 0 && t < kTP) {
         uint32_t p = p0 + t;
         uint32_t id = kDummy;
-        if (p < it.x_count) {
-          uint32_t row = src.xmap ? src.xmap[it.x_begin + p] : it.x_begin + p;
+        if (p < x_count) {
+          uint32_t row = src.xmap ? src.xmap[x_begin + p] : x_begin + p;
           id = src.xid ? src.xid[row] : row;
         }
         pid_s[t] = id;
@@ -112,8 +122,8 @@ __device__ void tile_run(const TileSrc &src, const ScanItem &it, Epi &epi, float
         uint32_t p = idx % kTP, c4 = idx / kTP;
         uint32_t c = c0 + 4 * c4;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (p0 + p < it.x_count) {
-          uint32_t row = src.xmap ? src.xmap[it.x_begin + p0 + p] : it.x_begin + p0 + p;
+        if (p0 + p < x_count) {
+          uint32_t row = src.xmap ? src.xmap[x_begin + p0 + p] : x_begin + p0 + p;
           v = *reinterpret_cast<const float4 *>(src.x + size_t(row) * src.ldx + c);
         }
         xs[(4 * c4 + 0) * kTP + p] = c + 0 < src.d ? v.x : 0.f;
@@ -137,6 +147,13 @@ __device__ void tile_run(const TileSrc &src, const ScanItem &it, Epi &epi, float
     }
     epi.step(ctx, acc, p0);
   }
+}
+
+template <int METRIC, class Epi>
+__device__ void tile_run(const TileSrc &src, const ScanItem &it, Epi &epi, float *smem) {
+  TileCtx ctx = tile_setup(src, it, smem);
+  epi.begin(ctx);
+  tile_steps<METRIC>(src, it.x_begin, it.x_count, epi, ctx, smem);
   epi.end(ctx);
 }
 

@@ -97,6 +97,9 @@ _SIGS = {
     "rbc_mlp_loss_grad": (_int, [_vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float), _vp]),
     "rbc_mlp_logits": (_int, [_vp, _vp, _u32, _u32, _vp]),
     "rbc_mlp_train": (_int, [_vp, _vp, _u32, _u32, _vp, _int, _u32, _u32, ctypes.c_float, _u64, _vp]),
+    "rbc_batched_build": (_int, [_u32, _vp, _vp, _vp, _vp, _vp, _int, _u64, _int, ctypes.POINTER(_vp)]),
+    "rbc_batched_kquery": (_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _int, _vp, _vp]),
+    "rbc_batched_destroy": (_int, [_vp]),
 }
 
 
@@ -106,6 +109,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RBCError(-1, f"{LIB_PATH} is not built")
+        # torch bundles its own libamdhip64.so.7.  Both it and this library resolve the
+        # HIP runtime by that soname, so whichever loads first serves the whole process.
+        # Loading torch first keeps one runtime (torch's) for torch tensors and our kernels;
+        # the other order leaves torch unable to initialise its devices.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             if hasattr(L, name):
@@ -349,6 +357,76 @@ class ProposalMLP:
     def close(self):
         if self.h:
             _check(lib().rbc_mlp_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- batched exact ----
+
+class BatchedStats(ctypes.Structure):
+    _fields_ = [("ms_graph", ctypes.c_double), ("tile_queries", _u32), ("graph_replayed", _u32),
+                ("uncertified", _u32), ("pad", _u32), ("tune_ms", ctypes.c_double * 3),
+                ("guard_fp64", ctypes.c_double)]
+
+
+def _ptr_array(ptrs, ctype=_vp):
+    arr = (ctype * len(ptrs))(*ptrs)
+    return arr
+
+
+class BatchedRBC:
+    """rbc_batched_*: many independent exact indexes queried in one persistent launch."""
+
+    def __init__(self, xs, num_reps, metric=L2, seed=0):
+        self._xs = [np.ascontiguousarray(x, np.float32) for x in xs]
+        c = len(self._xs)
+        n = np.array([x.shape[0] for x in self._xs], np.uint32)
+        d = np.array([x.shape[1] for x in self._xs], np.uint32)
+        r = np.ascontiguousarray(num_reps, np.uint32)
+        self.h = _vp()
+        _check(lib().rbc_batched_build(c, _ptr_array([x.ctypes.data for x in self._xs]), n.ctypes.data,
+                                       d.ctypes.data, d.ctypes.data, r.ctypes.data, metric, seed,
+                                       MEM_HOST, ctypes.byref(self.h)))
+        self.count, self.dims = c, d
+
+    def kquery(self, qs, k):
+        qs = [np.ascontiguousarray(q, np.float32) for q in qs]
+        m = np.array([q.shape[0] for q in qs], np.uint32)
+        ldq = np.array([q.shape[1] for q in qs], np.uint32)
+        ids = [np.empty((q.shape[0], k), np.uint32) for q in qs]
+        ds = [np.empty((q.shape[0], k), np.float32) for q in qs]
+        ldo = np.full(len(qs), k, np.uint32)
+        st = BatchedStats()
+        _check(lib().rbc_batched_kquery(self.h, _ptr_array([q.ctypes.data for q in qs]), m.ctypes.data,
+                                        ldq.ctypes.data, k, _ptr_array([i.ctypes.data for i in ids]),
+                                        _ptr_array([x.ctypes.data for x in ds]), ldo.ctypes.data,
+                                        MEM_HOST, None, ctypes.byref(st)))
+        stats = {n: getattr(st, n) for n, _ in st._fields_ if n != "tune_ms"}
+        stats["tune_ms"] = list(st.tune_ms)
+        return ids, ds, stats
+
+    def kquery_device(self, qs, k, ids, ds):
+        """qs, ids, ds: lists of device tensors; returns the stats dict."""
+        m = np.array([q.shape[0] for q in qs], np.uint32)
+        ldq = np.array([q.stride(0) for q in qs], np.uint32)
+        ldo = np.array([i.stride(0) for i in ids], np.uint32)
+        st = BatchedStats()
+        _check(lib().rbc_batched_kquery(self.h, _ptr_array([q.data_ptr() for q in qs]), m.ctypes.data,
+                                        ldq.ctypes.data, k, _ptr_array([i.data_ptr() for i in ids]),
+                                        _ptr_array([x.data_ptr() for x in ds]), ldo.ctypes.data,
+                                        MEM_DEVICE, None, ctypes.byref(st)))
+        stats = {n: getattr(st, n) for n, _ in st._fields_ if n != "tune_ms"}
+        stats["tune_ms"] = list(st.tune_ms)
+        return stats
+
+    def close(self):
+        if self.h:
+            _check(lib().rbc_batched_destroy(self.h))
             self.h = _vp()
 
     def __del__(self):
