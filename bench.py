@@ -3,20 +3,36 @@
 Workload (the metric's configuration): n = 4,194,304 points, d = 128, x = A z + 0.01 e
 with z ~ N(0, I_16) (the ``lowrank16_n4M_d128`` generator), m = 131,072 queries from
 the same model, 2,048 representatives, k = 10.  One step = one exact k-query of all m
-queries with inputs and outputs resident in HBM (build excluded, timed separately).
-The timed path is the BF16 MFMA filter + exact FP32 re-rank; the plain FP32 exact RBC
-is timed once on the same index for the speed-up, and the CPU baseline (the oracle's
-FP32 brute force, the reference's CPU arithmetic) on a bounded query sample.
+queries with inputs and outputs resident in HBM (build and proposal training excluded,
+timed separately).  The timed path is the BF16 MFMA filter + exact FP32 re-rank with the
+learned list proposal (MLP d -> 256 -> 256 -> R, FP16, trained on the GPU on 2^20
+(query, owner list of its exact nearest neighbour) pairs produced by the exact search
+itself).  Beside it, once each on the same index and queries: the same search without the
+proposal, and the plain FP32 exact RBC.
 
-Prints one JSON line (rank 0).  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+``roofline`` is for the dominant kernel, the BF16 list scan: FLOPs the scan evaluates
+(2 d per scanned (query, point) pair, counted on the device) over its hipEvent-timed
+duration on the query stream.  ``traffic`` is its HBM bytes per launch from two
+``rocprofv3 --pmc`` passes (FETCH_SIZE, WRITE_SIZE) of a child run of the same query,
+each scaled by what the counter reads for a 1 GiB device copy.
+
+``cpu_baseline`` is the oracle's FP32 brute force (the reference's bruteK arithmetic) on
+a bounded query sample, on the host's cores.
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import importlib.util
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -25,7 +41,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "randomballcover-gpu_amd")
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+SCAN_KERNEL = "bf16_scan_kernel"
 
 
 def _load(name, path):
@@ -36,69 +54,198 @@ def _load(name, path):
     return mod
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_calibration(mib):
+    """One device-to-device copy of ``mib`` MiB (reference kernel for the HBM counters)."""
+    import torch
+    a = torch.empty(mib << 20, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": a.numel()}))
+
+
+def train_proposal(rbc, datagen, idx, workload, d, num_reps, count, epochs, seed=0):
+    """Trains the list proposal on (query, owner list of the exact 1-NN) pairs."""
+    qt = datagen.training_queries(workload, count)
+    nn, _ = idx.kquery(qt, 1, precision=rbc.PREC_BF16)
+    owner = idx.lists()["owner"]
+    labels = owner[nn[:, 0]]
+    net = rbc.ProposalMLP(d, num_reps, seed=seed)
+    losses = net.train(qt, labels, epochs=epochs, batch=1024, lr=2e-3, seed=seed)
+    acc = float(np.mean(np.argmax(net.logits(qt[:4096]), 1) == labels[:4096]))
+    return net, losses, acc
+
+
+def build_index(args, rbc, datagen, torch):
+    x, q, num_reps, k = datagen.workload(args.workload, args.n, args.m)
+    dev = torch.device("cuda:0")
+    xd = torch.from_numpy(x).to(dev)
+    qd = torch.from_numpy(q).to(dev)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    idx = rbc.ExactRBC(xd, num_reps, seed=0, device=True)
+    torch.cuda.synchronize()
+    return x, q, num_reps, k, xd, qd, idx, time.time() - t0
+
+
+def pmc_child(args):
+    """Runs under rocprofv3 --pmc: the measured query (with the same proposal weights),
+    twice, so that the last scan launch has a warm candidate-capacity estimate."""
+    import torch
+    rbc = _load("rbc_ctypes", os.path.join(PKG, "rbc_ctypes.py"))
+    datagen = _load("rbc_datagen", os.path.join(PKG, "datagen.py"))
+    x, q, num_reps, k, xd, qd, idx, _ = build_index(args, rbc, datagen, torch)
+    net = None
+    if args.mlp_params:
+        net = rbc.ProposalMLP(x.shape[1], num_reps, seed=0)
+        net.set_params(np.load(args.mlp_params))
+        idx.set_proposal(net)
+    ids = torch.empty((q.shape[0], k), dtype=torch.int32, device="cuda:0")
+    ds = torch.empty((q.shape[0], k), dtype=torch.float32, device="cuda:0")
+    for _ in range(2):
+        idx.kquery_device(qd, k, ids, ds, precision=rbc.PREC_BF16)
+    torch.cuda.synchronize()
+    idx.close()
+
+
+def _pmc_pass(counter, child_args, outdir, timeout):
+    """One rocprofv3 counter pass of a child bench process; returns the CSV rows."""
+    prof = shutil.which("rocprofv3")
+    d = os.path.join(outdir, counter.lower())
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__)] + child_args
+    env = dict(os.environ, TMPDIR=tempfile.gettempdir())
+    r = subprocess.run(cmd, cwd=tempfile.gettempdir(), env=env, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.PIPE, timeout=timeout)
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter file (rc {r.returncode}): "
+                           f"{r.stderr.decode(errors='replace')[-400:]}")
+    with open(files[0]) as f:
+        return [row for row in csv.DictReader(f) if row["Counter_Name"] == counter]
+
+
+def measure_traffic(args, mlp_params, timeout=240):
+    """HBM bytes per launch of the scan kernel: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) of its last launch, each scaled by the counter's reading for a 1 GiB copy."""
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    out = tempfile.mkdtemp(prefix="rbc_pmc_")
+    child = ["--pmc-child", "--workload", args.workload]
+    if args.n:
+        child += ["--n", str(args.n)]
+    if args.m:
+        child += ["--m", str(args.m)]
+    if mlp_params:
+        child += ["--mlp-params", mlp_params]
+    res = {}
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            cal = _pmc_pass(counter, ["--copy-calibration", "1024"], os.path.join(out, "cal"), timeout)
+            cal_kib = max(float(r["Counter_Value"]) for r in cal)
+            factor = (1024.0 * 1024.0) / cal_kib  # 1 GiB = 1,048,576 KiB moved each way
+            rows = [r for r in _pmc_pass(counter, child, out, timeout) if SCAN_KERNEL in r["Kernel_Name"]]
+            if not rows:
+                return None, f"no {SCAN_KERNEL} dispatch in the {counter} pass"
+            last = max(rows, key=lambda r: int(r["Dispatch_Id"]))
+            res[counter] = {"kib": float(last["Counter_Value"]), "copy_factor": round(factor, 4)}
+            log(f"pmc {counter}: {float(last['Counter_Value']):.0f} KiB x {factor:.3f} (copy calibration)")
+    except subprocess.TimeoutExpired:
+        return None, "rocprofv3 pass timed out"
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+    tb = sum(v["kib"] * v["copy_factor"] * 1024.0 for v in res.values())
+    return tb, res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="lowrank16_n4M_d128")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--m", type=int, default=None)
     ap.add_argument("--precision", default="bf16", choices=["bf16", "fp32"])
+    ap.add_argument("--proposal", default="mlp", choices=["mlp", "none"],
+                    help="learned list proposal in the timed path (BF16 only)")
+    ap.add_argument("--train-queries", type=int, default=1 << 20)
+    ap.add_argument("--train-epochs", type=int, default=3)
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
+    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--check-sample", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--mlp-params", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.copy_calibration:
-        import torch
-        a = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda:0")
-        b = torch.empty_like(a)
-        a.fill_(1)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": a.numel()}))
+        copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build",
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              f"(queries shard across GPUs with no exchange; see the sharding design)",
               file=sys.stderr)
         sys.exit(2)
+    if args.pmc_child:
+        pmc_child(args)
+        return
 
     import torch
 
     rbc = _load("rbc_ctypes", os.path.join(PKG, "rbc_ctypes.py"))
     datagen = _load("rbc_datagen", os.path.join(PKG, "datagen.py"))
     rbc.lib()
-    dev = torch.device("cuda:0")
 
     t0 = time.time()
-    x, q, num_reps, k = datagen.workload(args.workload, args.n, args.m)
+    x, q, num_reps, k, xd, qd, idx, build_s = build_index(args, rbc, datagen, torch)
     n, d = x.shape
     m = q.shape[0]
-    gen_s = time.time() - t0
-    print(f"[bench] data {args.workload}: n={n} d={d} m={m} reps={num_reps} k={k} "
-          f"(generated in {gen_s:.1f}s)", file=sys.stderr, flush=True)
-
-    xd = torch.from_numpy(x).to(dev)
-    qd = torch.from_numpy(q).to(dev)
+    info = idx.info()
+    log(f"{args.workload}: n={n} d={d} m={m} reps={num_reps} k={k}; data+build {time.time()-t0:.1f}s "
+        f"(build {build_s:.2f}s), lists {info['min_list']}..{info['max_list']}")
+    prec = rbc.PREC_BF16 if args.precision == "bf16" else rbc.PREC_FP32
+    dev = torch.device("cuda:0")
     ids = torch.empty((m, k), dtype=torch.int32, device=dev)
     ds = torch.empty((m, k), dtype=torch.float32, device=dev)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    idx = rbc.ExactRBC(xd, num_reps, seed=0, device=True)
-    torch.cuda.synchronize()
-    build_s = time.time() - t0
-    info = idx.info()
-    print(f"[bench] build {build_s:.2f}s, lists {info['min_list']}..{info['max_list']}",
-          file=sys.stderr, flush=True)
 
-    prec = rbc.PREC_BF16 if args.precision == "bf16" else rbc.PREC_FP32
+    extra = {}
+    net = None
+    mlp_params = None
+    use_mlp = args.proposal == "mlp" and prec == rbc.PREC_BF16
+    if use_mlp:
+        t0 = time.time()
+        net, losses, acc = train_proposal(rbc, datagen, idx, args.workload, d, num_reps,
+                                          args.train_queries, args.train_epochs)
+        train_s = time.time() - t0
+        log(f"proposal trained on {args.train_queries} pairs in {train_s:.1f}s: loss "
+            f"{losses[0]:.3f} -> {losses[-1]:.3f}, top-1 accuracy {acc:.3f}")
+        extra["proposal"] = {"train_pairs": args.train_queries, "epochs": args.train_epochs,
+                             "train_s": round(train_s, 2), "loss_first": round(float(losses[0]), 4),
+                             "loss_last": round(float(losses[-1]), 4), "top1_train_acc": round(acc, 4),
+                             "params": net.num_params}
+        # Same search without the proposal, for the learned gain (one untimed-warm step).
+        idx.kquery_device(qd, k, ids, ds, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        s_plain = idx.kquery_device(qd, k, ids, ds, precision=prec)
+        torch.cuda.synchronize()
+        w_plain = time.time() - t0
+        ref_ids, ref_ds = ids.clone(), ds.clone()
+        idx.set_proposal(net)
+
     for _ in range(args.warmup):
         idx.kquery_device(qd, k, ids, ds, precision=prec)
     torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
+    log("warmup done")
     stats = []
     t0 = time.time()
     for _ in range(args.steps):
@@ -108,42 +255,72 @@ def main():
     ms_step = 1e3 * wall / args.steps
     qps = m * args.steps / wall
     st = stats[-1]
-    print(f"[bench] {args.precision}: {ms_step:.2f} ms/step, {qps:,.0f} q/s; stages "
-          f"qreps {st['ms_qreps']:.2f} probe {st['ms_probe']:.2f} plan {st['ms_plan']:.2f} "
-          f"scan {st['ms_scan']:.2f} select {st['ms_select']:.2f} ms", file=sys.stderr, flush=True)
+    log(f"{args.precision}{'+mlp' if use_mlp else ''}: {ms_step:.2f} ms/step, {qps:,.0f} q/s; stages "
+        f"qreps {st['ms_qreps']:.2f} probe {st['ms_probe']:.2f} plan {st['ms_plan']:.2f} "
+        f"scan {st['ms_scan']:.2f} select {st['ms_select']:.2f} ms")
+    if use_mlp:
+        same = bool(torch.equal(ref_ids, ids) and torch.equal(ref_ds, ds))
+        extra["without_proposal"] = {
+            "qps": round(m / w_plain, 1), "ms": round(1e3 * w_plain, 3),
+            "pairs_scan": s_plain["pairs_scan"], "pairs_probe": s_plain["pairs_probe"],
+            "scan_ms": round(s_plain["ms_scan"], 3), "same_results": same}
+        extra["with_proposal"] = {"pairs_scan": st["pairs_scan"], "pairs_probe": st["pairs_probe"]}
+        log(f"without proposal: {1e3*w_plain:.2f} ms ({m/w_plain:,.0f} q/s), pairs scanned "
+            f"{s_plain['pairs_scan']:.4e} -> {st['pairs_scan']:.4e} with it; identical results: {same}")
 
-    # Roofline of the dominant kernel (the list scan): algorithmic FLOPs and bytes per
-    # launch over its hipEvent-timed duration.
+    # Roofline of the dominant kernel (the list scan): algorithmic FLOPs per launch over
+    # its hipEvent-timed duration.
     scan_ms = float(np.mean([s["ms_scan"] for s in stats]))
     flops = 2.0 * d * st["pairs_scan"]
     tflops = flops / (scan_ms * 1e-3) / 1e12
-    gbs = st["bytes_scan"] / (scan_ms * 1e-3) / 1e9
     if prec == rbc.PREC_BF16:
         roof = {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-                "traffic": None, "kernel": "bf16_scan_kernel",
-                "flops_per_launch": flops, "alg_bytes_per_launch": st["bytes_scan"],
-                "alg_GBps": round(gbs, 1), "kernel_ms": round(scan_ms, 3)}
-    else:
-        peak = 157.3
-        roof = {"bound": "valu", "achieved": round(tflops * 1.5, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(tflops * 1.5 / peak, 4), "traffic": None, "kernel": "tile_kernel<EpiEmit>",
+                "traffic": None, "kernel": SCAN_KERNEL + "<4>",
+                "flops_per_launch": flops, "pairs_per_launch": st["pairs_scan"],
+                "alg_bytes_per_launch": st["bytes_scan"],
+                "alg_GBps": round(st["bytes_scan"] / (scan_ms * 1e-3) / 1e9, 1),
                 "kernel_ms": round(scan_ms, 3)}
+    else:
+        roof = {"bound": "valu", "achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "tile_kernel<EpiEmit>", "kernel_ms": round(scan_ms, 3)}
 
-    extra = {}
     if not args.no_fp32 and prec == rbc.PREC_BF16:
-        torch.cuda.synchronize()
-        t0 = time.time()
+        idx.set_proposal(None)
         ids32 = torch.empty_like(ids)
         ds32 = torch.empty_like(ds)
+        torch.cuda.synchronize()
+        t0 = time.time()
         s32 = idx.kquery_device(qd, k, ids32, ds32, precision=rbc.PREC_FP32)
         torch.cuda.synchronize()
         w32 = time.time() - t0
         extra["fp32_plain_qps"] = round(m / w32, 1)
         extra["speedup_vs_fp32_plain"] = round(qps / (m / w32), 2)
         extra["fp32_matches_bf16"] = bool(torch.equal(ids32, ids) and torch.equal(ds32, ds))
-        print(f"[bench] fp32 plain: {w32*1e3:.1f} ms ({m/w32:,.0f} q/s), scan {s32['ms_scan']:.1f} ms",
-              file=sys.stderr, flush=True)
+        log(f"fp32 plain: {w32*1e3:.1f} ms ({m/w32:,.0f} q/s), scan {s32['ms_scan']:.1f} ms")
+        if net is not None:
+            idx.set_proposal(net)
+
+    if prec == rbc.PREC_BF16 and not args.no_traffic:
+        tmp = None
+        if net is not None:
+            fd, tmp = tempfile.mkstemp(suffix=".npy", prefix="rbc_mlp_")
+            os.close(fd)
+            np.save(tmp, net.get_params())
+        try:
+            tb, detail = measure_traffic(args, tmp)
+        finally:
+            if tmp:
+                os.unlink(tmp)
+        if tb is not None:
+            roof["traffic"] = tb
+            roof["traffic_counters"] = detail
+            roof["traffic_over_alg"] = round(tb / st["bytes_scan"], 3)
+            log(f"scan HBM traffic {tb/1e9:.2f} GB/launch ({tb/st['bytes_scan']:.2f}x algorithmic)")
+        else:
+            roof["traffic_note"] = detail
+            log(f"traffic not measured: {detail}")
 
     cpu = None
     if not args.no_cpu:
@@ -153,21 +330,23 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         ns = min(args.cpu_sample, m)
         t0 = time.time()
-        oi32, od32 = oracle.brute_k_f32(x, q[:ns], k, 0, threads)
+        oracle.brute_k_f32(x, q[:ns], k, 0, threads)
         cs = time.time() - t0
         cpu = {"value": round(ns / cs, 2), "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": f"first {ns} of {m} queries, FP32 brute force over all {n} points "
                          f"(bruteK arithmetic, brute.cu:65-88), {cs:.1f}s"}
-        # Checker on the same sample: FP64 oracle vs the GPU result.
-        oi, od = oracle.brute_k_f64(x, q[:ns], k, 0, threads)
-        gi = ids[:ns].cpu().numpy().astype(np.uint32)
-        gd = ds[:ns].cpu().numpy()
-        recall = float(np.mean([len(set(gi[i]) & set(oi[i])) / k for i in range(ns)]))
+        log(f"cpu baseline {ns/cs:.2f} q/s on {threads} threads ({cs:.1f}s)")
+        # Checker on a sample: FP64 oracle vs the GPU result.
+        nc = min(args.check_sample, m)
+        oi, od = oracle.brute_k_f64(x, q[:nc], k, 0, threads)
+        gi = ids[:nc].cpu().numpy().astype(np.uint32)
+        gd = ds[:nc].cpu().numpy()
+        recall = float(np.mean([len(set(gi[i]) & set(oi[i])) / k for i in range(nc)]))
         rel = float(np.max(np.abs(gd.astype(np.float64) - od) / np.maximum(od, 1e-300)))
-        extra.update({"recall_at_k_sample": recall, "ids_exact_sample": bool((gi == oi).all()),
-                      "max_rel_err_vs_fp64_sample": rel})
-        print(f"[bench] cpu baseline {ns/cs:.2f} q/s on {threads} threads; recall {recall}, "
-              f"max rel err {rel:.2e}", file=sys.stderr, flush=True)
+        extra.update({"check_sample": nc, "recall_at_k_sample": recall,
+                      "ids_exact_sample": bool((gi == oi).all()), "max_rel_err_vs_fp64_sample": rel})
+        log(f"FP64 check on {nc} queries: recall {recall}, ids exact {bool((gi == oi).all())}, "
+            f"max rel err {rel:.2e}")
 
     line = {
         "metric": "exact k-NN queries/sec, n=4M d=128 k=10 recall=1.0 (BF16+learned vs FP32 plain)",
@@ -183,7 +362,8 @@ def main():
         "dtype": "bf16" if prec == rbc.PREC_BF16 else "fp32",
         "data": "synthetic",
         "config": {"workload": args.workload, "n": n, "d": d, "m": m, "k": k, "num_reps": num_reps,
-                   "precision": args.precision, "build_s": round(build_s, 2),
+                   "precision": args.precision, "proposal": "mlp" if use_mlp else "none",
+                   "build_s": round(build_s, 2),
                    "pairs_scan_frac_of_brute": round((st["pairs_scan"] + st["pairs_probe"]) / (n * m), 4)},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -110,3 +110,10 @@ def workload(name: str, n: int | None = None, m: int | None = None):
     n = n0 if n is None else n
     m = m0 if m is None else m
     return gen(n, d, 0), gen(m, d, 1), reps, k
+
+
+def training_queries(name: str, count: int):
+    """Training queries of a named workload (seed 2): same model, disjoint from the
+    database (seed 0) and the measured queries (seed 1)."""
+    gen, d = WORKLOADS[name][:2]
+    return gen(count, d, 2)
