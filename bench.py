@@ -72,9 +72,16 @@ def copy_calibration(mib):
 
 def train_proposal(rbc, datagen, idx, workload, d, num_reps, count, epochs, seed=0):
     """Trains the list proposal on (query, owner list of the exact 1-NN) pairs."""
+    import torch
     qt = datagen.training_queries(workload, count)
-    nn, _ = idx.kquery(qt, 1, precision=rbc.PREC_BF16)
+    qtd = torch.from_numpy(qt).to("cuda:0")
+    nnd = torch.empty((count, 1), dtype=torch.int32, device="cuda:0")
+    dnd = torch.empty((count, 1), dtype=torch.float32, device="cuda:0")
+    idx.kquery_device(qtd, 1, nnd, dnd, precision=rbc.PREC_BF16)
+    nn = nnd.cpu().numpy().view(np.uint32)
     owner = idx.lists()["owner"]
+    if (nn >= owner.shape[0]).any():
+        raise RuntimeError(f"exact search returned {(nn >= owner.shape[0]).sum()} invalid ids")
     labels = owner[nn[:, 0]]
     net = rbc.ProposalMLP(d, num_reps, seed=seed)
     losses = net.train(qt, labels, epochs=epochs, batch=1024, lr=2e-3, seed=seed)

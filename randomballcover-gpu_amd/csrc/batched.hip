@@ -473,8 +473,8 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
     RBC_TRY(oi.alloc(std::max<size_t>(1, ooff[count])));
     for (uint32_t i = 0; i < count; ++i)
       if (m[i])
-        RBC_HIP_TRY(hipMemcpy2DAsync(b->qstage.get() + qoff[i], pad16(b->idx[i]->d) * 4, q[i], size_t(ldq[i]) * 4,
-                                     size_t(b->idx[i]->d) * 4, m[i], hipMemcpyHostToDevice, st));
+        RBC_TRY(rows_in(b->qstage.get() + qoff[i], pad16(b->idx[i]->d) * 4, q[i], size_t(ldq[i]) * 4,
+                        size_t(b->idx[i]->d) * 4, m[i], RBC_MEM_HOST, st));
   }
   std::vector<BInst> hi(count);
   uint32_t mtot = 0;
@@ -627,10 +627,10 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
   if (!in_place) {
     for (uint32_t i = 0; i < count; ++i)
       if (m[i]) {
-        RBC_HIP_TRY(hipMemcpy2DAsync(nn_dists[i], size_t(ldo[i]) * 4, od.get() + ooff[i], size_t(k) * 4, size_t(k) * 4,
-                                     m[i], hipMemcpyDeviceToHost, st));
-        RBC_HIP_TRY(hipMemcpy2DAsync(nn_ids[i], size_t(ldo[i]) * 4, oi.get() + ooff[i], size_t(k) * 4, size_t(k) * 4,
-                                     m[i], hipMemcpyDeviceToHost, st));
+        RBC_TRY(rows_out(nn_dists[i], size_t(ldo[i]) * 4, od.get() + ooff[i], size_t(k) * 4, size_t(k) * 4, m[i],
+                         RBC_MEM_HOST, st));
+        RBC_TRY(rows_out(nn_ids[i], size_t(ldo[i]) * 4, oi.get() + ooff[i], size_t(k) * 4, size_t(k) * 4, m[i],
+                         RBC_MEM_HOST, st));
       }
   }
   RBC_HIP_TRY(hipStreamSynchronize(st));

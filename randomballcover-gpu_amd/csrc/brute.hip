@@ -147,10 +147,8 @@ int rbc_brute_k(const float *x, uint32_t n, uint32_t ldx, const float *q, uint32
   RBC_TRY(od.alloc(size_t(m) * k));
   RBC_TRY(oi.alloc(size_t(m) * k));
   RBC_TRY(brute_k_device(X.p, X.ld, n, Q.p, Q.ld, m, d, metric, k, od.get(), oi.get(), k, st));
-  RBC_HIP_TRY(hipMemcpy2D(nn_dists, size_t(ldo) * 4, od.get(), size_t(k) * 4, size_t(k) * 4, m,
-                          hipMemcpyDeviceToHost));
-  RBC_HIP_TRY(hipMemcpy2D(nn_ids, size_t(ldo) * 4, oi.get(), size_t(k) * 4, size_t(k) * 4, m,
-                          hipMemcpyDeviceToHost));
+  RBC_TRY(rows_out(nn_dists, size_t(ldo) * 4, od.get(), size_t(k) * 4, size_t(k) * 4, m, mem, st));
+  RBC_TRY(rows_out(nn_ids, size_t(ldo) * 4, oi.get(), size_t(k) * 4, size_t(k) * 4, m, mem, st));
   return RBC_OK;
 }
 

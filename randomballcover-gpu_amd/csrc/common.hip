@@ -78,14 +78,52 @@ int DevMat::stage(const float *x, uint32_t rows, uint32_t d, uint32_t ldx, int m
   }
   ld = pad16(d);
   RBC_TRY(own.alloc(size_t(rows) * ld));
-  RBC_TRY(fill_f32(own.get(), 0.f, size_t(rows) * ld, st));
+  if (ld != d) RBC_TRY(fill_f32(own.get(), 0.f, size_t(rows) * ld, st));
   if (rows)
-    RBC_HIP_TRY(hipMemcpy2DAsync(own.get(), ld * sizeof(float), x, size_t(ldx) * sizeof(float),
-                                 size_t(d) * sizeof(float), rows,
-                                 mem == RBC_MEM_DEVICE ? hipMemcpyDeviceToDevice
-                                                       : hipMemcpyHostToDevice,
-                                 st));
+    RBC_TRY(rows_in(own.get(), ld * sizeof(float), x, size_t(ldx) * sizeof(float), size_t(d) * sizeof(float),
+                    rows, mem, st));
   p = own.get();
+  return RBC_OK;
+}
+
+int rows_in(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows, int mem,
+            hipStream_t st) {
+  if (!rows || !width) return RBC_OK;
+  const size_t span = (rows - 1) * spitch + width;  // bytes the source rows cover
+  if (mem == RBC_MEM_DEVICE) {
+    RBC_HIP_TRY(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToDevice, st));
+    return RBC_OK;
+  }
+  if (spitch == dpitch || rows == 1) {
+    RBC_HIP_TRY(hipMemcpyAsync(dst, src, span, hipMemcpyHostToDevice, st));
+    RBC_HIP_TRY(hipStreamSynchronize(st));
+    return RBC_OK;
+  }
+  DevBuf<unsigned char> tmp;
+  RBC_TRY(tmp.alloc(span));
+  RBC_HIP_TRY(hipMemcpyAsync(tmp.get(), src, span, hipMemcpyHostToDevice, st));
+  RBC_HIP_TRY(hipMemcpy2DAsync(dst, dpitch, tmp.get(), spitch, width, rows, hipMemcpyDeviceToDevice, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  return RBC_OK;
+}
+
+int rows_out(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows, int mem,
+             hipStream_t st) {
+  if (!rows || !width) return RBC_OK;
+  if (mem == RBC_MEM_DEVICE) {
+    RBC_HIP_TRY(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, hipMemcpyDeviceToDevice, st));
+    return RBC_OK;
+  }
+  if (spitch == dpitch || rows == 1) {
+    RBC_HIP_TRY(hipMemcpyAsync(dst, src, (rows - 1) * spitch + width, hipMemcpyDeviceToHost, st));
+    RBC_HIP_TRY(hipStreamSynchronize(st));
+    return RBC_OK;
+  }
+  std::vector<unsigned char> tmp((rows - 1) * spitch + width);
+  RBC_HIP_TRY(hipMemcpyAsync(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  unsigned char *o = static_cast<unsigned char *>(dst);
+  for (size_t r = 0; r < rows; ++r) std::memcpy(o + r * dpitch, tmp.data() + r * spitch, width);
   return RBC_OK;
 }
 

@@ -679,9 +679,7 @@ extern "C" int rbc_exact_kquery(rbc_exact *h, const float *q, uint32_t m, uint32
   RBC_TRY(od.alloc(size_t(m) * k));
   RBC_TRY(oi.alloc(size_t(m) * k));
   RBC_TRY(exact_query_common(h, Q.p, Q.ld, m, k, precision, od.get(), oi.get(), k, st, stats));
-  RBC_HIP_TRY(hipMemcpy2D(nn_dists, size_t(ldo) * 4, od.get(), size_t(k) * 4, size_t(k) * 4, m,
-                          hipMemcpyDeviceToHost));
-  RBC_HIP_TRY(hipMemcpy2D(nn_ids, size_t(ldo) * 4, oi.get(), size_t(k) * 4, size_t(k) * 4, m,
-                          hipMemcpyDeviceToHost));
+  RBC_TRY(rows_out(nn_dists, size_t(ldo) * 4, od.get(), size_t(k) * 4, size_t(k) * 4, m, mem, st));
+  RBC_TRY(rows_out(nn_ids, size_t(ldo) * 4, oi.get(), size_t(k) * 4, size_t(k) * 4, m, mem, st));
   return RBC_OK;
 }

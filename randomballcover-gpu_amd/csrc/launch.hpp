@@ -50,9 +50,19 @@ struct DevMat {
   int stage(const float *x, uint32_t rows, uint32_t d, uint32_t ldx, int mem, hipStream_t st);
 };
 
-// Copies rows*cols elements of T from device to the caller's buffer (host or device).
+// Contiguous copies between the device and the caller's buffer (host or device).
 int copy_out(void *dst, const void *src, size_t bytes, int mem, hipStream_t st);
 int copy_in(void *dst, const void *src, size_t bytes, int mem, hipStream_t st);
+
+// Row-strided copies: `rows` rows of `width` bytes, source pitch `spitch`, destination
+// pitch `dpitch` (bytes).  The host end of a transfer is always moved as one contiguous
+// block (a strided host transfer can degrade to one DMA per row); re-pitching happens on
+// the device (rows_in) or by host memcpy (rows_out).  Both return with the copy done when
+// the caller's end is host memory.
+int rows_in(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows, int mem,
+            hipStream_t st);
+int rows_out(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows, int mem,
+             hipStream_t st);
 
 // Uploads a host vector of items.
 int upload_items(const std::vector<ScanItem> &v, DevBuf<ScanItem> &buf, hipStream_t st);

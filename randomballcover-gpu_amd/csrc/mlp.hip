@@ -138,13 +138,24 @@ __global__ void gelu_backward(const float *dH, const float *Z, int64_t n, _Float
   if (i < n) dZ[i] = (_Float16)(dH[i] * gelu_grad(Z[i]));
 }
 
-// db[j] = scale * sum_i dZ[i][j].
-__global__ void col_sums(const _Float16 *dZ, int B, int N, float scale, float *db) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= N) return;
+// db[j] = scale * sum_i dZ[i][j].  A block of 1024 threads owns 64 columns: 16 row
+// groups each sum every 16th row, then the 16 partials are added in a fixed order.
+__global__ __launch_bounds__(1024) void col_sums(const _Float16 *dZ, int B, int N, float scale, float *db) {
+  __shared__ float part[16][64];
+  const int c = threadIdx.x % 64, g = threadIdx.x / 64;
+  const int j = blockIdx.x * 64 + c;
   float s = 0.f;
-  for (int i = 0; i < B; ++i) s += (float)dZ[int64_t(i) * N + j];
-  db[j] = scale * s;
+  if (j < N) {
+#pragma unroll 8
+    for (int i = g; i < B; i += 16) s += (float)dZ[int64_t(i) * N + j];
+  }
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && j < N) {
+    float t = 0.f;
+    for (int r = 0; r < 16; ++r) t += part[r][c];
+    db[j] = scale * t;
+  }
 }
 
 __global__ void adam_step(float *w, _Float16 *w16, const float *g, float *m1, float *m2, int64_t n, float lr,
@@ -296,7 +307,7 @@ int mlp_backward(rbc_mlp *m, uint32_t B, hipStream_t st) {
     g.M = outd[layer]; g.N = ind[layer]; g.K = B;
     g.alpha = 1.f / (float)B;
     RBC_TRY((gemm<_Float16, _Float16>(g, st)));
-    hipLaunchKernelGGL(col_sums, dim3(cdiv(outd[layer], 256)), dim3(256), 0, st, dz[layer], (int)B,
+    hipLaunchKernelGGL(col_sums, dim3(cdiv(outd[layer], 64)), dim3(1024), 0, st, dz[layer], (int)B,
                        (int)outd[layer], 1.f / (float)B, m->GB(layer));
     RBC_HIP_TRY(hipGetLastError());
     if (layer == 0) break;

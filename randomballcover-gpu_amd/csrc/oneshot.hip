@@ -209,10 +209,8 @@ int oneshot_query_common(const float *q, uint32_t m, uint32_t ldq, const rbcStru
     EpiTopK epi{K, od.get(), oi.get(), K, 0};
     RBC_TRY(launch_tiles_m(metric, src, items.get(), nitems, epi, st));
   }
-  hipMemcpyKind kind = mem == RBC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  RBC_HIP_TRY(hipMemcpy2DAsync(ids, size_t(ldo) * 4, oi.get(), size_t(K) * 4, size_t(K) * 4, m, kind, st));
-  if (dists)
-    RBC_HIP_TRY(hipMemcpy2DAsync(dists, size_t(ldo) * 4, od.get(), size_t(K) * 4, size_t(K) * 4, m, kind, st));
+  RBC_TRY(rows_out(ids, size_t(ldo) * 4, oi.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
+  if (dists) RBC_TRY(rows_out(dists, size_t(ldo) * 4, od.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
   RBC_HIP_TRY(hipStreamSynchronize(st));
   return RBC_OK;
 }
@@ -240,9 +238,7 @@ int rbc_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32_t num
   DevBuf<uint32_t> dmap, drep;
   RBC_TRY(dx.alloc(size_t(pn) * pd));
   RBC_TRY(fill_f32(dx.get(), 0.f, size_t(pn) * pd, st));
-  RBC_HIP_TRY(hipMemcpy2DAsync(dx.get(), size_t(pd) * 4, x, size_t(ldx) * 4, size_t(d) * 4, n,
-                               mem == RBC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                               st));
+  RBC_TRY(rows_in(dx.get(), size_t(pd) * 4, x, size_t(ldx) * 4, size_t(d) * 4, n, mem, st));
   RBC_TRY(dr.alloc(size_t(pR) * pd));
   RBC_TRY(fill_f32(dr.get(), 0.f, size_t(pR) * pd, st));
   RBC_TRY(drep.alloc(R));

@@ -105,3 +105,30 @@ def test_exact_bf16_duplicates():
     idx = rbc.ExactRBC(x, 64, seed=1)
     ids, ds = idx.kquery(q, 10, precision=rbc.PREC_BF16)
     check_against_fp64(x, q, ids, ds, 10)
+
+
+def test_exact_strided_host_buffers_match_device_path():
+    """Host inputs with ldq > d and outputs with ldo > k (re-pitched copies) give the same
+    answers as the device path, and the output padding columns are left untouched."""
+    import ctypes
+    import torch
+    n, d, m, k, ldq, ldo = 20000, 24, 70000, 5, 32, 8
+    x = datagen.uniform(n, d, 0)
+    q = datagen.uniform(m, d, 1)
+    qp = np.zeros((m, ldq), np.float32)
+    qp[:, :d] = q
+    idx = rbc.ExactRBC(x, 141, seed=3)
+    ids = np.full((m, ldo), 0xFFFFFFFE, np.uint32)
+    ds = np.full((m, ldo), -7.0, np.float32)
+    st = rbc.QueryStats()
+    rc = rbc.lib().rbc_exact_kquery(idx.h, qp.ctypes.data, m, ldq, k, rbc.PREC_BF16, ids.ctypes.data,
+                                    ds.ctypes.data, ldo, rbc.MEM_HOST, None, ctypes.byref(st))
+    assert rc == 0, rbc.lib().rbc_last_error().decode()
+    assert (ids[:, k:] == 0xFFFFFFFE).all() and (ds[:, k:] == -7.0).all()
+    qd = torch.from_numpy(q).cuda()
+    gi = torch.empty((m, k), dtype=torch.int32, device="cuda")
+    gd = torch.empty((m, k), dtype=torch.float32, device="cuda")
+    idx.kquery_device(qd, k, gi, gd, precision=rbc.PREC_BF16)
+    np.testing.assert_array_equal(ids[:, :k], gi.cpu().numpy().view(np.uint32))
+    np.testing.assert_array_equal(ds[:, :k], gd.cpu().numpy())
+    check_against_fp64(x, q[:500], ids[:500, :k], ds[:500, :k], k)
