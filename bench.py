@@ -265,6 +265,12 @@ def main():
     log(f"{args.precision}{'+mlp' if use_mlp else ''}: {ms_step:.2f} ms/step, {qps:,.0f} q/s; stages "
         f"qreps {st['ms_qreps']:.2f} probe {st['ms_probe']:.2f} plan {st['ms_plan']:.2f} "
         f"scan {st['ms_scan']:.2f} select {st['ms_select']:.2f} ms")
+    # The same query with host buffers (queries in, results out over PCIe); not `value`.
+    t0 = time.time()
+    idx.kquery(q, k, precision=prec)
+    w_host = time.time() - t0
+    extra["pcie_inclusive_qps"] = round(m / w_host, 1)
+    log(f"host-buffer query (PCIe-inclusive): {1e3*w_host:.1f} ms ({m/w_host:,.0f} q/s)")
     if use_mlp:
         same = bool(torch.equal(ref_ids, ids) and torch.equal(ref_ds, ds))
         extra["without_proposal"] = {

@@ -92,14 +92,19 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
 // three-stage LDS ring (rows padded to 272 bytes so that the 16 lanes reading 16 different
 // points at one k-offset hit distinct banks).  Each thread stages 32 bytes of a future step
 // in registers: loads for step s+2 are issued while step s computes, written to LDS during
-// step s+1.  One barrier per step.
+// step s+1.  One barrier per step.  Candidates go to an LDS list first (an LDS atomic does
+// not wait for the global prefetch in flight) and are appended to the per-query buffers
+// after the last step; entries beyond the LDS list's capacity are appended directly.
 constexpr int kVQ = kBf16ScanQ;  // queries per item
 constexpr int kVP = 32;          // points per step
 constexpr int kRowB = 272;       // LDS bytes per point row (256 + 16 pad)
 constexpr int kStages = 3;
+constexpr int kCS = 1024;        // LDS candidate list entries per item
+
+__device__ __forceinline__ int max3i(int a, int b, int c) { return max(max(a, b), c); }
 
 template <int KS>
-__global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
+__global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   const uint32_t per = (a.nitems + kXCD - 1) / kXCD;
   const uint32_t item = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
   if (item >= a.nitems) return;
@@ -110,9 +115,14 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
   // in exact arithmetic; the margin covers the FP32 accumulation (gamma_{K+1} of the
   // absolute terms) and the rounding of the stored norms, so the test never drops it.
   __shared__ float sh[kVQ];
+  __shared__ float qnl[kVQ];
   __shared__ uint32_t qrow[kVQ];
   __shared__ __attribute__((aligned(16))) unsigned char ring[kStages][kVP * kRowB];
   __shared__ float hring[kStages][kVP];  // 0.5 * ||bf16(x)||^2 of the step's points (INF past the end)
+  __shared__ uint32_t cs_p[kCS];         // candidate: point index in the list
+  __shared__ float cs_v[kCS];            // candidate: approximate squared distance
+  __shared__ uint8_t cs_q[kCS];          // candidate: query slot in the item
+  __shared__ uint32_t cs_n;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const uint32_t list = it.aux;
   {
@@ -133,6 +143,8 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
       if ((double)hs < hh) hs = nextafterf(hs, INFINITY);
     }
     sh[t] = hs;
+    qnl[t] = row != kDummy ? a.qn[row] : 0.f;
+    if (t == 0) cs_n = 0;
   }
   __syncthreads();
   // This wave's queries: A operands and the initial accumulators.
@@ -156,7 +168,8 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
   // (KS*64 bytes per row are live; the rest of the 256 is zero padding that is never read).
   const uint32_t sp = t / 8, sc = (t % 8) * 32;  // point in step, byte offset in row
   const bool stage_live = sc < (uint32_t)KS * 64;
-  // Threads 0..31 also stage the half squared norm of point t of the step.
+  // Threads 0..31 also stage the squared norm of point t of the step (halved when it is
+  // written to LDS, so that the load's value is not needed before the next step).
   auto gload = [&](uint32_t step, uint4 &r0, uint4 &r1, float &hn) {
     const uint32_t p = step * kVP + sp;
     if (stage_live && step < nsteps && p < np) {
@@ -169,10 +182,10 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
       r1 = make_uint4(0, 0, 0, 0);
     }
     const uint32_t pn = step * kVP + t;
-    hn = (t < kVP && step < nsteps && pn < np) ? 0.5f * a.xn[x0 + pn] : INFINITY;
+    hn = (t < kVP && step < nsteps && pn < np) ? a.xn[x0 + pn] : INFINITY;
   };
   auto lstore = [&](int buf, const uint4 &r0, const uint4 &r1, float hn) {
-    if (t < kVP) hring[buf][t] = hn;
+    if (t < kVP) hring[buf][t] = 0.5f * hn;
     if (!stage_live) return;
     uint4 *dst = reinterpret_cast<uint4 *>(&ring[buf][sp * kRowB + sc]);
     dst[0] = r0;
@@ -207,14 +220,23 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
         for (int qb = 0; qb < 4; ++qb)
           acc[qb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][ks], B[ks], acc[qb][pb], 0, 0, 0);
     }
+    // Pre-filter on the float bit patterns as signed integers: for a threshold h > 0 that
+    // order agrees with the float order (negative values and negative NaNs compare below,
+    // positive NaNs above -- the exact test below rejects them); h == 0 (a zero-norm
+    // point) always passes the pre-filter.
     bool hit = false;
 #pragma unroll
     for (int pb = 0; pb < 2; ++pb) {
-      float mx = fmaxf(fmaxf(acc[0][pb][0], acc[0][pb][1]), fmaxf(acc[0][pb][2], acc[0][pb][3]));
-#pragma unroll
-      for (int qb = 1; qb < 4; ++qb)
-        mx = fmaxf(mx, fmaxf(fmaxf(acc[qb][pb][0], acc[qb][pb][1]), fmaxf(acc[qb][pb][2], acc[qb][pb][3])));
-      hit = hit || mx >= hx[pb];
+      int mx = max3i(__float_as_int(acc[0][pb][0]), __float_as_int(acc[0][pb][1]), __float_as_int(acc[0][pb][2]));
+      mx = max3i(mx, __float_as_int(acc[0][pb][3]), __float_as_int(acc[1][pb][0]));
+      mx = max3i(mx, __float_as_int(acc[1][pb][1]), __float_as_int(acc[1][pb][2]));
+      mx = max3i(mx, __float_as_int(acc[1][pb][3]), __float_as_int(acc[2][pb][0]));
+      mx = max3i(mx, __float_as_int(acc[2][pb][1]), __float_as_int(acc[2][pb][2]));
+      mx = max3i(mx, __float_as_int(acc[2][pb][3]), __float_as_int(acc[3][pb][0]));
+      mx = max3i(mx, __float_as_int(acc[3][pb][1]), __float_as_int(acc[3][pb][2]));
+      mx = max(mx, __float_as_int(acc[3][pb][3]));
+      const int thr = hx[pb] > 0.f ? __float_as_int(hx[pb]) : INT_MIN;
+      hit = hit || mx >= thr;
     }
     if (__any(hit)) {
 #pragma unroll
@@ -226,11 +248,20 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             if (acc[qb][pb][r] >= hx[pb]) {
-              const uint32_t row = qrow[qi0 + r];
-              const uint32_t slot = atomicAdd(&a.cand_n[row], 1u);
-              if (slot < a.cap) {
-                a.cand_d[size_t(row) * a.cap + slot] = a.qn[row] + 2.f * (hx[pb] - (acc[qb][pb][r] - sh[qi0 + r]));
-                a.cand_i[size_t(row) * a.cap + slot] = a.perm[x0 + p];
+              const int qi = qi0 + r;
+              const float v = qnl[qi] + 2.f * (hx[pb] - (acc[qb][pb][r] - sh[qi]));
+              const uint32_t slot = atomicAdd(&cs_n, 1u);
+              if (slot < (uint32_t)kCS) {
+                cs_p[slot] = p;
+                cs_v[slot] = v;
+                cs_q[slot] = (uint8_t)qi;
+              } else {
+                const uint32_t row = qrow[qi];
+                const uint32_t gs = atomicAdd(&a.cand_n[row], 1u);
+                if (gs < a.cap) {
+                  a.cand_d[size_t(row) * a.cap + gs] = v;
+                  a.cand_i[size_t(row) * a.cap + gs] = a.perm[x0 + p];
+                }
               }
             }
           }
@@ -238,8 +269,17 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
       }
     }
   }
+  __syncthreads();
+  const uint32_t nc = min(cs_n, (uint32_t)kCS);
+  for (uint32_t i = t; i < nc; i += 256) {
+    const uint32_t row = qrow[cs_q[i]];
+    const uint32_t gs = atomicAdd(&a.cand_n[row], 1u);
+    if (gs < a.cap) {
+      a.cand_d[size_t(row) * a.cap + gs] = cs_v[i];
+      a.cand_i[size_t(row) * a.cap + gs] = a.perm[x0 + cs_p[i]];
+    }
+  }
 }
-
 
 // ---- BF16 probe -----------------------------------------------------------------------
 // Same tiling as the scan; every query keeps its K' (<= 32) smallest approximate
