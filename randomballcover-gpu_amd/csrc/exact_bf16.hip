@@ -13,8 +13,8 @@
 // of a workgroup keep 64 queries each as MFMA A operands in VGPRs and share the list's
 // points, 32 per step, through a three-stage LDS ring filled two steps ahead.  Items are
 // list-major and remapped so that each XCD takes a contiguous range: the items of a list
-// run on one XCD and re-read the list from that XCD's L2.  The probe (top-K' of the
-// nearest list) uses 128-query items and per-wave point streams.
+// run on one XCD and re-read the list from that XCD's L2.  The probe (top-K' of one
+// list per query) uses 64-query items on the same LDS ring with per-lane register top-K'.
 #include "exact_query.hpp"
 
 namespace rbc {
@@ -24,8 +24,6 @@ namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
-constexpr int kBQ = 128;  // probe: queries per item
-constexpr int kBP = 64;   // probe: points per step (16 per wave)
 constexpr int kXCD = 8;
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -282,10 +280,21 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
 }
 
 // ---- BF16 probe -----------------------------------------------------------------------
-// Same tiling as the scan; every query keeps its K' (<= 32) smallest approximate
-// distances in LDS.  Candidates under the running K'-th are staged per step; one thread
-// per query then folds them in by insertion.
-constexpr int kPK = 32;
+// Work item = (one list, up to 64 queries).  The MFMA operands are swapped with respect to
+// the scan: A = 16 points of the list (from the same three-stage LDS ring), B = 16 queries
+// of the wave, so that lane l's accumulator holds four points of ONE query (column l%16).
+// The four lanes l, l^16, l^32, l^48 see disjoint quarters of the list for their query;
+// each keeps its 8 smallest approximate distances sorted in registers (strict '<': among
+// equal values the earlier point stays ahead).  After the scan the four lists of a query
+// are merged by one thread into the query's K' <= 32 best.
+//
+// The result is the exact top-K' of the list unless one quarter held more than 8 of them;
+// either way every point not returned has an approximate distance >= the reported value
+// kth = min(K'-th returned, 8th kept of every quarter that filled its 8 slots), which is
+// the property the completeness test needs.
+constexpr int kPL = 8;                 // kept per lane
+constexpr int kPK = 4 * kPL;           // largest K'
+constexpr int kPQ = kBf16ProbeQ;       // queries per probe item
 
 struct Bf16ProbeArgs {
   const ScanItem *items;
@@ -298,124 +307,143 @@ struct Bf16ProbeArgs {
   const uint32_t *perm;
   uint32_t ldb, Kp;
   uint32_t *out_i;   // [m][Kp] original ids, ascending approximate distance
-  float *out_kth;    // [m] approximate D~ of the K'-th (FLT_MAX if the list is shorter)
+  float *out_kth;    // [m] lower bound of D~ over the points not returned (FLT_MAX if none)
 };
 
 template <int KS>
-__global__ __launch_bounds__(256) void bf16_probe_kernel(Bf16ProbeArgs a) {
+__global__ __launch_bounds__(256, 2) void bf16_probe_kernel(Bf16ProbeArgs a) {
   const ScanItem it = a.items[blockIdx.x];
-  __shared__ uint32_t qrow[kBQ];
-  __shared__ float qnrm[kBQ];
-  __shared__ float thd[kBQ];
-  __shared__ uint32_t thi[kBQ];
-  __shared__ uint32_t cn[kBQ];
-  __shared__ float td[kBQ][kPK];
-  __shared__ uint32_t ti[kBQ][kPK];
-  __shared__ float cd[kBQ][kWave];
-  __shared__ uint32_t ci[kBQ][kWave];
+  __shared__ uint32_t qrow[kPQ];
+  __shared__ __attribute__((aligned(16))) unsigned char ring[kStages][kVP * kRowB];
+  __shared__ __attribute__((aligned(16))) float xring[kStages][kVP];  // ||x~||^2 (INF past the end)
+  __shared__ float lfull[kPQ][4];  // 8th kept value of each quarter that filled its list, else INF
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  if (t < kBQ) {
-    uint32_t row = (uint32_t)t < it.q_count ? a.qids[it.q_begin + t] : kDummy;
-    qrow[t] = row;
-    qnrm[t] = row != kDummy ? a.qn[row] : 0.f;
-    thd[t] = FLT_MAX;
-    thi[t] = kDummy;
-    cn[t] = 0;
-  }
-  for (int i = t; i < kBQ * kPK; i += 256) {
-    td[i / kPK][i % kPK] = FLT_MAX;
-    ti[i / kPK][i % kPK] = kDummy;
-  }
+  if (t < kPQ) qrow[t] = (uint32_t)t < it.q_count ? a.qids[it.q_begin + t] : kDummy;
   __syncthreads();
-  bf16x8 A[8][KS];
+  const uint32_t myrow = qrow[w * 16 + (l % 16)];
+  const bool wave_live = (uint32_t)(w * 16) < it.q_count;
   const uint32_t koff = (l / 16) * 8;
+  bf16x8 Bq[KS];
 #pragma unroll
-  for (int qb = 0; qb < 8; ++qb) {
-    uint32_t row = qrow[qb * 16 + (l % 16)];
+  for (int s = 0; s < KS; ++s)
+    Bq[s] = myrow != kDummy ? load8(a.qb + size_t(myrow) * a.ldb + s * 32 + koff) : bf16x8{};
+  const float qn = myrow != kDummy ? a.qn[myrow] : 0.f;
+  float td[kPL];
+  uint32_t tp[kPL];
 #pragma unroll
-    for (int s = 0; s < KS; ++s)
-      A[qb][s] = row != kDummy ? load8(a.qb + size_t(row) * a.ldb + s * 32 + koff) : bf16x8{};
+  for (int j = 0; j < kPL; ++j) {
+    td[j] = FLT_MAX;
+    tp[j] = kDummy;
   }
   const uint32_t x0 = it.x_begin, np = it.x_count;
-  const uint32_t pl = w * 16 + (l % 16);
-  bf16x8 Bn[KS];
-  {
-    const bool pv = pl < np;
-    const uint16_t *src = a.xb + size_t(x0 + (pv ? pl : 0)) * a.ldb + koff;
-#pragma unroll
-    for (int s = 0; s < KS; ++s) Bn[s] = pv ? load8(src + s * 32) : bf16x8{};
-  }
-  for (uint32_t p0 = 0; p0 < np; p0 += kBP) {
-    const uint32_t p = p0 + pl;
-    const bool pv = p < np;
-    bf16x8 B[KS];
-#pragma unroll
-    for (int s = 0; s < KS; ++s) B[s] = Bn[s];
-    {
-      const uint32_t pn = p + kBP;
-      const bool nv = pn < np;
-      const uint16_t *src = a.xb + size_t(x0 + (nv ? pn : 0)) * a.ldb + koff;
-#pragma unroll
-      for (int s = 0; s < KS; ++s) Bn[s] = nv ? load8(src + s * 32) : bf16x8{};
+  const uint32_t nsteps = (np + kVP - 1) / kVP;
+  const uint32_t sp = t / 8, sc = (t % 8) * 32;
+  const bool stage_live = sc < (uint32_t)KS * 64;
+  auto gload = [&](uint32_t step, uint4 &r0, uint4 &r1, float &hn) {
+    const uint32_t p = step * kVP + sp;
+    if (stage_live && step < nsteps && p < np) {
+      const uint4 *src = reinterpret_cast<const uint4 *>(
+          reinterpret_cast<const unsigned char *>(a.xb + size_t(x0 + p) * a.ldb) + sc);
+      r0 = src[0];
+      r1 = src[1];
+    } else {
+      r0 = make_uint4(0, 0, 0, 0);
+      r1 = make_uint4(0, 0, 0, 0);
     }
-    const float nx = pv ? a.xn[x0 + p] : 0.f;
-    const uint32_t id = pv ? a.perm[x0 + p] : kDummy;
-    f32x4 acc[8];
+    const uint32_t pn = step * kVP + t;
+    hn = (t < kVP && step < nsteps && pn < np) ? a.xn[x0 + pn] : INFINITY;
+  };
+  auto lstore = [&](int buf, const uint4 &r0, const uint4 &r1, float hn) {
+    if (t < kVP) xring[buf][t] = hn;
+    if (!stage_live) return;
+    uint4 *dst = reinterpret_cast<uint4 *>(&ring[buf][sp * kRowB + sc]);
+    dst[0] = r0;
+    dst[1] = r1;
+  };
+  uint4 st0, st1;
+  float sn;
+  gload(0, st0, st1, sn);
+  lstore(0, st0, st1, sn);
+  gload(1, st0, st1, sn);
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    __syncthreads();
+    lstore((s + 1) % kStages, st0, st1, sn);
+    gload(s + 2, st0, st1, sn);
+    if (!wave_live) continue;
+    const unsigned char *buf = ring[s % kStages];
 #pragma unroll
-    for (int qb = 0; qb < 8; ++qb) acc[qb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int pt = 0; pt < 2; ++pt) {
+      const unsigned char *rowp = buf + (pt * 16 + (l % 16)) * kRowB + koff * 2;
+      f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s = 0; s < KS; ++s)
-#pragma unroll
-      for (int qb = 0; qb < 8; ++qb)
-        acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][s], B[s], acc[qb], 0, 0, 0);
-#pragma unroll
-    for (int qb = 0; qb < 8; ++qb) {
+      for (int ks = 0; ks < KS; ++ks)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8 *>(rowp + ks * 64), Bq[ks],
+                                                      acc, 0, 0, 0);
+      // Lane l holds points pt*16 + (l/16)*4 + r of query l%16.
+      const float4 xn4 = *reinterpret_cast<const float4 *>(&xring[s % kStages][pt * 16 + (l / 16) * 4]);
+      const float xv[4] = {xn4.x, xn4.y, xn4.z, xn4.w};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int qi = qb * 16 + (l / 16) * 4 + r;
-        float v = __builtin_fmaf(-2.f, acc[qb][r], nx) + qnrm[qi];
-        if (pv && pair_less(v, id, thd[qi], thi[qi])) {
-          uint32_t slot = atomicAdd(&cn[qi], 1u);
-          cd[qi][slot] = v;
-          ci[qi][slot] = id;
-        }
-      }
-    }
-    __syncthreads();
-    if (t < kBQ) {
-      const uint32_t n = cn[t];
-      if (n) {
-        float kd = thd[t];
-        uint32_t ki = thi[t];
-        for (uint32_t c = 0; c < n; ++c) {
-          const float cv = cd[t][c];
-          const uint32_t cid = ci[t][c];
-          if (!pair_less(cv, cid, kd, ki)) continue;
-          int pos = (int)a.Kp - 1;
-          while (pos > 0 && pair_less(cv, cid, td[t][pos - 1], ti[t][pos - 1])) {
-            td[t][pos] = td[t][pos - 1];
-            ti[t][pos] = ti[t][pos - 1];
-            --pos;
+        float v = __builtin_fmaf(-2.f, acc[r], xv[r]) + qn;
+        if (v < td[kPL - 1]) {
+          uint32_t p = s * kVP + pt * 16 + (l / 16) * 4 + r;
+#pragma unroll
+          for (int j = 0; j < kPL; ++j) {
+            const bool lt = v < td[j];
+            const float ov = td[j];
+            const uint32_t op = tp[j];
+            td[j] = lt ? v : ov;
+            tp[j] = lt ? p : op;
+            v = lt ? ov : v;
+            p = lt ? op : p;
           }
-          td[t][pos] = cv;
-          ti[t][pos] = cid;
-          kd = td[t][a.Kp - 1];
-          ki = ti[t][a.Kp - 1];
         }
-        thd[t] = kd;
-        thi[t] = ki;
-        cn[t] = 0;
       }
     }
-    __syncthreads();
   }
-  for (int i = t; i < kBQ * kPK; i += 256) {
-    const int qi = i / kPK, j = i % kPK;
-    const uint32_t row = qrow[qi];
-    if (row == kDummy || j >= (int)a.Kp) continue;
-    a.out_i[size_t(row) * a.Kp + j] = ti[qi][j];
-    if (j == (int)a.Kp - 1) a.out_kth[row] = td[qi][j];
+  // Hand the per-lane lists to one thread per query through LDS (the ring is free now).
+  __syncthreads();
+  float *ld = reinterpret_cast<float *>(&ring[0][0]);              // [kPQ][4][kPL]
+  uint32_t *lp = reinterpret_cast<uint32_t *>(&ring[1][0]);        // [kPQ][4][kPL]
+  if (wave_live) {
+    const int q = w * 16 + (l % 16), g = l / 16;
+#pragma unroll
+    for (int j = 0; j < kPL; ++j) {
+      ld[(q * 4 + g) * kPL + j] = td[j];
+      lp[(q * 4 + g) * kPL + j] = tp[j];
+    }
+    lfull[q][g] = tp[kPL - 1] != kDummy ? td[kPL - 1] : INFINITY;
   }
+  __syncthreads();
+  if (t >= kPQ || qrow[t] == kDummy) return;
+  const uint32_t row = qrow[t];
+  uint32_t head[4] = {0, 0, 0, 0};
+  float kth = FLT_MAX;
+  for (uint32_t j = 0; j < a.Kp; ++j) {
+    int best = -1;
+    float bv = FLT_MAX;
+    uint32_t bp = kDummy;
+    for (int g = 0; g < 4; ++g) {
+      if (head[g] >= (uint32_t)kPL) continue;
+      const float v = ld[(t * 4 + g) * kPL + head[g]];
+      const uint32_t p = lp[(t * 4 + g) * kPL + head[g]];
+      if (p != kDummy && (best < 0 || pair_less(v, p, bv, bp))) {
+        best = g;
+        bv = v;
+        bp = p;
+      }
+    }
+    if (best >= 0) ++head[best];
+    a.out_i[size_t(row) * a.Kp + j] = bp != kDummy ? a.perm[x0 + bp] : kDummy;
+    if (j + 1 == a.Kp) kth = bp != kDummy ? bv : FLT_MAX;
+  }
+  for (int g = 0; g < 4; ++g) kth = fminf(kth, lfull[t][g]);
+  a.out_kth[row] = kth;
+}
+
+template <int KS>
+void launch_probe(const Bf16ProbeArgs &a, uint32_t nitems, hipStream_t st) {
+  hipLaunchKernelGGL((bf16_probe_kernel<KS>), dim3(nitems), dim3(256), 0, st, a);
 }
 
 }  // namespace
@@ -491,10 +519,10 @@ int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t
   a.out_i = out_i;
   a.out_kth = out_kth;
   switch (h->ldb / 32) {
-    case 1: hipLaunchKernelGGL(bf16_probe_kernel<1>, dim3(nitems), dim3(256), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(bf16_probe_kernel<2>, dim3(nitems), dim3(256), 0, st, a); break;
-    case 3: hipLaunchKernelGGL(bf16_probe_kernel<3>, dim3(nitems), dim3(256), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(bf16_probe_kernel<4>, dim3(nitems), dim3(256), 0, st, a); break;
+    case 1: launch_probe<1>(a, nitems, st); break;
+    case 2: launch_probe<2>(a, nitems, st); break;
+    case 3: launch_probe<3>(a, nitems, st); break;
+    case 4: launch_probe<4>(a, nitems, st); break;
     default:
       set_error("BF16 filter supports d <= 128 (d=%u)", h->d);
       return RBC_ERR_LIMIT;

@@ -415,7 +415,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     uint32_t *off = static_cast<uint32_t *>(pp);
     RBC_TRY(counting_sort(keys, m, R, order, off, ws, 0, st));
     RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st,
-                              bprobe ? 128u : (uint32_t)kTQ));
+                              bprobe ? kBf16ProbeQ : (uint32_t)kTQ));
     RBC_TRY(ws.get(slot_d, size_t(m) * KpUse * 4, &pp));
     float *pd = static_cast<float *>(pp);
     RBC_TRY(ws.get(slot_i, size_t(m) * KpUse * 4, &pp));

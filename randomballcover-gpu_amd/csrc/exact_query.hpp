@@ -37,6 +37,8 @@ int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t
                hipStream_t st);
 // Queries per work item of the BF16 scan (its items must be cut at this size).
 constexpr uint32_t kBf16ScanQ = 256;
+// Queries per work item of the BF16 probe.
+constexpr uint32_t kBf16ProbeQ = 64;
 // Relative error constant of the BF16 GEMM distance: gamma_K + 6u', u' = 2^-23.
 double bf16_gemm_err(const rbc_exact *h);
 int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe, const double *gam,
