@@ -2,8 +2,8 @@
 //
 // Pipeline (one stream):
 //  1. q->reps   : FP32 distances of every query to every representative.
-//  2. row stats : nearest rep of each query and the k-th smallest rep distance (reps are
-//                 database points, so that is an upper bound on the k-th NN distance).
+//  2. row stats : nearest rep of each query and an upper bound on its k-th smallest rep
+//                 distance (reps are database points, so that bounds the k-th NN distance).
 //  3. probe     : queries grouped by nearest rep scan that rep's list, keeping the best
 //                 K' = k+8 (distance, id) pairs -> an upper bound gamma on the k-th NN
 //                 distance and a cut-off tau for candidates.
@@ -34,47 +34,31 @@ namespace {
 
 // ---- 2. row stats ----------------------------------------------------------------
 // One wave per query over its row of R distances: nearest rep (lowest index on ties)
-// and the k-th smallest (distance, rep) pair.
+// and an upper bound on the k-th smallest rep distance (k <= 64).
 __global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uint32_t R, uint32_t k,
                                                  uint32_t *near, float *kth) {
   const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
   if (q >= m) return;
   const int l = lane_id();
-  float tv = FLT_MAX;
-  uint32_t ti = kDummy;
-  float thr = FLT_MAX;
-  uint32_t thri = kDummy;
   const float *row = D + size_t(q) * R;
-  // The first 32 chunks are loaded up front so that their latencies overlap.
-  constexpr int kPre = 32;
-  float pre[kPre];
-#pragma unroll
-  for (int c = 0; c < kPre; ++c) {
-    const uint32_t j = c * kWave + l;
-    pre[c] = j < R ? row[j] : FLT_MAX;
-  }
-  for (uint32_t c0 = 0; c0 < R; c0 += kWave) {
-    const uint32_t ci = c0 / kWave;
-    float v;
-    if (ci < (uint32_t)kPre) {
-      v = FLT_MAX;
-#pragma unroll
-      for (int c = 0; c < kPre; ++c)
-        if ((uint32_t)c == ci) v = pre[c];
-    } else {
-      v = c0 + l < R ? row[c0 + l] : FLT_MAX;
+  // Lane l scans columns l, l+64, ... in ascending order and keeps its smallest value with
+  // the lowest column among equals.
+  float lm = FLT_MAX;
+  uint32_t li = kDummy;
+#pragma unroll 8
+  for (uint32_t j = l; j < R; j += kWave) {
+    const float v = row[j];
+    if (v < lm) {
+      lm = v;
+      li = j;
     }
-    uint32_t id = c0 + l < R ? c0 + l : kDummy;
-    // Skip the chunk when nothing in it beats the current k-th.
-    bool any = pair_less(v, id, thr, thri);
-    if (!__any(any)) continue;
-    wave_bitonic_sort(v, id);
-    wave_merge_lower(tv, ti, v, id);
-    thr = __shfl(tv, (int)k - 1, kWave);
-    thri = __shfl(ti, (int)k - 1, kWave);
   }
-  if (l == 0) near[q] = ti;
-  if (l == (int)k - 1) kth[q] = tv;
+  // Sorted by (value, column): lane 0 holds the nearest rep (lowest index on ties); the
+  // k-th lane minimum is the k-th smallest of k distinct rep distances, so it bounds the
+  // k-th smallest rep distance from above.
+  wave_bitonic_sort(lm, li);
+  if (l == 0) near[q] = li;
+  if (l == (int)k - 1) kth[q] = li != kDummy ? lm : FLT_MAX;
 }
 
 // ---- 3b. bounds from the probe ----------------------------------------------------
@@ -87,7 +71,7 @@ struct BoundArgs {
   const float *pd0, *pd1;        // probe values per slot (ascending), [m][Kp]
   const float *kth0, *kth1;      // BF16 probe: approximate D~ of the K'-th per slot
   uint32_t Kp, k, m, R;
-  const float *rep_kth;          // k-th smallest FP32 rep distance (FLT_MAX if R < k)
+  const float *rep_kth;          // upper bound on the k-th smallest FP32 rep distance (FLT_MAX if R < k)
   const uint32_t *near, *key1, *list_len;
   double e_probe, e_rep;         // relative error of the probe values and of rep_kth
   int metric;

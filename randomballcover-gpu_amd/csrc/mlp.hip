@@ -53,10 +53,12 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return 0.5f * (1.f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
-template <typename TA, typename TB>
+// KC: both operands are FP16 and contiguous along k (a_cs == b_rs == 1, 16-byte aligned
+// rows), so every thread stages 8 consecutive k of one row with one 16-byte load.
+template <typename TA, typename TB, bool KC>
 __global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
-  __shared__ _Float16 As[kGT][kLdsPad];
-  __shared__ _Float16 Bs[kGT][kLdsPad];  // Bs[j][k]
+  __shared__ __attribute__((aligned(16))) _Float16 As[kGT][kLdsPad];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[kGT][kLdsPad];  // Bs[j][k]
   const int t = threadIdx.x, w = t / 64, l = t % 64;
   const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
   const TA *A = static_cast<const TA *>(g.A);
@@ -67,11 +69,20 @@ __global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
   const int lr = t / 4, lk = (t % 4) * 8;
   for (int k0 = 0; k0 < g.K; k0 += kGK) {
     __syncthreads();
+    if constexpr (KC) {
+      const int i = i0 + lr, j = j0 + lr, k = k0 + lk;
+      const f16x8 z = f16x8{};
+      *reinterpret_cast<f16x8 *>(&As[lr][lk]) =
+          (i < g.M && k < g.K) ? *reinterpret_cast<const f16x8 *>(A + int64_t(i) * g.a_rs + k) : z;
+      *reinterpret_cast<f16x8 *>(&Bs[lr][lk]) =
+          (j < g.N && k < g.K) ? *reinterpret_cast<const f16x8 *>(B + int64_t(j) * g.b_cs + k) : z;
+    } else {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      int i = i0 + lr, k = k0 + lk + e, j = j0 + lr;
-      As[lr][lk + e] = (i < g.M && k < g.K) ? to_h(A[i * g.a_rs + k * g.a_cs]) : (_Float16)0.f;
-      Bs[lr][lk + e] = (j < g.N && k < g.K) ? to_h(B[k * g.b_rs + j * g.b_cs]) : (_Float16)0.f;
+      for (int e = 0; e < 8; ++e) {
+        int i = i0 + lr, k = k0 + lk + e, j = j0 + lr;
+        As[lr][lk + e] = (i < g.M && k < g.K) ? to_h(A[i * g.a_rs + k * g.a_cs]) : (_Float16)0.f;
+        Bs[lr][lk + e] = (j < g.N && k < g.K) ? to_h(B[k * g.b_rs + j * g.b_cs]) : (_Float16)0.f;
+      }
     }
     __syncthreads();
     const int rb = (w / 2) * 32, cb = (w % 2) * 32;
@@ -103,7 +114,13 @@ __global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
 template <typename TA, typename TB>
 int gemm(const GemmArgs &g, hipStream_t st) {
   dim3 grid(cdiv(g.N, kGT), cdiv(g.M, kGT));
-  hipLaunchKernelGGL((gemm_f16<TA, TB>), grid, dim3(256), 0, st, g);
+  const bool kc = sizeof(TA) == 2 && sizeof(TB) == 2 && g.a_cs == 1 && g.b_rs == 1 && g.K % 8 == 0 &&
+                  g.a_rs % 8 == 0 && g.b_cs % 8 == 0 && reinterpret_cast<uintptr_t>(g.A) % 16 == 0 &&
+                  reinterpret_cast<uintptr_t>(g.B) % 16 == 0;
+  if (kc)
+    hipLaunchKernelGGL((gemm_f16<TA, TB, true>), grid, dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((gemm_f16<TA, TB, false>), grid, dim3(256), 0, st, g);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
