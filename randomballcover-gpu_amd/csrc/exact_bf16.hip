@@ -87,19 +87,74 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
 // ---- BF16 list scan ------------------------------------------------------------------
 // Work item = (one list, up to 256 queries).  Wave w keeps queries 64w..64w+63 as MFMA A
 // operands in VGPRs; the 32 points of a step are shared by all four waves through a
-// three-stage LDS ring (rows padded to 272 bytes so that the 16 lanes reading 16 different
-// points at one k-offset hit distinct banks).  Each thread stages 32 bytes of a future step
-// in registers: loads for step s+2 are issued while step s computes, written to LDS during
-// step s+1.  One barrier per step.  Candidates go to an LDS list first (an LDS atomic does
-// not wait for the global prefetch in flight) and are appended to the per-query buffers
-// after the last step; entries beyond the LDS list's capacity are appended directly.
+// four-stage LDS ring that is filled by direct global->LDS loads (global_load_lds_dwordx4)
+// three steps ahead, so no VGPRs hold data in flight.  Each wave waits for its own loads of
+// the step (s_waitcnt vmcnt with the count of younger loads) before the step's barrier.
+// A stage holds 32 rows of 256 bytes; the 16-byte chunk c of row r sits at chunk c ^ (r%16),
+// which keeps the 64-lane reads (16 rows x 4 chunks) free of bank conflicts.  The compiler
+// would order every LDS read after any LDS DMA in flight, so the ring is read by one asm
+// block per step that waits for its own reads.
+// Candidates go to an LDS list first (an LDS atomic does not wait for the loads in flight)
+// and are appended to the per-query buffers after the last step; entries beyond the LDS
+// list's capacity are appended directly.
 constexpr int kVQ = kBf16ScanQ;  // queries per item
 constexpr int kVP = 32;          // points per step
-constexpr int kRowB = 272;       // LDS bytes per point row (256 + 16 pad)
-constexpr int kStages = 3;
+constexpr int kSStages = 4;      // scan ring stages (prefetch distance 3)
+constexpr int kSStageB = kVP * 256;
 constexpr int kCS = 1024;        // LDS candidate list entries per item
+// Probe ring (register-staged): rows padded to 288 bytes, which measured free of bank
+// conflicts for the same 64-lane read pattern.
+constexpr int kRowB = 288;
+constexpr int kStages = 3;
+
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i32x4;
 
 __device__ __forceinline__ int max3i(int a, int b, int c) { return max(max(a, b), c); }
+
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+  return (uint32_t)(uintptr_t)(const lds_void *)p;
+}
+
+// Direct global->LDS loads (lane l writes bytes [16 l, 16 l + 16) or [4 l, 4 l + 4) after
+// the wave-uniform LDS address).  Issued from asm so that the compiler's wait insertion,
+// which would otherwise drain them before unrelated instructions, leaves them alone: the
+// kernel waits for them itself (wait_vm) before the barrier that publishes the data.
+__device__ __forceinline__ void dma16(const void *g, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+}
+__device__ __forceinline__ void dma4(const void *g, uint32_t lds) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+}
+
+// vmcnt(n) with the other counters left alone (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+// One step's ring reads: eight 16-byte B fragments and the two half-norms of the lane's
+// points, then a wait for them (the asm's outputs are complete when it ends).
+__device__ __forceinline__ void ring_read(const uint32_t (&ad)[8], uint32_t ah, i32x4 (&b)[8], float &h0,
+                                          float &h1) {
+  asm volatile(
+      "ds_read_b128 %0, %10\n\t"
+      "ds_read_b128 %1, %11\n\t"
+      "ds_read_b128 %2, %12\n\t"
+      "ds_read_b128 %3, %13\n\t"
+      "ds_read_b128 %4, %14\n\t"
+      "ds_read_b128 %5, %15\n\t"
+      "ds_read_b128 %6, %16\n\t"
+      "ds_read_b128 %7, %17\n\t"
+      "ds_read_b32 %8, %18\n\t"
+      "ds_read_b32 %9, %18 offset:64\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]), "=&v"(b[5]), "=&v"(b[6]),
+        "=&v"(b[7]), "=&v"(h0), "=&v"(h1)
+      : "v"(ad[0]), "v"(ad[1]), "v"(ad[2]), "v"(ad[3]), "v"(ad[4]), "v"(ad[5]), "v"(ad[6]), "v"(ad[7]),
+        "v"(ah)
+      : "memory");
+}
 
 template <int KS>
 __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
@@ -115,11 +170,11 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   __shared__ float sh[kVQ];
   __shared__ float qnl[kVQ];
   __shared__ uint32_t qrow[kVQ];
-  __shared__ __attribute__((aligned(16))) unsigned char ring[kStages][kVP * kRowB];
-  __shared__ float hring[kStages][kVP];  // 0.5 * ||bf16(x)||^2 of the step's points (INF past the end)
-  __shared__ uint32_t cs_p[kCS];         // candidate: point index in the list
-  __shared__ float cs_v[kCS];            // candidate: approximate squared distance
-  __shared__ uint8_t cs_q[kCS];          // candidate: query slot in the item
+  __shared__ __attribute__((aligned(16))) unsigned char ring[kSStages * kSStageB];
+  __shared__ __attribute__((aligned(16))) float nring[kSStages][4][kWave];  // ||x~||^2, per wave copy
+  __shared__ uint32_t cs_p[kCS];  // candidate: point index in the list
+  __shared__ float cs_v[kCS];     // candidate: approximate squared distance
+  __shared__ uint8_t cs_q[kCS];   // candidate: query slot in the item
   __shared__ uint32_t cs_n;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   const uint32_t list = it.aux;
@@ -162,54 +217,72 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   }
   const uint32_t x0 = it.x_begin, np = it.x_count;
   const uint32_t nsteps = (np + kVP - 1) / kVP;
-  // Staging: thread t copies bytes [32*(t%8), 32*(t%8)+32) of point t/8 of a step
-  // (KS*64 bytes per row are live; the rest of the 256 is zero padding that is never read).
-  const uint32_t sp = t / 8, sc = (t % 8) * 32;  // point in step, byte offset in row
-  const bool stage_live = sc < (uint32_t)KS * 64;
-  // Threads 0..31 also stage the squared norm of point t of the step (halved when it is
-  // written to LDS, so that the load's value is not needed before the next step).
-  auto gload = [&](uint32_t step, uint4 &r0, uint4 &r1, float &hn) {
-    const uint32_t p = step * kVP + sp;
-    if (stage_live && step < nsteps && p < np) {
-      const uint4 *src = reinterpret_cast<const uint4 *>(
-          reinterpret_cast<const unsigned char *>(a.xb + size_t(x0 + p) * a.ldb) + sc);
-      r0 = src[0];
-      r1 = src[1];
-    } else {
-      r0 = make_uint4(0, 0, 0, 0);
-      r1 = make_uint4(0, 0, 0, 0);
+  // Loads of one step: two 1 KB blocks per wave (rows 8w..8w+7, all 16 chunks; lane l
+  // fills chunk slot l%16 of row 8w + 4j + l/16 with the chunk that belongs there) and the
+  // norms of the step's 32 points (lanes 32..63 repeat lanes 0..31).  Rows past the list end
+  // and chunks past the row length load clamped, valid addresses: every lane issues its
+  // load, so each wave issues exactly three loads per step.
+  auto issue = [&](uint32_t step) {
+    const int stage = (int)(step % kSStages);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint32_t row = w * 8 + j * 4 + l / 16;
+      uint32_t lc = (uint32_t)(l % 16) ^ (row & 15);
+      lc = lc < (uint32_t)KS * 4 ? lc : 0;
+      const uint32_t p = min(step * kVP + row, np - 1);
+      dma16(a.xb + size_t(x0 + p) * a.ldb + lc * 8,
+            __builtin_amdgcn_readfirstlane(lds_addr(ring + stage * kSStageB + (w * 2 + j) * 1024)));
     }
-    const uint32_t pn = step * kVP + t;
-    hn = (t < kVP && step < nsteps && pn < np) ? a.xn[x0 + pn] : INFINITY;
+    const uint32_t pn = min(step * kVP + (uint32_t)(l & 31), np - 1);
+    dma4(a.xn + x0 + pn, __builtin_amdgcn_readfirstlane(lds_addr(&nring[stage][w][0])));
   };
-  auto lstore = [&](int buf, const uint4 &r0, const uint4 &r1, float hn) {
-    if (t < kVP) hring[buf][t] = 0.5f * hn;
-    if (!stage_live) return;
-    uint4 *dst = reinterpret_cast<uint4 *>(&ring[buf][sp * kRowB + sc]);
-    dst[0] = r0;
-    dst[1] = r1;
-  };
-  uint4 st0, st1;
-  float sn;
-  gload(0, st0, st1, sn);
-  lstore(0, st0, st1, sn);
-  gload(1, st0, st1, sn);
+  // Per-lane read addresses (stage 0): B fragment (pb, ks) is row pb*16 + l%16, chunk
+  // l/16 + 4 ks; chunks at or past 4 KS are read but never used.
+  uint32_t ad0[8];
+#pragma unroll
+  for (int pb = 0; pb < 2; ++pb)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const uint32_t r = pb * 16 + l % 16;
+      const uint32_t c = ((uint32_t)(l / 16) + 4 * ks) ^ (r & 15);
+      ad0[pb * 4 + ks] = lds_addr(ring) + r * 256 + c * 16;
+    }
+  const uint32_t ah0 = lds_addr(&nring[0][w][0]) + (l % 16) * 4;
+  // The A operands must have arrived before the first MFMA; waiting here keeps the
+  // compiler from placing that wait inside the loop, where it would also drain the
+  // ring loads in flight.
+  wait_vm<0>();
+  if (nsteps > 0) issue(0);
+  if (nsteps > 1) issue(1);
+  if (nsteps > 2) issue(2);
   for (uint32_t s = 0; s < nsteps; ++s) {
-    __syncthreads();
-    // Stage s+1 goes to LDS, the loads for s+2 start.
-    lstore((s + 1) % kStages, st0, st1, sn);
-    gload(s + 2, st0, st1, sn);
+    // This wave's loads of step s are done when at most the loads of steps s+1 and s+2
+    // (those that exist) are outstanding; the barrier then publishes every wave's.
+    if (s + 2 < nsteps)
+      wait_vm<6>();
+    else if (s + 1 < nsteps)
+      wait_vm<3>();
+    else
+      wait_vm<0>();
+    asm volatile("s_barrier" ::: "memory");
+    if (s + 3 < nsteps) issue(s + 3);
     if (!wave_live) continue;
-    const unsigned char *buf = ring[s % kStages];
-    // This lane's two points of the step: pb*16 + l%16.
-    const float hx[2] = {hring[s % kStages][l % 16], hring[s % kStages][16 + l % 16]};
+    const uint32_t so = (s % kSStages) * kSStageB, ho = (s % kSStages) * (4 * kWave * 4);
+    uint32_t ad[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ad[i] = ad0[i] + so;
+    i32x4 braw[8];
+    float n0, n1;
+    ring_read(ad, ah0 + ho, braw, n0, n1);
+    float hx[2];
+    hx[0] = s * kVP + (l % 16) < np ? 0.5f * n0 : INFINITY;
+    hx[1] = s * kVP + 16 + (l % 16) < np ? 0.5f * n1 : INFINITY;
     f32x4 acc[4][2];
 #pragma unroll
     for (int pb = 0; pb < 2; ++pb) {
-      const unsigned char *rowp = buf + (pb * 16 + (l % 16)) * kRowB + koff * 2;
       bf16x8 B[KS];
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) B[ks] = *reinterpret_cast<const bf16x8 *>(rowp + ks * 64);
+      for (int ks = 0; ks < KS; ++ks) B[ks] = __builtin_bit_cast(bf16x8, braw[pb * 4 + ks]);
 #pragma unroll
       for (int qb = 0; qb < 4; ++qb) acc[qb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][0], B[0], S[qb], 0, 0, 0);
 #pragma unroll
@@ -245,7 +318,7 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
           const int qi0 = w * 64 + qb * 16 + (l / 16) * 4;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            if (acc[qb][pb][r] >= hx[pb]) {
+            if (acc[qb][pb][r] >= hx[pb] && p < np) {
               const int qi = qi0 + r;
               const float v = qnl[qi] + 2.f * (hx[pb] - (acc[qb][pb][r] - sh[qi]));
               const uint32_t slot = atomicAdd(&cs_n, 1u);
