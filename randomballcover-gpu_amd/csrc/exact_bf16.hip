@@ -62,18 +62,18 @@ __global__ __launch_bounds__(256) void prep_queries(const float *q, size_t ldq, 
 struct Bf16ScanArgs {
   const ScanItem *items;
   uint32_t nitems;
-  const uint32_t *qids;
+  const uint32_t *qids;  // list-major (query, list) pairs: query ids
   const uint16_t *qb;    // queries, BF16, ldb
   const float *qn;       // |q~|^2
   const float *qe;       // |q - q~|
   const double *gam;     // per query: upper bound on the k-th NN distance (metric units)
+  const float *lqe;      // per list: max |x - x~|
+  const float *lnm;      // per list: max |x~|^2
+  double c_gemm;         // gamma_K + 6u' (u' = 2^-23)
   const uint16_t *xb;    // permuted database, BF16, ldb
   const float *xn;       // |x~|^2 per permuted row
   const uint32_t *perm;  // permuted row -> original id
-  const float *lqe;      // per list: max |x - x~|
-  const float *lnm;      // per list: max |x~|^2
   uint32_t ldb;
-  double c_gemm;         // gamma_K + 6u'
   float *cand_d;
   uint32_t *cand_i;
   uint32_t *cand_n;
@@ -162,59 +162,18 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   const uint32_t item = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
   if (item >= a.nitems) return;
   const ScanItem it = a.items[item];
-  // Emission test per (query i, point x): MFMA(s_i + q~.x~) >= |x~|^2 / 2, with the
-  // accumulator initialised to s_i = (T_i - |q~|^2)/2 + margin, T_i = (G_i + eta)^2.  Any x
-  // with |q - x| <= G_i has |q~ - x~| <= G_i + eta, i.e. q~.x~ + (T_i - |q~|^2)/2 >= |x~|^2/2
-  // in exact arithmetic; the margin covers the FP32 accumulation (gamma_{K+1} of the
-  // absolute terms) and the rounding of the stored norms, so the test never drops it.
+  // Emission test per (query i, point x): MFMA(s_i + q~.x~) >= |x~|^2 / 2 with s_i the
+  // threshold of the (query, list) pair (bf16_threshold), so the test never drops an x with
+  // |q - x| <= G_i.
   __shared__ float sh[kVQ];
-  __shared__ float qnl[kVQ];
   __shared__ uint32_t qrow[kVQ];
   __shared__ __attribute__((aligned(16))) unsigned char ring[kSStages * kSStageB];
   __shared__ __attribute__((aligned(16))) float nring[kSStages][4][kWave];  // ||x~||^2, per wave copy
   __shared__ uint32_t cs_p[kCS];  // candidate: point index in the list
-  __shared__ float cs_v[kCS];     // candidate: approximate squared distance
+  __shared__ float cs_v[kCS];     // candidate: approximate squared distance minus |q~|^2
   __shared__ uint8_t cs_q[kCS];   // candidate: query slot in the item
   __shared__ uint32_t cs_n;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  const uint32_t list = it.aux;
-  {
-    uint32_t row = (uint32_t)t < it.q_count ? a.qids[it.q_begin + t] : kDummy;
-    qrow[t] = row;
-    float hs = -INFINITY;
-    if (row != kDummy) {
-      const double G = a.gam[row];
-      const double eta = (double)a.qe[row] + (double)a.lqe[list];
-      const double nq = (double)a.qn[row];
-      const double nxm = (double)a.lnm[list];
-      const double T0 = (G + eta) * (G + eta);
-      const double s0 = 0.5 * (T0 - nq);
-      const double up = 0x1p-23;
-      const double margin = a.c_gemm * (fabs(s0) + 0.5 * (nq + nxm)) + up * (nq + nxm) + 1e-30;
-      const double hh = (s0 + margin) * (1.0 + 1e-9);
-      hs = hh >= (double)FLT_MAX ? INFINITY : (float)hh;
-      if ((double)hs < hh) hs = nextafterf(hs, INFINITY);
-    }
-    sh[t] = hs;
-    qnl[t] = row != kDummy ? a.qn[row] : 0.f;
-    if (t == 0) cs_n = 0;
-  }
-  __syncthreads();
-  // This wave's queries: A operands and the initial accumulators.
-  const bool wave_live = (uint32_t)(w * 64) < it.q_count;
-  bf16x8 A[4][KS];
-  f32x4 S[4];
-  const uint32_t koff = (l / 16) * 8;
-#pragma unroll
-  for (int qb = 0; qb < 4; ++qb) {
-    const int q0 = w * 64 + qb * 16;
-    const uint32_t row = qrow[q0 + (l % 16)];
-#pragma unroll
-    for (int s = 0; s < KS; ++s)
-      A[qb][s] = row != kDummy ? load8(a.qb + size_t(row) * a.ldb + s * 32 + koff) : bf16x8{};
-    const float4 s4 = *reinterpret_cast<const float4 *>(&sh[q0 + (l / 16) * 4]);
-    S[qb] = f32x4{s4.x, s4.y, s4.z, s4.w};
-  }
   const uint32_t x0 = it.x_begin, np = it.x_count;
   const uint32_t nsteps = (np + kVP - 1) / kVP;
   // Loads of one step: two 1 KB blocks per wave (rows 8w..8w+7, all 16 chunks; lane l
@@ -236,6 +195,34 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
     const uint32_t pn = min(step * kVP + (uint32_t)(l & 31), np - 1);
     dma4(a.xn + x0 + pn, __builtin_amdgcn_readfirstlane(lds_addr(&nring[stage][w][0])));
   };
+  // The ring's first loads depend only on the item, so they go out before the query setup.
+  if (nsteps > 0) issue(0);
+  if (nsteps > 1) issue(1);
+  if (nsteps > 2) issue(2);
+  {
+    const bool live = (uint32_t)t < it.q_count;
+    const uint32_t row = live ? a.qids[it.q_begin + t] : kDummy;
+    qrow[t] = row;
+    sh[t] = live ? bf16_threshold(a.gam[row], a.qe[row], a.qn[row], a.lqe[it.aux], a.lnm[it.aux], a.c_gemm)
+                 : -INFINITY;
+    if (t == 0) cs_n = 0;
+  }
+  __syncthreads();
+  // This wave's queries: A operands and the initial accumulators.
+  const bool wave_live = (uint32_t)(w * 64) < it.q_count;
+  bf16x8 A[4][KS];
+  f32x4 S[4];
+  const uint32_t koff = (l / 16) * 8;
+#pragma unroll
+  for (int qb = 0; qb < 4; ++qb) {
+    const int q0 = w * 64 + qb * 16;
+    const uint32_t row = qrow[q0 + (l % 16)];
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+      A[qb][s] = row != kDummy ? load8(a.qb + size_t(row) * a.ldb + s * 32 + koff) : bf16x8{};
+    const float4 s4 = *reinterpret_cast<const float4 *>(&sh[q0 + (l / 16) * 4]);
+    S[qb] = f32x4{s4.x, s4.y, s4.z, s4.w};
+  }
   // Per-lane read addresses (stage 0): B fragment (pb, ks) is row pb*16 + l%16, chunk
   // l/16 + 4 ks; chunks at or past 4 KS are read but never used.
   uint32_t ad0[8];
@@ -248,13 +235,10 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
       ad0[pb * 4 + ks] = lds_addr(ring) + r * 256 + c * 16;
     }
   const uint32_t ah0 = lds_addr(&nring[0][w][0]) + (l % 16) * 4;
-  // The A operands must have arrived before the first MFMA; waiting here keeps the
-  // compiler from placing that wait inside the loop, where it would also drain the
-  // ring loads in flight.
+  // The A operands must have arrived before the first MFMA; waiting here (for everything,
+  // the ring's first loads included) keeps the compiler from placing that wait inside the
+  // loop, where it would also drain the ring loads in flight.
   wait_vm<0>();
-  if (nsteps > 0) issue(0);
-  if (nsteps > 1) issue(1);
-  if (nsteps > 2) issue(2);
   for (uint32_t s = 0; s < nsteps; ++s) {
     // This wave's loads of step s are done when at most the loads of steps s+1 and s+2
     // (those that exist) are outstanding; the barrier then publishes every wave's.
@@ -294,20 +278,22 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
     // Pre-filter on the float bit patterns as signed integers: for a threshold h > 0 that
     // order agrees with the float order (negative values and negative NaNs compare below,
     // positive NaNs above -- the exact test below rejects them); h == 0 (a zero-norm
-    // point) always passes the pre-filter.
+    // point) always passes the pre-filter.  One flag per group of 4 accumulators (one
+    // query block, one point block), so that a step with a hit only tests the groups that
+    // have one.
+    bool gh[2][4];
     bool hit = false;
 #pragma unroll
     for (int pb = 0; pb < 2; ++pb) {
-      int mx = max3i(__float_as_int(acc[0][pb][0]), __float_as_int(acc[0][pb][1]), __float_as_int(acc[0][pb][2]));
-      mx = max3i(mx, __float_as_int(acc[0][pb][3]), __float_as_int(acc[1][pb][0]));
-      mx = max3i(mx, __float_as_int(acc[1][pb][1]), __float_as_int(acc[1][pb][2]));
-      mx = max3i(mx, __float_as_int(acc[1][pb][3]), __float_as_int(acc[2][pb][0]));
-      mx = max3i(mx, __float_as_int(acc[2][pb][1]), __float_as_int(acc[2][pb][2]));
-      mx = max3i(mx, __float_as_int(acc[2][pb][3]), __float_as_int(acc[3][pb][0]));
-      mx = max3i(mx, __float_as_int(acc[3][pb][1]), __float_as_int(acc[3][pb][2]));
-      mx = max(mx, __float_as_int(acc[3][pb][3]));
       const int thr = hx[pb] > 0.f ? __float_as_int(hx[pb]) : INT_MIN;
-      hit = hit || mx >= thr;
+#pragma unroll
+      for (int qb = 0; qb < 4; ++qb) {
+        const int mx = max(max3i(__float_as_int(acc[qb][pb][0]), __float_as_int(acc[qb][pb][1]),
+                                 __float_as_int(acc[qb][pb][2])),
+                           __float_as_int(acc[qb][pb][3]));
+        gh[pb][qb] = mx >= thr;
+        hit = hit || gh[pb][qb];
+      }
     }
     if (__any(hit)) {
 #pragma unroll
@@ -315,12 +301,13 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
         const uint32_t p = s * kVP + pb * 16 + (l % 16);
 #pragma unroll
         for (int qb = 0; qb < 4; ++qb) {
+          if (!gh[pb][qb]) continue;
           const int qi0 = w * 64 + qb * 16 + (l / 16) * 4;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             if (acc[qb][pb][r] >= hx[pb] && p < np) {
               const int qi = qi0 + r;
-              const float v = qnl[qi] + 2.f * (hx[pb] - (acc[qb][pb][r] - sh[qi]));
+              const float v = 2.f * (hx[pb] - (acc[qb][pb][r] - sh[qi]));
               const uint32_t slot = atomicAdd(&cs_n, 1u);
               if (slot < (uint32_t)kCS) {
                 cs_p[slot] = p;
@@ -330,7 +317,7 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
                 const uint32_t row = qrow[qi];
                 const uint32_t gs = atomicAdd(&a.cand_n[row], 1u);
                 if (gs < a.cap) {
-                  a.cand_d[size_t(row) * a.cap + gs] = v;
+                  a.cand_d[size_t(row) * a.cap + gs] = a.qn[row] + v;
                   a.cand_i[size_t(row) * a.cap + gs] = a.perm[x0 + p];
                 }
               }
@@ -346,7 +333,7 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
     const uint32_t row = qrow[cs_q[i]];
     const uint32_t gs = atomicAdd(&a.cand_n[row], 1u);
     if (gs < a.cap) {
-      a.cand_d[size_t(row) * a.cap + gs] = cs_v[i];
+      a.cand_d[size_t(row) * a.cap + gs] = a.qn[row] + cs_v[i];
       a.cand_i[size_t(row) * a.cap + gs] = a.perm[x0 + cs_p[i]];
     }
   }
@@ -541,13 +528,13 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
   a.qn = qn;
   a.qe = qe;
   a.gam = gam;
+  a.lqe = h->list_qerr.get();
+  a.lnm = h->list_nmax.get();
+  a.c_gemm = bf16_gemm_err(h);
   a.xb = h->xb.get();
   a.xn = h->xb_norm.get();
   a.perm = h->perm.get();
-  a.lqe = h->list_qerr.get();
-  a.lnm = h->list_nmax.get();
   a.ldb = h->ldb;
-  a.c_gemm = bf16_gemm_err(h);
   a.cand_d = cand_d;
   a.cand_i = cand_i;
   a.cand_n = cand_n;

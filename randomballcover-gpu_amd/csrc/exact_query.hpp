@@ -45,4 +45,25 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
               const uint32_t *qlist, const ScanItem *items, uint32_t nitems, float *cand_d,
               uint32_t *cand_i, uint32_t *cand_n, uint32_t cap, hipStream_t st);
 
+// Emission threshold of the BF16 list scan for one (query, list) pair.  The scan emits x
+// when MFMA(s + q~.x~) >= |x~|^2 / 2 with the accumulator initialised to the returned s =
+// (T - |q~|^2)/2 + margin, T = (G + eta)^2, eta = |q - q~| + max_list |x - x~|: any x with
+// |q - x| <= G has |q~ - x~| <= G + eta, i.e. q~.x~ + (T - |q~|^2)/2 >= |x~|^2/2 in exact
+// arithmetic; the margin covers the FP32 accumulation (c_gemm = gamma_K + 6u' of the
+// absolute terms) and the rounding of the stored norms.  Rounded up to float.
+__device__ __forceinline__ float bf16_threshold(double G, float qe, float qn, float lqe, float lnm,
+                                                double c_gemm) {
+  const double eta = (double)qe + (double)lqe;
+  const double nq = (double)qn;
+  const double nxm = (double)lnm;
+  const double T0 = (G + eta) * (G + eta);
+  const double s0 = 0.5 * (T0 - nq);
+  const double up = 0x1p-23;
+  const double margin = c_gemm * (fabs(s0) + 0.5 * (nq + nxm)) + up * (nq + nxm) + 1e-30;
+  const double hh = (s0 + margin) * (1.0 + 1e-9);
+  float hs = hh >= (double)FLT_MAX ? INFINITY : (float)hh;
+  if ((double)hs < hh) hs = nextafterf(hs, INFINITY);
+  return hs;
+}
+
 }  // namespace rbc
