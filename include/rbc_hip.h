@@ -159,6 +159,13 @@ int rbc_exact_get_lists(const rbc_exact *h, uint32_t *rep_ids, uint32_t *list_of
 
 int rbc_exact_destroy(rbc_exact *h);
 
+/* Build once, query many (the reference's readme.txt:62-65; the reference has no file
+ * format for an index): writes every array of the index to path, and loads such a file
+ * into a new index that answers exactly like the saved one.  No counterpart in the
+ * reference. */
+int rbc_exact_save(const rbc_exact *h, const char *path);
+int rbc_exact_load(const char *path, rbc_exact **out);
+
 /* ------------------------------------------------------- learned list proposal -- */
 
 /* A small MLP d_in -> h1 -> h2 -> n_out (GELU, FP16 weights and activations, FP32
@@ -175,6 +182,9 @@ int rbc_mlp_num_params(const rbc_mlp *m, size_t *count);
  * W3[n_out][h2], b3[n_out] (host buffers). */
 int rbc_mlp_get_params(const rbc_mlp *m, float *params);
 int rbc_mlp_set_params(rbc_mlp *m, const float *params);
+/* Proposal network to / from a file: layer sizes, parameters and Adam state. */
+int rbc_mlp_save(const rbc_mlp *m, const char *path);
+int rbc_mlp_load(const char *path, rbc_mlp **out);
 /* Mean softmax cross-entropy of one batch (host x, labels) and its gradient w.r.t. the
  * parameters (host, same layout as the parameters); no update. */
 int rbc_mlp_loss_grad(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx,

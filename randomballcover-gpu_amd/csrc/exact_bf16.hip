@@ -120,11 +120,24 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p) {
 // the wave-uniform LDS address).  Issued from asm so that the compiler's wait insertion,
 // which would otherwise drain them before unrelated instructions, leaves them alone: the
 // kernel waits for them itself (wait_vm) before the barrier that publishes the data.
+// M0 is saved and restored inside the block, so the compiler's use of it is unaffected;
+// the s_nop is the wait state the hardware needs between writing M0 and an LDS DMA that
+// reads it (the compiler inserts the same for its own LDS DMA).
 __device__ __forceinline__ void dma16(const void *g, uint32_t lds) {
-  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+  uint32_t keep;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\ts_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off\n\ts_mov_b32 m0, %0"
+      : "=&s"(keep)
+      : "v"(g), "s"(lds)
+      : "memory");
 }
 __device__ __forceinline__ void dma4(const void *g, uint32_t lds) {
-  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+  uint32_t keep;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\ts_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dword %1, off\n\ts_mov_b32 m0, %0"
+      : "=&s"(keep)
+      : "v"(g), "s"(lds)
+      : "memory");
 }
 
 // vmcnt(n) with the other counters left alone (gfx9 encoding).

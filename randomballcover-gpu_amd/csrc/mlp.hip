@@ -12,6 +12,8 @@
 // with bias and GELU fused in its epilogue; softmax cross-entropy; GELU backward;
 // column sums; Adam on FP32 master weights with FP16 copies.
 #include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -443,6 +445,80 @@ int rbc_mlp_num_params(const rbc_mlp *m, size_t *count) {
 int rbc_mlp_get_params(const rbc_mlp *m, float *params) {
   RBC_ARG_CHECK(m && params, "NULL argument");
   RBC_HIP_TRY(hipMemcpy(params, m->w.get(), m->np * 4, hipMemcpyDeviceToHost));
+  return RBC_OK;
+}
+
+// File layout: the 8-byte magic "RBCMLP01", uint32 {d_in, h1, h2, n_out}, uint64 Adam step,
+// then the FP32 parameters and the two Adam moment vectors (num_params each).
+int rbc_mlp_save(const rbc_mlp *m, const char *path) {
+  RBC_ARG_CHECK(m && path, "NULL argument");
+  FILE *f = fopen(path, "wb");
+  RBC_ARG_CHECK(f, "cannot open %s for writing", path);
+  static const char magic[8] = {'R', 'B', 'C', 'M', 'L', 'P', '0', '1'};
+  const uint32_t dims[4] = {m->D, m->H1, m->H2, m->C};
+  const uint64_t step = m->step;
+  std::vector<float> hv(m->np);
+  bool ok = fwrite(magic, 1, 8, f) == 8 && fwrite(dims, 4, 4, f) == 4 && fwrite(&step, 8, 1, f) == 1;
+  const float *src[3] = {m->w.get(), m->m1.get(), m->m2.get()};
+  for (int i = 0; ok && i < 3; ++i) {
+    if (hipMemcpy(hv.data(), src[i], m->np * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      fclose(f);
+      set_error("parameter download failed");
+      return RBC_ERR_HIP;
+    }
+    ok = fwrite(hv.data(), 4, m->np, f) == m->np;
+  }
+  ok = (fclose(f) == 0) && ok;
+  RBC_ARG_CHECK(ok, "write to %s failed", path);
+  return RBC_OK;
+}
+
+int rbc_mlp_load(const char *path, rbc_mlp **out) {
+  RBC_ARG_CHECK(path && out, "NULL argument");
+  *out = nullptr;
+  FILE *f = fopen(path, "rb");
+  RBC_ARG_CHECK(f, "cannot open %s", path);
+  char magic[8];
+  uint32_t dims[4];
+  uint64_t step = 0;
+  if (fread(magic, 1, 8, f) != 8 || memcmp(magic, "RBCMLP01", 8) || fread(dims, 4, 4, f) != 4 ||
+      fread(&step, 8, 1, f) != 1) {
+    fclose(f);
+    set_error("%s is not a proposal-network file", path);
+    return RBC_ERR_ARG;
+  }
+  rbc_mlp *m = nullptr;
+  int st = rbc_mlp_create(dims[0], dims[1], dims[2], dims[3], 0, &m);
+  if (st != RBC_OK) {
+    fclose(f);
+    return st;
+  }
+  std::vector<float> hv(m->np);
+  float *dst[3] = {m->w.get(), m->m1.get(), m->m2.get()};
+  for (int i = 0; i < 3; ++i) {
+    if (fread(hv.data(), 4, m->np, f) != m->np) {
+      fclose(f);
+      delete m;
+      set_error("%s is truncated", path);
+      return RBC_ERR_ARG;
+    }
+    if (hipMemcpy(dst[i], hv.data(), m->np * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      fclose(f);
+      delete m;
+      set_error("parameter upload failed");
+      return RBC_ERR_HIP;
+    }
+  }
+  fclose(f);
+  m->step = step;
+  hipLaunchKernelGGL(to_half, dim3((uint32_t)cdivz(m->np, 256)), dim3(256), 0, nullptr, m->w.get(),
+                     (int64_t)m->np, m->w16.get());
+  if (hipDeviceSynchronize() != hipSuccess) {
+    delete m;
+    set_error("proposal network load failed");
+    return RBC_ERR_HIP;
+  }
+  *out = m;
   return RBC_OK;
 }
 

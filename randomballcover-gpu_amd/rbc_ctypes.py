@@ -88,12 +88,16 @@ _SIGS = {
     "rbc_exact_get_info": (_int, [_vp, ctypes.POINTER(ExactInfo)]),
     "rbc_exact_get_lists": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rbc_exact_destroy": (_int, [_vp]),
+    "rbc_exact_save": (_int, [_vp, ctypes.c_char_p]),
+    "rbc_exact_load": (_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "rbc_exact_set_proposal": (_int, [_vp, _vp]),
     "rbc_mlp_create": (_int, [_u32, _u32, _u32, _u32, _u64, ctypes.POINTER(_vp)]),
     "rbc_mlp_destroy": (_int, [_vp]),
     "rbc_mlp_num_params": (_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
     "rbc_mlp_get_params": (_int, [_vp, _vp]),
     "rbc_mlp_set_params": (_int, [_vp, _vp]),
+    "rbc_mlp_save": (_int, [_vp, ctypes.c_char_p]),
+    "rbc_mlp_load": (_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "rbc_mlp_loss_grad": (_int, [_vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float), _vp]),
     "rbc_mlp_logits": (_int, [_vp, _vp, _u32, _u32, _vp]),
     "rbc_mlp_train": (_int, [_vp, _vp, _u32, _u32, _vp, _int, _u32, _u32, ctypes.c_float, _u64, _vp]),
@@ -259,6 +263,20 @@ class ExactRBC:
                                      ctypes.byref(self.h)))
         self.n, self.d, self.num_reps, self.metric = n, d, num_reps, metric
 
+    def save(self, path):
+        """rbc_exact_save: the whole index to a file."""
+        _check(lib().rbc_exact_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        """rbc_exact_load: an index saved by save(), ready to query."""
+        self = cls.__new__(cls)
+        self.h = _vp()
+        _check(lib().rbc_exact_load(os.fsencode(path), ctypes.byref(self.h)))
+        inf = self.info()
+        self.n, self.d, self.num_reps, self.metric = inf["n"], inf["d"], inf["num_reps"], inf["metric"]
+        return self
+
     def kquery(self, q, k, precision=PREC_FP32, stats=False):
         q, qp = _host(q, np.float32)
         m = q.shape[0]
@@ -321,6 +339,22 @@ class ProposalMLP:
         n = ctypes.c_size_t()
         _check(lib().rbc_mlp_num_params(self.h, ctypes.byref(n)))
         self.num_params = n.value
+
+    def save(self, path):
+        """rbc_mlp_save: layer sizes, parameters and Adam state to a file."""
+        _check(lib().rbc_mlp_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        self = cls.__new__(cls)
+        self.h = _vp()
+        _check(lib().rbc_mlp_load(os.fsencode(path), ctypes.byref(self.h)))
+        with open(path, "rb") as f:
+            self.dims = tuple(int(v) for v in np.frombuffer(f.read(24)[8:], dtype=np.uint32))
+        n = ctypes.c_size_t()
+        _check(lib().rbc_mlp_num_params(self.h, ctypes.byref(n)))
+        self.num_params = n.value
+        return self
 
     def get_params(self):
         out = np.empty(self.num_params, np.float32)
