@@ -132,3 +132,35 @@ def test_c5_learned_proposal_full_size():
     assert s1["pairs_scan"] <= s0["pairs_scan"]
     check_sample(x, q, i0, d0, k, 0, sample_rows(m, 96))
     print(f"C5 pairs scanned {s0['pairs_scan']:.4e} -> {s1['pairs_scan']:.4e} with the proposal")
+
+
+def _device_rows(ptr, rows, ld, count):
+    """Copies rows x ld uint32 from a device pointer (the rbcStruct's dxMap)."""
+    import ctypes
+    import torch  # noqa: F401  (its HIP runtime serves this process)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    out = np.empty((rows, ld), np.uint32)
+    assert hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(ptr),
+                         ctypes.c_size_t(out.nbytes), 2) == 0
+    return out[:, :count]
+
+
+def test_oneshot_build_at_c3_scale():
+    """buildRBC at n = 2^22, d = 128, 2,048 reps, s = 2,048 (sizes at which the reference's
+    32-bit arithmetic overflows): counts within the bisection's target band, and three
+    4-rep blocks (the bisection couples the reps of a block) equal to the oracle's
+    restatement run on just those reps."""
+    x, _, _, _ = datagen.workload("lowrank16_n4M_d128", m=1)
+    n, R, s = x.shape[0], 2048, 2048
+    reps = np.sort(np.random.default_rng(7).choice(n, R, replace=False)).astype(np.uint32)
+    one = rbc.OneShotRBC(x, R, s=s, rep_ids=reps)
+    cnt = one.group_counts()
+    assert (cnt <= s).all() and (cnt >= int(0.9 * s)).all()
+    mp = _device_rows(one.rbcs.dxMap.mat, R, one.rbcs.dxMap.ld, s)
+    for b0 in (0, 1000, 2044):
+        omap, ocnt, _, bad = oracle.oneshot_build(x, reps[b0:b0 + 4], s)
+        assert bad == 0
+        np.testing.assert_array_equal(cnt[b0:b0 + 4], ocnt)
+        for r in range(4):
+            np.testing.assert_array_equal(mp[b0 + r, :ocnt[r]], omap[r, :ocnt[r]])
+    one.close()
