@@ -303,6 +303,7 @@ def main():
         idx.set_proposal(None)
         ids32 = torch.empty_like(ids)
         ds32 = torch.empty_like(ds)
+        idx.kquery_device(qd, k, ids32, ds32, precision=rbc.PREC_FP32)  # warm-up, as for BF16
         torch.cuda.synchronize()
         t0 = time.time()
         s32 = idx.kquery_device(qd, k, ids32, ds32, precision=rbc.PREC_FP32)

@@ -31,9 +31,9 @@ struct rbc_exact {
   rbc::DevBuf<float> rr;          // R x R FP32 rep-rep distances (metric value)
   std::vector<uint32_t> h_list_off;
   rbc::Workspace ws;
-  // Candidate-buffer capacity per query, grown to 1.25x the largest count seen, so that
-  // repeated queries on one index do not re-run the scan.
-  uint32_t cand_cap = 0;
+  // Candidate-buffer capacity per query and distance filter (FP32, BF16), grown to 1.25x
+  // the largest count seen, so that repeated queries on one index do not re-run the scan.
+  uint32_t cand_cap[2] = {0, 0};
   // Learned list proposal (not owned); NULL = probe only the nearest rep's list.
   struct rbc_mlp *proposal = nullptr;
 };

@@ -353,18 +353,16 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
 }
 
 // ---- BF16 probe -----------------------------------------------------------------------
-// Work item = (one list, up to 64 queries).  The MFMA operands are swapped with respect to
-// the scan: A = 16 points of the list (from the same three-stage LDS ring), B = 16 queries
-// of the wave, so that lane l's accumulator holds four points of ONE query (column l%16).
-// The four lanes l, l^16, l^32, l^48 see disjoint quarters of the list for their query;
-// each keeps its 8 smallest approximate distances sorted in registers (strict '<': among
-// equal values the earlier point stays ahead).  After the scan the four lists of a query
-// are merged by one thread into the query's K' <= 32 best.
-//
-// The result is the exact top-K' of the list unless one quarter held more than 8 of them;
-// either way every point not returned has an approximate distance >= the reported value
-// kth = min(K'-th returned, 8th kept of every quarter that filled its 8 slots), which is
-// the property the completeness test needs.
+// Work item = (one list, up to 64 probe entries; entry e belongs to query e / P).  The MFMA
+// operands are swapped with respect to the scan: A = 16 points of the list (from a
+// three-stage LDS ring), B = 16 queries of the wave, so that lane l's accumulator holds
+// four points of ONE entry (column l%16).  The four lanes l, l^16, l^32, l^48 see disjoint
+// quarters of the list for their entry; each keeps its 8 smallest approximate distances
+// sorted in registers (strict '<': among equal values the earlier point stays ahead).
+// After the scan the four lists of an entry are merged by one thread into its K' <= 32
+// best, with their approximate values.  The result is the exact top-K' of the list unless
+// one quarter held more than 8 of them; the caller only needs K' close points of the list
+// (any K' distinct points bound the k-th NN distance from above).
 constexpr int kPL = 8;                 // kept per lane
 constexpr int kPK = 4 * kPL;           // largest K'
 constexpr int kPQ = kBf16ProbeQ;       // queries per probe item
@@ -379,8 +377,9 @@ struct Bf16ProbeArgs {
   const float *xn;
   const uint32_t *perm;
   uint32_t ldb, Kp;
-  uint32_t *out_i;   // [m][Kp] original ids, ascending approximate distance
-  float *out_kth;    // [m] lower bound of D~ over the points not returned (FLT_MAX if none)
+  uint32_t P;        // entries per query: entry e belongs to query e / P
+  uint32_t *out_i;   // [entries][Kp] original ids, ascending approximate distance
+  float *out_v;      // [entries][Kp] their approximate squared distances D~
 };
 
 template <int KS>
@@ -389,11 +388,11 @@ __global__ __launch_bounds__(256, 2) void bf16_probe_kernel(Bf16ProbeArgs a) {
   __shared__ uint32_t qrow[kPQ];
   __shared__ __attribute__((aligned(16))) unsigned char ring[kStages][kVP * kRowB];
   __shared__ __attribute__((aligned(16))) float xring[kStages][kVP];  // ||x~||^2 (INF past the end)
-  __shared__ float lfull[kPQ][4];  // 8th kept value of each quarter that filled its list, else INF
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
   if (t < kPQ) qrow[t] = (uint32_t)t < it.q_count ? a.qids[it.q_begin + t] : kDummy;
   __syncthreads();
-  const uint32_t myrow = qrow[w * 16 + (l % 16)];
+  const uint32_t myent = qrow[w * 16 + (l % 16)];
+  const uint32_t myrow = myent != kDummy ? myent / a.P : kDummy;
   const bool wave_live = (uint32_t)(w * 16) < it.q_count;
   const uint32_t koff = (l / 16) * 8;
   bf16x8 Bq[KS];
@@ -485,13 +484,11 @@ __global__ __launch_bounds__(256, 2) void bf16_probe_kernel(Bf16ProbeArgs a) {
       ld[(q * 4 + g) * kPL + j] = td[j];
       lp[(q * 4 + g) * kPL + j] = tp[j];
     }
-    lfull[q][g] = tp[kPL - 1] != kDummy ? td[kPL - 1] : INFINITY;
   }
   __syncthreads();
   if (t >= kPQ || qrow[t] == kDummy) return;
-  const uint32_t row = qrow[t];
+  const uint32_t row = qrow[t];  // the entry
   uint32_t head[4] = {0, 0, 0, 0};
-  float kth = FLT_MAX;
   for (uint32_t j = 0; j < a.Kp; ++j) {
     int best = -1;
     float bv = FLT_MAX;
@@ -508,10 +505,8 @@ __global__ __launch_bounds__(256, 2) void bf16_probe_kernel(Bf16ProbeArgs a) {
     }
     if (best >= 0) ++head[best];
     a.out_i[size_t(row) * a.Kp + j] = bp != kDummy ? a.perm[x0 + bp] : kDummy;
-    if (j + 1 == a.Kp) kth = bp != kDummy ? bv : FLT_MAX;
+    a.out_v[size_t(row) * a.Kp + j] = bp != kDummy ? bv : FLT_MAX;
   }
-  for (int g = 0; g < 4; ++g) kth = fminf(kth, lfull[t][g]);
-  a.out_kth[row] = kth;
 }
 
 template <int KS>
@@ -570,8 +565,8 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
 
 namespace rbc {
 
-int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t *qorder,
-               const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_kth,
+int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t *entries, uint32_t P,
+               const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v,
                hipStream_t st) {
   if (nitems == 0) return RBC_OK;
   if (Kp > (uint32_t)kPK) {
@@ -581,7 +576,7 @@ int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t
   Bf16ProbeArgs a;
   a.items = items;
   a.nitems = nitems;
-  a.qids = qorder;
+  a.qids = entries;
   a.qb = qb;
   a.qn = qn;
   a.xb = h->xb.get();
@@ -589,8 +584,9 @@ int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t
   a.perm = h->perm.get();
   a.ldb = h->ldb;
   a.Kp = Kp;
+  a.P = P;
   a.out_i = out_i;
-  a.out_kth = out_kth;
+  a.out_v = out_v;
   switch (h->ldb / 32) {
     case 1: launch_probe<1>(a, nitems, st); break;
     case 2: launch_probe<2>(a, nitems, st); break;

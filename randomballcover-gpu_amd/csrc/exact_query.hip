@@ -4,22 +4,26 @@
 //  1. q->reps   : FP32 distances of every query to every representative.
 //  2. row stats : nearest rep of each query and an upper bound on its k-th smallest rep
 //                 distance (reps are database points, so that bounds the k-th NN distance).
-//  3. probe     : queries grouped by nearest rep scan that rep's list, keeping the best
-//                 K' = k+8 (distance, id) pairs -> an upper bound gamma on the k-th NN
-//                 distance and a cut-off tau for candidates.
+//  3. probe     : an upper bound gamma on the k-th NN distance (and a cut-off tau for
+//                 FP32 candidates) from K' = k+8 database points near the query.  FP32:
+//                 queries grouped by nearest rep scan that rep's list (and the proposal's
+//                 list) keeping the best K'; a list whose probe is complete is not scanned
+//                 again.  BF16: each query's 8 lists of smallest lane-minimum rep distance
+//                 (one slot going to the proposal's list) are scanned in one pass of MFMA
+//                 tiles; the query's best K' approximate pairs are re-ranked in FP32.
 //  4. plan      : per (query, list) lower bounds on the distance to any point of the
 //                 list -- the triangle inequality with the list radius
 //                 (d(q,r_j) - psi_j) and, for L2, the Voronoi plane between the list's
 //                 rep and the query's nearest rep -- computed in FP64 with explicit FP32
 //                 error bounds; lists whose bound exceeds gamma are pruned.  The
 //                 surviving (list, query) pairs are compacted list-major.
-//  5. scan      : every (list, <=64 queries) tile computes all distances and appends the
-//                 ones <= tau to per-query candidate buffers.
+//  5. scan      : every (list, query tile) item computes all distances and appends the
+//                 ones that can be within gamma to per-query candidate buffers.
 //  6. select    : one wave per query re-ranks the candidates with tree-ordered FP32 sums,
 //                 sorts by (distance, id) with wave-level bitonic merges, settles
 //                 near-ties in FP64 and writes the k results.
-// The BF16 variant (precision RBC_PREC_BF16) replaces steps 1, 3 and 5 with BF16 MFMA
-// tiles whose results carry explicit error bounds (exact_bf16.hip).
+// The BF16 variant (precision RBC_PREC_BF16) runs steps 3 and 5 on BF16 MFMA tiles whose
+// results carry explicit error bounds (exact_bf16.hip).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -34,9 +38,10 @@ namespace {
 
 // ---- 2. row stats ----------------------------------------------------------------
 // One wave per query over its row of R distances: nearest rep (lowest index on ties)
-// and an upper bound on the k-th smallest rep distance (k <= 64).
+// and an upper bound on the k-th smallest rep distance (k <= 64); optionally the reps of
+// the P <= 64 smallest lane minima, the lists the BF16 probe scans.
 __global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uint32_t R, uint32_t k,
-                                                 uint32_t *near, float *kth) {
+                                                 uint32_t *near, float *kth, uint32_t P, uint32_t *nearP) {
   const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
   if (q >= m) return;
   const int l = lane_id();
@@ -59,6 +64,8 @@ __global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uin
   wave_bitonic_sort(lm, li);
   if (l == 0) near[q] = li;
   if (l == (int)k - 1) kth[q] = li != kDummy ? lm : FLT_MAX;
+  // The P smallest lane minima: P distinct reps near the query (the nearest among them).
+  if (nearP && l < (int)P) nearP[size_t(q) * P + l] = li;
 }
 
 // ---- 3b. bounds from the probe ----------------------------------------------------
@@ -69,18 +76,15 @@ __global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uin
 // list that could pass tau (bit set also when the slot is unused).
 struct BoundArgs {
   const float *pd0, *pd1;        // probe values per slot (ascending), [m][Kp]
-  const float *kth0, *kth1;      // BF16 probe: approximate D~ of the K'-th per slot
   uint32_t Kp, k, m, R;
   const float *rep_kth;          // upper bound on the k-th smallest FP32 rep distance (FLT_MAX if R < k)
   const uint32_t *near, *key1, *list_len;
   double e_probe, e_rep;         // relative error of the probe values and of rep_kth
   int metric;
-  bool bf16;
-  const float *qn, *qe, *lqe, *lnm;  // BF16 error terms
-  double c_gemm;
   float *tau;
   double *gam;
   uint32_t *cert;
+  bool no_cert;                  // every list is scanned again (BF16 multi-list probe)
 };
 
 __global__ void probe_bounds(BoundArgs b) {
@@ -112,23 +116,14 @@ __global__ void probe_bounds(BoundArgs b) {
   b.tau[q] = tf;
   const double G = b.metric == 0 ? sqrt(ghi) * (1.0 + 1e-12) : ghi;
   b.gam[q] = G;
-  uint32_t c0, c1 = 1u;
-  if (!b.bf16) {
-    c0 = (b.list_len[b.near[q]] <= b.Kp || a0[b.Kp - 1] > tf) ? 1u : 0u;
-    if (two) c1 = (b.list_len[b.key1[q]] <= b.Kp || a1[b.Kp - 1] > tf) ? 1u : 0u;
-  } else {
-    // Every point of the list not kept has D~ >= the K'-th kept D~; its true distance is
-    // at least sqrt(D~ - E) - |q - q~| - max_list |x - x~|.
-    auto complete = [&](uint32_t list, float kthv) {
-      if (b.list_len[list] <= b.Kp) return 1u;
-      if (kthv >= FLT_MAX) return 0u;
-      double E = b.c_gemm * ((double)b.qn[q] + (double)b.lnm[list]) + 1e-30;
-      double lb = sqrt(fmax(0.0, (double)kthv - E)) - (double)b.qe[q] - (double)b.lqe[list];
-      return lb * (1.0 - 1e-12) > G ? 1u : 0u;
-    };
-    c0 = complete(b.near[q], b.kth0[q]);
-    if (two) c1 = complete(b.key1[q], b.kth1[q]);
+  if (b.no_cert) {
+    b.cert[q] = 0u;
+    return;
   }
+  // FP32 probe: a list is complete when it is no longer than K' or its K'-th kept value
+  // already exceeds tau (every point left out is farther than the bound).
+  uint32_t c0 = (b.list_len[b.near[q]] <= b.Kp || a0[b.Kp - 1] > tf) ? 1u : 0u, c1 = 1u;
+  if (two) c1 = (b.list_len[b.key1[q]] <= b.Kp || a1[b.Kp - 1] > tf) ? 1u : 0u;
   b.cert[q] = c0 | (c1 << 1);
 }
 
@@ -140,6 +135,66 @@ __global__ void second_probe_key(const uint32_t *prop, const uint32_t *near, uin
   if (q >= m) return;
   uint32_t p = prop[q];
   key1[q] = (p < R && p != near[q]) ? p : R;
+}
+
+// Multi-list probe: the proposal network's list takes the last of a query's P probe slots
+// unless one of the other slots already holds it.
+__global__ void proposal_slot(const uint32_t *prop, uint32_t m, uint32_t P, uint32_t R, uint32_t *keys) {
+  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const uint32_t p = prop[q];
+  if (p >= R) return;
+  uint32_t *kq = keys + size_t(q) * P;
+  for (uint32_t s = 0; s + 1 < P; ++s)
+    if (kq[s] == p) return;
+  kq[P - 1] = p;
+}
+
+// Multi-list probe, per query (one wave): the P lists' K' best approximate pairs are merged
+// by approximate value, and the K' best of them are re-ranked with tree-ordered FP32
+// distances into (pd, pi) ascending -- K' distinct database points, so the k-th of them
+// bounds the k-th NN distance from above.
+template <int METRIC, int NPL>
+__global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, const float *xp, size_t ldx,
+                                                   const uint32_t *inv, uint32_t m, uint32_t d, uint32_t P,
+                                                   uint32_t Kp, const float *pv, const uint32_t *pid,
+                                                   uint32_t *pi, float *pd) {
+  const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (q >= m) return;
+  const int l = lane_id();
+  const float *qrow = Q + size_t(q) * ldq;
+  float qv[NPL];
+#pragma unroll
+  for (int j = 0; j < NPL; ++j) {
+    uint32_t c = l + kWave * j;
+    qv[j] = c < d ? qrow[c] : 0.f;
+  }
+  float tv = FLT_MAX;
+  uint32_t ti = kDummy;
+  const uint32_t total = P * Kp;
+  const float *vq = pv + size_t(q) * total;
+  const uint32_t *iq = pid + size_t(q) * total;
+  for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
+    const uint32_t e = b0 + l;
+    float v = e < total ? vq[e] : FLT_MAX;
+    uint32_t id = e < total ? iq[e] : kDummy;
+    if (id == kDummy) v = FLT_MAX;
+    wave_bitonic_sort(v, id);
+    wave_merge_lower(tv, ti, v, id);
+  }
+  uint32_t myid = l < (int)Kp ? ti : kDummy;
+  float myv = FLT_MAX;
+  for (uint32_t cc = 0; cc < Kp; ++cc) {
+    uint32_t id = __shfl(myid, cc, kWave);
+    if (id == kDummy) continue;
+    float v = tree_dist_row<METRIC, NPL>(xp + size_t(inv[id]) * ldx, qv, d);
+    if (l == (int)cc) myv = v;
+  }
+  wave_bitonic_sort(myv, myid);
+  if (l < (int)Kp) {
+    pi[size_t(q) * Kp + l] = myid;
+    pd[size_t(q) * Kp + l] = myv;
+  }
 }
 
 // ---- 4. plan ------------------------------------------------------------------------
@@ -245,48 +300,20 @@ __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
 
 // Exact FP32 (tree-order) distances of a probe's K' candidates, re-sorted by
 // (distance, id); one wave per query.
-template <int METRIC, int NPL>
-__global__ __launch_bounds__(256) void probe_rerank(const float *Q, size_t ldq, const float *xp, size_t ldx,
-                                                    const uint32_t *inv, uint32_t m, uint32_t d, uint32_t Kp,
-                                                    uint32_t *pi, float *pd) {
-  const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
-  if (q >= m) return;
-  const int l = lane_id();
-  const float *qrow = Q + size_t(q) * ldq;
-  float qv[NPL];
-#pragma unroll
-  for (int j = 0; j < NPL; ++j) {
-    uint32_t c = l + kWave * j;
-    qv[j] = c < d ? qrow[c] : 0.f;
-  }
-  uint32_t myid = l < (int)Kp ? pi[size_t(q) * Kp + l] : kDummy;
-  float myv = FLT_MAX;
-  for (uint32_t cc = 0; cc < Kp; ++cc) {
-    uint32_t id = __shfl(myid, cc, kWave);
-    if (id == kDummy) continue;
-    float v = tree_dist_row<METRIC, NPL>(xp + size_t(inv[id]) * ldx, qv, d);
-    if (l == (int)cc) myv = v;
-  }
-  wave_bitonic_sort(myv, myid);
-  if (l < (int)Kp) {
-    pi[size_t(q) * Kp + l] = myid;
-    pd[size_t(q) * Kp + l] = myv;
-  }
-}
 
-int launch_probe_rerank(const float *Q, size_t ldq, const rbc_exact *h, uint32_t m, uint32_t Kp, uint32_t *pi,
-                        float *pd, hipStream_t st) {
-  uint32_t P = tree_width(h->d);
+int launch_probe_merge(const float *Q, size_t ldq, const rbc_exact *h, uint32_t m, uint32_t P, uint32_t Kp,
+                       const float *pv, const uint32_t *pid, uint32_t *pi, float *pd, hipStream_t st) {
+  uint32_t T = tree_width(h->d);
   dim3 g(cdiv(m, 4)), b(256);
-  if (P <= 64)
-    hipLaunchKernelGGL((probe_rerank<0, 1>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, Kp, pi, pd);
-  else if (P == 128)
-    hipLaunchKernelGGL((probe_rerank<0, 2>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, Kp, pi, pd);
+  if (T <= 64)
+    hipLaunchKernelGGL((probe_merge<0, 1>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
+                       h->d, P, Kp, pv, pid, pi, pd);
+  else if (T == 128)
+    hipLaunchKernelGGL((probe_merge<0, 2>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
+                       h->d, P, Kp, pv, pid, pi, pd);
   else
-    hipLaunchKernelGGL((probe_rerank<0, 4>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, Kp, pi, pd);
+    hipLaunchKernelGGL((probe_merge<0, 4>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
+                       h->d, P, Kp, pv, pid, pi, pd);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
@@ -359,23 +386,34 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     RBC_TRY(launch_tiles_m(h->metric, src, items.get(), (uint32_t)iv.size(), epi, st));
     RBC_HIP_TRY(hipStreamSynchronize(st));
   }
+  const bool bf16 = precision == RBC_PREC_BF16;
+  // The BF16 probe keeps at most 32 candidates, so it serves k <= 32.  It scans the Pp
+  // lists of the smallest lane minima of each query's rep distances in one pass.
+  const bool bprobe = bf16 && k <= 32;
+  const uint32_t KpUse = bprobe ? std::min<uint32_t>(32, k + 8) : Kp;
+  const uint32_t Pp = bprobe ? std::min<uint32_t>(kProbeLists, R) : 0;
   // 2. row stats
   RBC_TRY(ws.get(21, size_t(m) * 4, &p));
   uint32_t *near = static_cast<uint32_t *>(p);
   RBC_TRY(ws.get(22, size_t(m) * 4, &p));
   float *rep_kth = static_cast<float *>(p);
+  uint32_t *nearP = nullptr;
+  if (bprobe) {
+    RBC_TRY(ws.get(51, size_t(m) * Pp * 4, &p));
+    nearP = static_cast<uint32_t *>(p);
+  }
   RBC_TRY(fill_f32(rep_kth, FLT_MAX, m, st));
   hipLaunchKernelGGL(row_stats, dim3(cdiv(m, 4)), dim3(256), 0, st, Dqr, m, R, std::min(k, R), near,
-                     rep_kth);
+                     rep_kth, Pp, nearP);
   RBC_HIP_TRY(hipGetLastError());
   if (R < k) RBC_TRY(fill_f32(rep_kth, FLT_MAX, m, st));
   tm.mark();
-  // 3. probe: group queries by nearest rep and scan that list (FP32 tiles keeping the K'
-  // best, or BF16 MFMA tiles keeping the K' best approximate distances, re-ranked in FP32).
-  const bool bf16 = precision == RBC_PREC_BF16;
-  // The BF16 probe keeps at most 32 candidates, so it serves k <= 32.
-  const bool bprobe = bf16 && k <= 32;
-  const uint32_t KpUse = bprobe ? std::min<uint32_t>(32, k + 8) : Kp;
+  // 3. probe.  FP32: group queries by nearest rep and scan that list with FP32 tiles
+  // keeping the K' best (and the proposal's list likewise); lists whose probe is complete
+  // are not scanned again.  BF16: every query's Pp probe lists are scanned in one pass of
+  // BF16 MFMA tiles keeping the K' best approximate distances per (query, list); the
+  // query's K' best over its lists are re-ranked in FP32.  Only the bound is used from it:
+  // the scan visits every surviving list, probed or not.
   uint16_t *qb = nullptr;
   float *qn = nullptr, *qe = nullptr;
   if (bf16) {
@@ -391,7 +429,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint32_t npitems = 0;
   // Runs one probe slot for the queries whose key (list) is < R.
   auto run_probe = [&](const uint32_t *keys, size_t slot_order, size_t slot_off, size_t slot_d, size_t slot_i,
-                       size_t slot_kth, float **pd_out, uint32_t **pi_out, float **kth_out) -> int {
+                       float **pd_out, uint32_t **pi_out) -> int {
     void *pp;
     RBC_TRY(ws.get(slot_order, size_t(m) * 4, &pp));
     uint32_t *order = static_cast<uint32_t *>(pp);
@@ -399,35 +437,57 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     uint32_t *off = static_cast<uint32_t *>(pp);
     RBC_TRY(counting_sort(keys, m, R, order, off, ws, 0, st));
     RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st,
-                              bprobe ? kBf16ProbeQ : (uint32_t)kTQ));
+                              (uint32_t)kTQ));
     RBC_TRY(ws.get(slot_d, size_t(m) * KpUse * 4, &pp));
     float *pd = static_cast<float *>(pp);
     RBC_TRY(ws.get(slot_i, size_t(m) * KpUse * 4, &pp));
     uint32_t *pi = static_cast<uint32_t *>(pp);
     RBC_TRY(fill_f32(pd, FLT_MAX, size_t(m) * KpUse, st));
     RBC_TRY(fill_u32(pi, kDummy, size_t(m) * KpUse, st));
-    if (bprobe) {
-      RBC_TRY(ws.get(slot_kth, size_t(m) * 4, &pp));
-      float *kth = static_cast<float *>(pp);
-      RBC_TRY(fill_f32(kth, FLT_MAX, m, st));
-      RBC_TRY(bf16_probe(h, qb, qn, order, pitems.get(), npitems, KpUse, pi, kth, st));
-      RBC_TRY(launch_probe_rerank(qd, ldq, h, m, KpUse, pi, pd, st));
-      *kth_out = kth;
-    } else {
-      TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, order, d, h->dpad};
-      EpiTopK epi{KpUse, pd, pi, KpUse, 0};
-      RBC_TRY(launch_tiles_m(h->metric, src, pitems.get(), npitems, epi, st));
-    }
+    TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, order, d, h->dpad};
+    EpiTopK epi{KpUse, pd, pi, KpUse, 0};
+    RBC_TRY(launch_tiles_m(h->metric, src, pitems.get(), npitems, epi, st));
     *pd_out = pd;
     *pi_out = pi;
     return RBC_OK;
   };
-  float *probe_d = nullptr, *probe_d1 = nullptr, *kth0 = nullptr, *kth1 = nullptr;
+  float *probe_d = nullptr, *probe_d1 = nullptr;
   uint32_t *probe_i = nullptr, *probe_i1 = nullptr, *key1 = nullptr;
-  RBC_TRY(run_probe(near, 23, 24, 25, 26, 49, &probe_d, &probe_i, &kth0));
-  // 3a. second probe: the proposal network's top-1 list where it differs from the
+  if (bprobe) {
+    // Entries e = q * Pp + s (query q, probe slot s), keyed by list, sorted list-major.
+    if (h->proposal) {
+      RBC_TRY(ws.get(43, size_t(m) * 4, &p));
+      uint32_t *prop = static_cast<uint32_t *>(p);
+      RBC_TRY(mlp_propose(h->proposal, qd, ldq, m, prop, st));
+      hipLaunchKernelGGL(proposal_slot, dim3(cdiv(m, 256)), dim3(256), 0, st, prop, m, Pp, R, nearP);
+      RBC_HIP_TRY(hipGetLastError());
+    }
+    const size_t ne = size_t(m) * Pp;
+    RBC_TRY(ws.get(23, ne * 4, &p));
+    uint32_t *order = static_cast<uint32_t *>(p);
+    RBC_TRY(ws.get(24, (size_t(R) + 1) * 4, &p));
+    uint32_t *off = static_cast<uint32_t *>(p);
+    RBC_TRY(counting_sort(nearP, ne, R, order, off, ws, 0, st));
+    RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st,
+                              kBf16ProbeQ));
+    RBC_TRY(ws.get(52, ne * KpUse * 4, &p));
+    float *ev = static_cast<float *>(p);
+    RBC_TRY(ws.get(53, ne * KpUse * 4, &p));
+    uint32_t *eid = static_cast<uint32_t *>(p);
+    RBC_TRY(fill_f32(ev, FLT_MAX, ne * KpUse, st));
+    RBC_TRY(fill_u32(eid, kDummy, ne * KpUse, st));
+    RBC_TRY(bf16_probe(h, qb, qn, order, Pp, pitems.get(), npitems, KpUse, eid, ev, st));
+    RBC_TRY(ws.get(25, size_t(m) * KpUse * 4, &p));
+    probe_d = static_cast<float *>(p);
+    RBC_TRY(ws.get(26, size_t(m) * KpUse * 4, &p));
+    probe_i = static_cast<uint32_t *>(p);
+    RBC_TRY(launch_probe_merge(qd, ldq, h, m, Pp, KpUse, ev, eid, probe_i, probe_d, st));
+  } else {
+    RBC_TRY(run_probe(near, 23, 24, 25, 26, &probe_d, &probe_i));
+  }
+  // 3a. FP32 second probe: the proposal network's top-1 list where it differs from the
   // nearest rep's.
-  if (h->proposal) {
+  if (h->proposal && !bprobe) {
     RBC_TRY(ws.get(43, size_t(m) * 4, &p));
     uint32_t *prop = static_cast<uint32_t *>(p);
     RBC_TRY(mlp_propose(h->proposal, qd, ldq, m, prop, st));
@@ -435,7 +495,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     key1 = static_cast<uint32_t *>(p);
     hipLaunchKernelGGL(second_probe_key, dim3(cdiv(m, 256)), dim3(256), 0, st, prop, near, m, R, key1);
     RBC_HIP_TRY(hipGetLastError());
-    RBC_TRY(run_probe(key1, 45, 46, 47, 48, 50, &probe_d1, &probe_i1, &kth1));
+    RBC_TRY(run_probe(key1, 45, 46, 47, 48, &probe_d1, &probe_i1));
   }
   // 3b. bounds
   RBC_TRY(ws.get(27, size_t(m) * 4, &p));
@@ -448,8 +508,6 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     BoundArgs ba{};
     ba.pd0 = probe_d;
     ba.pd1 = probe_d1;
-    ba.kth0 = kth0;
-    ba.kth1 = kth1;
     ba.Kp = KpUse;
     ba.k = k;
     ba.m = m;
@@ -461,12 +519,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     ba.e_probe = bprobe ? tree_rel_err(tree_width(d)) : e;
     ba.e_rep = e;
     ba.metric = h->metric;
-    ba.bf16 = bprobe;
-    ba.qn = qn;
-    ba.qe = qe;
-    ba.lqe = h->list_qerr.get();
-    ba.lnm = h->list_nmax.get();
-    ba.c_gemm = bprobe ? bf16_gemm_err(h) : 0.0;
+    ba.no_cert = bprobe;
     ba.tau = tau;
     ba.gam = gam;
     ba.cert = cert;
@@ -503,7 +556,8 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
                             bf16 ? kBf16ScanQ : (uint32_t)kTQ));
   tm.mark();
   // 5. scan with candidate emission; re-run with a larger buffer on overflow.
-  uint32_t cap = std::max<uint32_t>({256u, 4 * k, h->cand_cap});
+  uint32_t &cap_seen = h->cand_cap[bf16 ? 1 : 0];
+  uint32_t cap = std::max<uint32_t>({256u, 4 * k, cap_seen});
   RBC_TRY(ws.get(36, size_t(m) * 4, &p));
   uint32_t *cand_n = static_cast<uint32_t *>(p);
   float *cand_d = nullptr;
@@ -533,7 +587,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
       rerank_total += v;
       if (v > cap) fallbacks += 1;
     }
-    h->cand_cap = std::max(h->cand_cap, mx + mx / 4);
+    cap_seen = std::max(cap_seen, mx + mx / 4);
     if (mx <= cap) break;
     if (attempt == 1) {
       set_error("candidate buffer overflow persisted (%u > %u)", mx, cap);
@@ -588,7 +642,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     void *pq0;
     RBC_TRY(ws.get(24, (size_t(R) + 1) * 4, &pq0));
     RBC_HIP_TRY(hipMemcpy(hq.data(), pq0, (R + 1) * 4, hipMemcpyDeviceToHost));
-    if (h->proposal) {
+    if (h->proposal && !bprobe) {
       void *pq1;
       RBC_TRY(ws.get(46, (size_t(R) + 1) * 4, &pq1));
       RBC_HIP_TRY(hipMemcpy(hq1.data(), pq1, (R + 1) * 4, hipMemcpyDeviceToHost));

@@ -32,9 +32,13 @@ int launch_select(const SelectArgs &a, int metric, hipStream_t st);
 // BF16 MFMA stages (exact_bf16.hip).
 int bf16_prepare_queries(rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint16_t *qb, float *qn,
                          float *qe, hipStream_t st);
-int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t *qorder,
-               const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_kth,
+// Probe entries e (query e / P): for each, the K' smallest approximate distances of its
+// item's list, ascending, as ids out_i[e*Kp..] and values out_v[e*Kp..].
+int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t *entries, uint32_t P,
+               const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v,
                hipStream_t st);
+// Lists probed per query by the BF16 probe.
+constexpr uint32_t kProbeLists = 8;
 // Queries per work item of the BF16 scan (its items must be cut at this size).
 constexpr uint32_t kBf16ScanQ = 256;
 // Queries per work item of the BF16 probe.
