@@ -29,6 +29,9 @@ struct rbc_exact {
   rbc::DevBuf<uint32_t> rep_ids;  // R
   rbc::DevBuf<float> radius;      // R, metric distance, rounded up
   rbc::DevBuf<float> rr;          // R x R FP32 rep-rep distances (metric value)
+  // L2: R x R denominators of the Voronoi-plane bound, 2 sqrt(rr (1 + e)) rounded up to
+  // float (derived from rr at build and at load; e = fma_rel_err(dpad)).
+  rbc::DevBuf<float> rr_den;
   std::vector<uint32_t> h_list_off;
   rbc::Workspace ws;
   // Candidate-buffer capacity per query and distance filter (FP32, BF16), grown to 1.25x
@@ -43,6 +46,9 @@ namespace rbc {
 // Relative error bound of an FP32 distance summed with one FMA (or add) per coordinate
 // over dpad coordinates: (dpad + 4) u with u = 2^-24, plus 1% headroom.
 inline double fma_rel_err(uint32_t dpad) { return (double(dpad) + 4.0) * 0x1p-24 * 1.01; }
+
+// Fills h->rr_den from h->rr (exact_build.hip).
+int voronoi_denominators(rbc_exact *h, hipStream_t st);
 // Relative error bound of the tree-ordered re-rank over P (power of two) coordinates.
 inline double tree_rel_err(uint32_t P) {
   int lg = 0;

@@ -27,6 +27,21 @@ __global__ void gather_rows(const float *x, size_t ldx, const uint32_t *ids, uin
   }
 }
 
+// den = 2 sqrt(rr (1 + e)) rounded up to float (0 where rr <= 0: no plane bound).
+__global__ void voronoi_den(const float *rr, size_t n, double e, float *den) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const double v = (double)rr[i];
+    if (v <= 0.0) {
+      den[i] = 0.f;
+      continue;
+    }
+    const double t = 2.0 * sqrt(v * (1.0 + e)) * (1.0 + 1e-15);
+    float f = (float)t;
+    if ((double)f < t) f = nextafterf(f, INFINITY);
+    den[i] = f;
+  }
+}
+
 __global__ void invert_perm(const uint32_t *perm, uint32_t n, uint32_t *inv) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) inv[perm[i]] = i;
@@ -154,6 +169,15 @@ uint64_t splitmix64(uint64_t &s) {
 
 }  // namespace
 
+int voronoi_denominators(rbc_exact *h, hipStream_t st) {
+  const size_t n = size_t(h->R) * h->R;
+  RBC_TRY(h->rr_den.alloc(n));
+  const uint32_t g = (uint32_t)std::min<size_t>(cdivz(n, 256), 4096);
+  hipLaunchKernelGGL(voronoi_den, dim3(g), dim3(256), 0, st, h->rr.get(), n, fma_rel_err(h->dpad), h->rr_den.get());
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
 // Seeded partial Fisher-Yates: l distinct indices in [0, n) (the role of subRandPerm,
 // utils.cu:17-37, with a reproducible generator in place of the clock-seeded rand()).
 std::vector<uint32_t> sample_distinct(uint32_t l, uint32_t n, uint64_t seed) {
@@ -280,6 +304,7 @@ int rbc_exact_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32
     RBC_TRY(launch_tiles_m(metric, src, items.get(), (uint32_t)iv.size(), epi, st));
     RBC_HIP_TRY(hipStreamSynchronize(st));
   }
+  RBC_TRY(voronoi_denominators(h, st));
   h->h_list_off.resize(R + 1);
   RBC_HIP_TRY(hipMemcpy(h->h_list_off.data(), h->list_off.get(), (R + 1) * 4, hipMemcpyDeviceToHost));
   RBC_HIP_TRY(hipDeviceSynchronize());

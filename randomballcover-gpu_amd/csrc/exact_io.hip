@@ -155,6 +155,7 @@ int rbc_exact_load(const char *path, rbc_exact **out) {
   RBC_TRY(read_array(fp, h->rep_ids, R, stage, "rep_ids"));
   RBC_TRY(read_array(fp, h->radius, R, stage, "radius"));
   RBC_TRY(read_array(fp, h->rr, R * R, stage, "rr"));
+  RBC_TRY(voronoi_denominators(h.get(), nullptr));
   h->h_list_off.resize(R + 1);
   RBC_HIP_TRY(hipMemcpy(h->h_list_off.data(), h->list_off.get(), (R + 1) * 4, hipMemcpyDeviceToHost));
   RBC_ARG_CHECK(h->h_list_off[0] == 0 && h->h_list_off[R] == h->n, "index file lists do not cover the database");

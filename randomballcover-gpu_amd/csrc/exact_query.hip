@@ -137,6 +137,33 @@ __global__ void second_probe_key(const uint32_t *prop, const uint32_t *near, uin
   key1[q] = (p < R && p != near[q]) ? p : R;
 }
 
+__global__ void fill_u64(unsigned long long *p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// out[0] = max count, out[1] = sum of counts, out[2] = number of counts above cap.
+__global__ __launch_bounds__(256) void cand_stats(const uint32_t *cnt, uint32_t m, uint32_t cap,
+                                                  unsigned long long *out) {
+  uint32_t mx = 0, over = 0;
+  unsigned long long sum = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    const uint32_t v = cnt[i];
+    mx = max(mx, v);
+    sum += v;
+    over += v > cap;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, kWave));
+    sum += __shfl_xor(sum, o, kWave);
+    over += (uint32_t)__shfl_xor((int)over, o, kWave);
+  }
+  if (lane_id() == 0) {
+    atomicMax(&out[0], (unsigned long long)mx);
+    atomicAdd(&out[1], sum);
+    atomicAdd(&out[2], (unsigned long long)over);
+  }
+}
+
 // Multi-list probe: the proposal network's list takes the last of a query's P probe slots
 // unless one of the other slots already holds it.
 __global__ void proposal_slot(const uint32_t *prop, uint32_t m, uint32_t P, uint32_t R, uint32_t *keys) {
@@ -206,7 +233,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
                                                   const uint32_t *near, const uint32_t *key1,
                                                   const double *gam, const uint32_t *cert,
                                                   const float *radius,
-                                                  const float *rr, double e, int metric,
+                                                  const float *rr_den, double e, int metric,
                                                   uint32_t W, uint32_t *bits, uint32_t *list_cnt) {
   const uint32_t q0 = blockIdx.x * kPlanQ;
   const int l = lane_id();
@@ -226,11 +253,26 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
         } else if (j == j1) {
           keep = (cb & 2u) == 0;
         } else {
-          double Dqj = (double)Dqr[size_t(q) * R + j];
-          double Dq0 = (double)Dqr[size_t(q) * R + j0];
-          double lb = list_lower_bound(Dqj, Dq0, (double)rr[size_t(j0) * R + j], psi, e, false,
-                                       metric);
-          keep = lb <= gam[q];
+          const double Dqj = (double)Dqr[size_t(q) * R + j];
+          const double G = gam[q];
+          if (metric == 0) {
+            // list_lower_bound's test without square roots or divisions: the triangle
+            // bound sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as
+            // Dqj (1-e) <= (G' + psi)^2, and the plane bound num / den <= G' as
+            // num <= G' den with den rounded up; both keep at least what it keeps.
+            const double Gp = G * (1.0 + 2e-12);
+            const double rhs = Gp + psi;
+            keep = Dqj * (1.0 - e) <= rhs * rhs * (1.0 + 1e-13);
+            const double den = (double)rr_den[size_t(j0) * R + j];
+            if (keep && den > 0.0) {
+              const double Dq0 = (double)Dqr[size_t(q) * R + j0];
+              const double num = Dqj * (1.0 - e) - Dq0 * (1.0 + e) - 2.0 * e * psi * psi;
+              keep = !(num > 0.0) || num <= Gp * den * (1.0 + 1e-13);
+            }
+          } else {
+            const double lb = list_lower_bound(Dqj, 0.0, 0.0, psi, e, true, metric);
+            keep = lb <= G;
+          }
         }
       }
       cnt += keep;
@@ -538,7 +580,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
   hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, Dqr, m, R, near, key1, gam, cert,
-                     h->radius.get(), h->rr.get(), e, h->metric, W, bits, lcnt);
+                     h->radius.get(), h->rr_den.get(), e, h->metric, W, bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));
   RBC_TRY(ws.get(33, 4, &p));
@@ -576,17 +618,20 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
       EpiEmit epi{tau, cand_d, cand_i, cand_n, cap};
       RBC_TRY(launch_tiles_m(h->metric, src, sitems.get(), nsitems, epi, st));
     }
-    // Largest candidate count.
-    std::vector<uint32_t> hn(m);
-    RBC_HIP_TRY(hipMemcpyAsync(hn.data(), cand_n, size_t(m) * 4, hipMemcpyDeviceToHost, st));
+    // Largest candidate count, total and overflows, reduced on the device (a pageable copy
+    // of all m counts would cost more than the reduction).
+    RBC_TRY(ws.get(54, 3 * 8, &p));
+    unsigned long long *cs = static_cast<unsigned long long *>(p);
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, cs, 3);
+    hipLaunchKernelGGL(cand_stats, dim3(std::min<uint32_t>(cdiv(m, 256), 1024)), dim3(256), 0, st, cand_n, m, cap,
+                       cs);
+    RBC_HIP_TRY(hipGetLastError());
+    unsigned long long hcs[3];
+    RBC_HIP_TRY(hipMemcpyAsync(hcs, cs, sizeof(hcs), hipMemcpyDeviceToHost, st));
     RBC_HIP_TRY(hipStreamSynchronize(st));
-    uint32_t mx = 0;
-    rerank_total = 0;
-    for (uint32_t v : hn) {
-      mx = std::max(mx, v);
-      rerank_total += v;
-      if (v > cap) fallbacks += 1;
-    }
+    const uint32_t mx = (uint32_t)hcs[0];
+    rerank_total = (double)hcs[1];
+    fallbacks += (double)hcs[2];
     cap_seen = std::max(cap_seen, mx + mx / 4);
     if (mx <= cap) break;
     if (attempt == 1) {
