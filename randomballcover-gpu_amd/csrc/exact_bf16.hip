@@ -212,28 +212,31 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   if (nsteps > 0) issue(0);
   if (nsteps > 1) issue(1);
   if (nsteps > 2) issue(2);
-  {
-    const bool live = (uint32_t)t < it.q_count;
-    const uint32_t row = live ? a.qids[it.q_begin + t] : kDummy;
-    qrow[t] = row;
-    sh[t] = live ? bf16_threshold(a.gam[row], a.qe[row], a.qn[row], a.lqe[it.aux], a.lnm[it.aux], a.c_gemm)
-                 : -INFINITY;
-    if (t == 0) cs_n = 0;
-  }
-  __syncthreads();
-  // This wave's queries: A operands and the initial accumulators.
+  // Thread t = 64 w + l owns query slot t, so a wave already holds the rows of its own 64
+  // queries: the A operands are fetched through lane shuffles, in parallel with the
+  // threshold inputs, without waiting for the LDS copy of the rows.
+  const bool live = (uint32_t)t < it.q_count;
+  const uint32_t myq = live ? a.qids[it.q_begin + t] : kDummy;
   const bool wave_live = (uint32_t)(w * 64) < it.q_count;
   bf16x8 A[4][KS];
-  f32x4 S[4];
   const uint32_t koff = (l / 16) * 8;
 #pragma unroll
   for (int qb = 0; qb < 4; ++qb) {
-    const int q0 = w * 64 + qb * 16;
-    const uint32_t row = qrow[q0 + (l % 16)];
+    const uint32_t row = (uint32_t)__shfl((int)myq, qb * 16 + (l % 16), kWave);
 #pragma unroll
     for (int s = 0; s < KS; ++s)
       A[qb][s] = row != kDummy ? load8(a.qb + size_t(row) * a.ldb + s * 32 + koff) : bf16x8{};
-    const float4 s4 = *reinterpret_cast<const float4 *>(&sh[q0 + (l / 16) * 4]);
+  }
+  qrow[t] = myq;
+  sh[t] = live ? bf16_threshold(a.gam[myq], a.qe[myq], a.qn[myq], a.lqe[it.aux], a.lnm[it.aux], a.c_gemm)
+               : -INFINITY;
+  if (t == 0) cs_n = 0;
+  __syncthreads();
+  // The initial accumulators.
+  f32x4 S[4];
+#pragma unroll
+  for (int qb = 0; qb < 4; ++qb) {
+    const float4 s4 = *reinterpret_cast<const float4 *>(&sh[w * 64 + qb * 16 + (l / 16) * 4]);
     S[qb] = f32x4{s4.x, s4.y, s4.z, s4.w};
   }
   // Per-lane read addresses (stage 0): B fragment (pb, ks) is row pb*16 + l%16, chunk
