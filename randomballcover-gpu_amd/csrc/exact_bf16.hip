@@ -61,7 +61,8 @@ __global__ __launch_bounds__(256) void prep_queries(const float *q, size_t ldq, 
 
 struct Bf16ScanArgs {
   const ScanItem *items;
-  uint32_t nitems;
+  const uint32_t *bounds;  // [kXCD + 1]: items [bounds[x], bounds[x+1]) run on XCD x
+  uint32_t *next;          // [kXCD]: items of XCD x's range taken so far
   const uint32_t *qids;  // list-major (query, list) pairs: query ids
   const uint16_t *qb;    // queries, BF16, ldb
   const float *qn;       // |q~|^2
@@ -169,12 +170,113 @@ __device__ __forceinline__ void ring_read(const uint32_t (&ad)[8], uint32_t ah, 
       : "memory");
 }
 
+// Splits the list-major scan items into kXCD contiguous ranges of about equal work and
+// clears the per-range counters.  The work of an item is its number of steps times (its
+// live waves + 1): the live waves issue the MFMAs, and every item pays the ring loads and
+// barriers of all its steps.  A list's items all have the same steps, so a range boundary
+// inside a list falls between its full (4-wave) items.  One block of 1024 threads; items
+// are numbered as make_bucket_items numbers them (bucket-major, ceil(count / kVQ) each).
+__global__ __launch_bounds__(1024) void scan_split(const uint32_t *qoff, const uint32_t *len, uint32_t R,
+                                                   uint32_t *bounds, uint32_t *next) {
+  __shared__ unsigned long long sw[16];
+  __shared__ uint32_t si[16];
+  __shared__ unsigned long long carry_w;
+  __shared__ uint32_t carry_i;
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  auto bucket = [&](uint32_t b, unsigned long long &wt, uint32_t &ni, uint32_t &steps) {
+    wt = 0;
+    ni = 0;
+    steps = 0;
+    if (b < R) {
+      const uint32_t cnt = qoff[b + 1] - qoff[b];
+      ni = (cnt + kVQ - 1) / kVQ;
+      steps = (len[b] + kVP - 1) / kVP;
+      wt = (unsigned long long)steps * ((cnt + kWave - 1) / kWave + ni);
+    }
+  };
+  // Total work.
+  unsigned long long tw = 0;
+  uint32_t ti = 0;
+  for (uint32_t b = t; b < R; b += 1024) {
+    unsigned long long wt;
+    uint32_t ni, steps;
+    bucket(b, wt, ni, steps);
+    tw += wt;
+    ti += ni;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    tw += __shfl_xor(tw, o, kWave);
+    ti += __shfl_xor(ti, o, kWave);
+  }
+  if (l == 0) {
+    sw[w] = tw;
+    si[w] = ti;
+  }
+  __syncthreads();
+  unsigned long long W = 0;
+  uint32_t NI = 0;
+  for (int i = 0; i < 16; ++i) {
+    W += sw[i];
+    NI += si[i];
+  }
+  // Boundaries not placed below (no work at all) leave everything to XCD 0's range.
+  if (t <= kXCD) bounds[t] = t == 0 ? 0 : NI;
+  if (t < kXCD) next[t] = 0;
+  if (t == 0) {
+    carry_w = 0;
+    carry_i = 0;
+  }
+  __syncthreads();
+  // Exclusive prefix of work and items, 1024 buckets at a time; the bucket holding target
+  // x W / kXCD places boundary x.
+  for (uint32_t c0 = 0; c0 < R; c0 += 1024) {
+    unsigned long long wt;
+    uint32_t ni, steps;
+    bucket(c0 + t, wt, ni, steps);
+    unsigned long long pw = wt;
+    uint32_t pi = ni;
+    for (int o = 1; o < kWave; o <<= 1) {
+      const unsigned long long nw = __shfl_up(pw, o, kWave);
+      const uint32_t nn = __shfl_up(pi, o, kWave);
+      if (l >= o) {
+        pw += nw;
+        pi += nn;
+      }
+    }
+    __syncthreads();
+    if (l == kWave - 1) {
+      sw[w] = pw;
+      si[w] = pi;
+    }
+    __syncthreads();
+    unsigned long long bw = carry_w;
+    uint32_t bi = carry_i;
+    for (int i = 0; i < w; ++i) {
+      bw += sw[i];
+      bi += si[i];
+    }
+    pw = bw + pw - wt;  // exclusive
+    pi = bi + pi - ni;
+    if (wt > 0) {
+      for (int x = 1; x < kXCD; ++x) {
+        const unsigned long long T = W * (unsigned long long)x / kXCD;
+        if (T >= pw && T < pw + wt) {
+          const unsigned long long full = (unsigned long long)steps * (4 + 1);
+          const uint32_t j = (uint32_t)min((unsigned long long)(ni - 1), (T - pw) / full);
+          bounds[x] = pi + j;
+        }
+      }
+    }
+    __syncthreads();
+    if (t == 1023) {
+      carry_w = pw + wt;
+      carry_i = pi + ni;
+    }
+  }
+}
+
 template <int KS>
 __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
-  const uint32_t per = (a.nitems + kXCD - 1) / kXCD;
-  const uint32_t item = (blockIdx.x % kXCD) * per + blockIdx.x / kXCD;
-  if (item >= a.nitems) return;
-  const ScanItem it = a.items[item];
   // Emission test per (query i, point x): MFMA(s_i + q~.x~) >= |x~|^2 / 2 with s_i the
   // threshold of the (query, list) pair (bf16_threshold), so the test never drops an x with
   // |q - x| <= G_i.
@@ -186,7 +288,19 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   __shared__ float cs_v[kCS];     // candidate: approximate squared distance minus |q~|^2
   __shared__ uint8_t cs_q[kCS];   // candidate: query slot in the item
   __shared__ uint32_t cs_n;
+  __shared__ uint32_t s_item;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  // Persistent blocks: block b serves XCD b % kXCD (workgroups are dispatched to the XCDs
+  // round robin) and takes that XCD's items in order, so the items of a list run at the
+  // same time on one XCD and re-read the list from its L2.
+  const uint32_t xcd = blockIdx.x % kXCD;
+  const uint32_t lo = a.bounds[xcd], hi = a.bounds[xcd + 1];
+  for (;;) {
+  if (t == 0) s_item = lo + atomicAdd(&a.next[xcd], 1u);
+  __syncthreads();
+  const uint32_t item = s_item;
+  if (item >= hi) break;
+  const ScanItem it = a.items[item];
   const uint32_t x0 = it.x_begin, np = it.x_count;
   const uint32_t nsteps = (np + kVP - 1) / kVP;
   // Loads of one step: two 1 KB blocks per wave (rows 8w..8w+7, all 16 chunks; lane l
@@ -352,6 +466,9 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
       a.cand_d[size_t(row) * a.cap + gs] = a.qn[row] + cs_v[i];
       a.cand_i[size_t(row) * a.cap + gs] = a.perm[x0 + cs_p[i]];
     }
+  }
+  // The next item rewrites qrow, sh, cs_* and s_item.
+  __syncthreads();
   }
 }
 
@@ -526,14 +643,43 @@ int bf16_prepare_queries(rbc_exact *h, const float *q, size_t ldq, uint32_t m, u
   return RBC_OK;
 }
 
+// One persistent block per resident slot: (CUs x blocks per CU), a multiple of kXCD so that
+// every XCD gets blocks.
+template <int KS>
+int launch_scan(const Bf16ScanArgs &a, uint32_t nitems, hipStream_t st) {
+  static int per_cu = 0, cus = 0;
+  if (per_cu == 0) {
+    int dev = 0;
+    RBC_HIP_TRY(hipGetDevice(&dev));
+    RBC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RBC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bf16_scan_kernel<KS>, 256, 0));
+    per_cu = std::max(per_cu, 1);
+  }
+  uint32_t grid = (uint32_t)(cus * per_cu);
+  grid = std::min(grid, nitems + kXCD);
+  grid = cdiv(std::max(grid, (uint32_t)kXCD), kXCD) * kXCD;
+  hipLaunchKernelGGL(bf16_scan_kernel<KS>, dim3(grid), dim3(256), 0, st, a);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
 int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe, const double *gam,
-              const uint32_t *qlist, const ScanItem *items, uint32_t nitems, float *cand_d,
-              uint32_t *cand_i, uint32_t *cand_n, uint32_t cap, hipStream_t st) {
+              const uint32_t *qlist, const uint32_t *list_qoff, const ScanItem *items, uint32_t nitems,
+              uint32_t *sched, float *cand_d, uint32_t *cand_i, uint32_t *cand_n, uint32_t cap,
+              hipStream_t st) {
   if (nitems == 0) return RBC_OK;
   const uint32_t KS = h->ldb / 32;
+  if (KS < 1 || KS > 4) {
+    set_error("BF16 filter supports d <= 128 (d=%u)", h->d);
+    return RBC_ERR_LIMIT;
+  }
+  hipLaunchKernelGGL(scan_split, dim3(1), dim3(1024), 0, st, list_qoff, h->list_len.get(), h->R, sched,
+                     sched + kXCD + 1);
+  RBC_HIP_TRY(hipGetLastError());
   Bf16ScanArgs a;
   a.items = items;
-  a.nitems = nitems;
+  a.bounds = sched;
+  a.next = sched + kXCD + 1;
   a.qids = qlist;
   a.qb = qb;
   a.qn = qn;
@@ -550,17 +696,12 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
   a.cand_i = cand_i;
   a.cand_n = cand_n;
   a.cap = cap;
-  const uint32_t grid = cdiv(nitems, kXCD) * kXCD;
   switch (KS) {
-    case 1: hipLaunchKernelGGL(bf16_scan_kernel<1>, dim3(grid), dim3(256), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(bf16_scan_kernel<2>, dim3(grid), dim3(256), 0, st, a); break;
-    case 3: hipLaunchKernelGGL(bf16_scan_kernel<3>, dim3(grid), dim3(256), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(bf16_scan_kernel<4>, dim3(grid), dim3(256), 0, st, a); break;
-    default:
-      set_error("BF16 filter supports d <= 128 (d=%u)", h->d);
-      return RBC_ERR_LIMIT;
+    case 1: RBC_TRY(launch_scan<1>(a, nitems, st)); break;
+    case 2: RBC_TRY(launch_scan<2>(a, nitems, st)); break;
+    case 3: RBC_TRY(launch_scan<3>(a, nitems, st)); break;
+    default: RBC_TRY(launch_scan<4>(a, nitems, st)); break;
   }
-  RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
 

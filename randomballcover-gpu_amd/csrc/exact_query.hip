@@ -612,7 +612,9 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     cand_i = static_cast<uint32_t *>(p);
     RBC_TRY(fill_u32(cand_n, 0, m, st));
     if (precision == RBC_PREC_BF16) {
-      RBC_TRY(bf16_scan(h, qb, qn, qe, gam, qlist, sitems.get(), nsitems, cand_d, cand_i, cand_n, cap, st));
+      RBC_TRY(ws.get(55, 32 * 4, &p));
+      RBC_TRY(bf16_scan(h, qb, qn, qe, gam, qlist, lcnt, sitems.get(), nsitems, static_cast<uint32_t *>(p),
+                        cand_d, cand_i, cand_n, cap, st));
     } else {
       TileSrc src{h->xp.get(), h->ld, nullptr, h->perm.get(), qd, ldq, qlist, d, h->dpad};
       EpiEmit epi{tau, cand_d, cand_i, cand_n, cap};

@@ -45,9 +45,12 @@ constexpr uint32_t kBf16ScanQ = 256;
 constexpr uint32_t kBf16ProbeQ = 64;
 // Relative error constant of the BF16 GEMM distance: gamma_K + 6u', u' = 2^-23.
 double bf16_gemm_err(const rbc_exact *h);
+// Scan items must come from make_bucket_items over list_qoff (R + 1 offsets of the
+// list-major query ids) with tile kBf16ScanQ; sched holds 17 uint32 of device scratch.
 int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe, const double *gam,
-              const uint32_t *qlist, const ScanItem *items, uint32_t nitems, float *cand_d,
-              uint32_t *cand_i, uint32_t *cand_n, uint32_t cap, hipStream_t st);
+              const uint32_t *qlist, const uint32_t *list_qoff, const ScanItem *items, uint32_t nitems,
+              uint32_t *sched, float *cand_d, uint32_t *cand_i, uint32_t *cand_n, uint32_t cap,
+              hipStream_t st);
 
 // Emission threshold of the BF16 list scan for one (query, list) pair.  The scan emits x
 // when MFMA(s + q~.x~) >= |x~|^2 / 2 with the accumulator initialised to the returned s =
