@@ -117,27 +117,33 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p) {
   return (uint32_t)(uintptr_t)(const lds_void *)p;
 }
 
-// Direct global->LDS loads (lane l writes bytes [16 l, 16 l + 16) or [4 l, 4 l + 4) after
-// the wave-uniform LDS address).  Issued from asm so that the compiler's wait insertion,
+// Direct global->LDS loads (global_load_lds_dword[x4]: lane l writes bytes [16 l, 16 l + 16)
+// or [4 l, 4 l + 4) after the wave-uniform LDS address in M0).  Issued from asm so that the compiler's wait insertion,
 // which would otherwise drain them before unrelated instructions, leaves them alone: the
 // kernel waits for them itself (wait_vm) before the barrier that publishes the data.
 // M0 is saved and restored inside the block, so the compiler's use of it is unaffected;
 // the s_nop is the wait state the hardware needs between writing M0 and an LDS DMA that
 // reads it (the compiler inserts the same for its own LDS DMA).
-__device__ __forceinline__ void dma16(const void *g, uint32_t lds) {
+// The three ring loads of one wave for one step: two 1 KB row blocks and the norms, from
+// scalar base addresses (xs, ns) with 32-bit per-lane byte offsets, into LDS at m0a, m0b
+// and m0n.
+__device__ __forceinline__ void dma_step(const void *xs, const void *ns, uint32_t o0, uint32_t o1, uint32_t on,
+                                         uint32_t m0a, uint32_t m0b, uint32_t m0n) {
   uint32_t keep;
   asm volatile(
-      "s_mov_b32 %0, m0\n\ts_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off\n\ts_mov_b32 m0, %0"
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %4\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %1, %7\n\t"
+      "s_mov_b32 m0, %5\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %2, %7\n\t"
+      "s_mov_b32 m0, %6\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dword %3, %8\n\t"
+      "s_mov_b32 m0, %0"
       : "=&s"(keep)
-      : "v"(g), "s"(lds)
-      : "memory");
-}
-__device__ __forceinline__ void dma4(const void *g, uint32_t lds) {
-  uint32_t keep;
-  asm volatile(
-      "s_mov_b32 %0, m0\n\ts_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dword %1, off\n\ts_mov_b32 m0, %0"
-      : "=&s"(keep)
-      : "v"(g), "s"(lds)
+      : "v"(o0), "v"(o1), "v"(on), "s"(m0a), "s"(m0b), "s"(m0n), "s"(xs), "s"(ns)
       : "memory");
 }
 
@@ -293,15 +299,44 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   // Persistent blocks: block b serves XCD b % kXCD (workgroups are dispatched to the XCDs
   // round robin) and takes that XCD's items in order, so the items of a list run at the
   // same time on one XCD and re-read the list from its L2.
+  // A block whose XCD range is exhausted takes items from the other ranges (in XCD order
+  // after its own), so the XCDs finish together.
   const uint32_t xcd = blockIdx.x % kXCD;
   const uint32_t lo = a.bounds[xcd], hi = a.bounds[xcd + 1];
+  // LDS destinations of this wave's ring loads (stage 0).
+  const uint32_t wu = __builtin_amdgcn_readfirstlane((uint32_t)w);
+  const uint32_t m0x = lds_addr(ring) + wu * 2048, m0n = lds_addr(&nring[0][0][0]) + wu * (kWave * 4);
+  // Per-lane parts of the ring loads: block j of this wave holds rows 8w + 4j + l/16, and
+  // lane l fills chunk slot l%16 of its row with the chunk that belongs there (chunks past
+  // the row length load chunk 0).
+  uint32_t lrow[2], lcb[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    lrow[j] = w * 8 + j * 4 + l / 16;
+    const uint32_t lc = (uint32_t)(l % 16) ^ (lrow[j] & 15);
+    lcb[j] = (lc < (uint32_t)KS * 4 ? lc : 0) * 16;
+  }
   for (;;) {
-  if (t == 0) s_item = lo + atomicAdd(&a.next[xcd], 1u);
+  if (t == 0) {
+    uint32_t v = lo + atomicAdd(&a.next[xcd], 1u);
+    if (v >= hi) {
+      v = kDummy;
+      for (uint32_t d = 1; d < (uint32_t)kXCD; ++d) {
+        const uint32_t y = (xcd + d) % kXCD;
+        const uint32_t u = a.bounds[y] + atomicAdd(&a.next[y], 1u);
+        if (u < a.bounds[y + 1]) {
+          v = u;
+          break;
+        }
+      }
+    }
+    s_item = v;
+  }
   __syncthreads();
   const uint32_t item = s_item;
-  if (item >= hi) break;
+  if (item == kDummy) break;
   const ScanItem it = a.items[item];
-  const uint32_t x0 = it.x_begin, np = it.x_count;
+  const uint32_t x0 = __builtin_amdgcn_readfirstlane(it.x_begin), np = __builtin_amdgcn_readfirstlane(it.x_count);
   const uint32_t nsteps = (np + kVP - 1) / kVP;
   // Loads of one step: two 1 KB blocks per wave (rows 8w..8w+7, all 16 chunks; lane l
   // fills chunk slot l%16 of row 8w + 4j + l/16 with the chunk that belongs there) and the
@@ -309,18 +344,12 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   // and chunks past the row length load clamped, valid addresses: every lane issues its
   // load, so each wave issues exactly three loads per step.
   auto issue = [&](uint32_t step) {
-    const int stage = (int)(step % kSStages);
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint32_t row = w * 8 + j * 4 + l / 16;
-      uint32_t lc = (uint32_t)(l % 16) ^ (row & 15);
-      lc = lc < (uint32_t)KS * 4 ? lc : 0;
-      const uint32_t p = min(step * kVP + row, np - 1);
-      dma16(a.xb + size_t(x0 + p) * a.ldb + lc * 8,
-            __builtin_amdgcn_readfirstlane(lds_addr(ring + stage * kSStageB + (w * 2 + j) * 1024)));
-    }
-    const uint32_t pn = min(step * kVP + (uint32_t)(l & 31), np - 1);
-    dma4(a.xn + x0 + pn, __builtin_amdgcn_readfirstlane(lds_addr(&nring[stage][w][0])));
+    const uint32_t stage = step % kSStages, p0 = step * kVP, last = np - 1 - p0;
+    const uint16_t *xs = a.xb + size_t(x0 + p0) * a.ldb;
+    const float *ns = a.xn + x0 + p0;
+    const uint32_t m0a = m0x + stage * kSStageB;
+    dma_step(xs, ns, min(lrow[0], last) * (a.ldb * 2) + lcb[0], min(lrow[1], last) * (a.ldb * 2) + lcb[1],
+             min((uint32_t)(l & 31), last) * 4, m0a, m0a + 1024, m0n + stage * (4 * kWave * 4));
   };
   // The ring's first loads depend only on the item, so they go out before the query setup.
   if (nsteps > 0) issue(0);
