@@ -17,6 +17,15 @@ void queryRBC(const matrix q, const rbcStruct rbcS, unint *NNs, real *NNdists);
 void kqueryRBC(const matrix q, const rbcStruct rbcS, intMatrix NNs, matrix NNdists);
 /* rbc.cu:363-368 */
 void destroyRBC(rbcStruct *rbcS);
+/* rbc.cu:296-302 and 305-312: fill a computation matrix (numReps x numReps, host) with
+ * the identity (each query group scans its own rep's list) or with all ones. */
+void idIntersection(charMatrix cM);
+void fullIntersection(charMatrix cM);
+/* queryRBC / kqueryRBC under the computation matrix cM instead of the identity (the
+ * reference builds the plan from cM with initCompPlan, rbc.cu:375-431, and hard-codes
+ * idIntersection, rbc.cu:105-107; these take the matrix as an argument). */
+void queryRBCPlan(const matrix q, const rbcStruct rbcS, const charMatrix cM, unint *NNs, real *NNdists);
+void kqueryRBCPlan(const matrix q, const rbcStruct rbcS, const charMatrix cM, intMatrix NNs, matrix NNdists);
 #endif
 
 #endif

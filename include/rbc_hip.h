@@ -101,6 +101,22 @@ int rbc_query(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs,
 int rbc_kquery(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs,
                uint32_t *nn_ids, float *nn_dists, uint32_t ldo, int mem, void *stream);
 
+/* The one-shot queries under a general computation plan (compPlan, defs.h:86-92, which
+ * initCompPlan builds from a charMatrix, rbc.cu:375-431; queryRBC/kqueryRBC fix it to
+ * idIntersection, rbc.cu:46-53 and 99-109).  plan is a host R x R byte matrix with row
+ * stride ldp >= R: the queries whose nearest rep is i scan the list of every rep j with
+ * plan[i * ldp + j] != 0 (planNNKernel / planKNNKernel, kernels.cu:15-161).  The result
+ * is the best (k = 1) or the KMAX best of the union of those lists as a multiset: a point
+ * in two planned lists appears twice in a k-NN row, as in the reference.  A group with an
+ * empty plan row gets (MAX_REAL, DUMMY_IDX).  fullIntersection (rbc.cu:305-312) is a
+ * plan of all ones. */
+int rbc_query_plan(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs,
+                   const uint8_t *plan, uint32_t ldp, uint32_t *nn_ids, float *nn_dists,
+                   int mem, void *stream);
+int rbc_kquery_plan(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs,
+                    const uint8_t *plan, uint32_t ldp, uint32_t *nn_ids, float *nn_dists,
+                    uint32_t ldo, int mem, void *stream);
+
 /* destroyRBC (rbc.cu:363-368). */
 int rbc_destroy(rbcStruct *rbcs);
 

@@ -19,6 +19,7 @@ _lib = None
 
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _u32 = ctypes.c_uint32
 _int = ctypes.c_int
@@ -56,6 +57,9 @@ def lib():
                                         _u32, _u32p, _f32p, _u32, _int]
         L.orc_oneshot_kquery.argtypes = [_f32p, _u32, _u32, _int, _u32p, _u32, _u32p, _u32, _u32p,
                                          _f32p, _u32, _u32, _u32, _u32p, _f32p, _u32, _int]
+        L.orc_oneshot_kquery_plan.argtypes = [_f32p, _u32, _u32, _int, _u32p, _u32, _u32p, _u32, _u32p,
+                                              _u8p, _u32, _f32p, _u32, _u32, _u32, _u32p, _f32p, _u32,
+                                              _int]
         L.orc_exact_owners.argtypes = [_f32p, _u32, _u32, _u32, _int, _u32p, _u32, _u32p, _int]
         L.orc_exact_radii.argtypes = [_f32p, _u32, _u32, _u32, _int, _u32p, _u32, _u32p, _f32p]
         _lib = L
@@ -156,6 +160,20 @@ def oneshot_kquery(x, rep_ids, mp, counts, q, k=32, metric=0, threads=0):
     lib().orc_oneshot_kquery(x, x.shape[1], x.shape[1], metric, rep_ids, rep_ids.shape[0], mp,
                              mp.shape[1], _c(counts, np.uint32), q, m, q.shape[1], k, ids, dists,
                              k, threads)
+    return ids, dists
+
+
+def oneshot_kquery_plan(x, rep_ids, mp, counts, plan, q, k=32, metric=0, threads=0):
+    x, q = _c(x, np.float32), _c(q, np.float32)
+    rep_ids = _c(rep_ids, np.uint32)
+    mp = _c(mp, np.uint32)
+    plan = _c(plan, np.uint8)
+    m = q.shape[0]
+    ids = np.empty((m, k), np.uint32)
+    dists = np.empty((m, k), np.float32)
+    lib().orc_oneshot_kquery_plan(x, x.shape[1], x.shape[1], metric, rep_ids, rep_ids.shape[0], mp,
+                                  mp.shape[1], _c(counts, np.uint32), plan, plan.shape[1], q, m,
+                                  q.shape[1], k, ids, dists, k, threads)
     return ids, dists
 
 

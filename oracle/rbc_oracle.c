@@ -313,6 +313,46 @@ void orc_oneshot_kquery(const float *x, uint32_t ldx, uint32_t d, int metric,
   }
 }
 
+void orc_oneshot_kquery_plan(const float *x, uint32_t ldx, uint32_t d, int metric,
+                             const uint32_t *rep_ids, uint32_t R, const uint32_t *map,
+                             uint32_t ldmap, const uint32_t *counts, const uint8_t *plan,
+                             uint32_t ldp, const float *q, uint32_t m, uint32_t ldq, uint32_t k,
+                             uint32_t *ids, float *dists, uint32_t ldo, int threads) {
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long i = 0; i < (long)m; ++i) {
+    const float *qi = q + (size_t)i * ldq;
+    /* computeReps (rbc.cu:233-247): nearest rep, lowest index on ties; it names the
+     * query's group. */
+    float best = FLT_MAX;
+    uint32_t g = 0;
+    for (uint32_t r = 0; r < R; ++r) {
+      float t = orc_dist_f32_fma(x + (size_t)rep_ids[r] * ldx, qi, d, metric);
+      if (t < best) {
+        best = t;
+        g = r;
+      }
+    }
+    float *dv = dists + (size_t)i * ldo;
+    uint32_t *iv = ids + (size_t)i * ldo;
+    for (uint32_t j = 0; j < k; ++j) {
+      dv[j] = FLT_MAX;
+      iv[j] = ORC_DUMMY;
+    }
+    /* planNNKernel / planKNNKernel (kernels.cu:38-72, 120-151): every list the plan row
+     * of the group names (initCompPlan, rbc.cu:405-412, keeps them in ascending order),
+     * every point merged without a check for points seen before. */
+    for (uint32_t xg = 0; xg < R; ++xg) {
+      if (!plan[(size_t)g * ldp + xg]) continue;
+      uint32_t cnt = counts[xg] < ldmap ? counts[xg] : ldmap;
+      for (uint32_t t = 0; t < cnt; ++t) {
+        uint32_t j = map[(size_t)xg * ldmap + t];
+        topk_insert_f(dv, iv, k, orc_dist_f32_fma(x + (size_t)j * ldx, qi, d, metric), j);
+      }
+    }
+  }
+}
+
 void orc_exact_owners(const float *x, uint32_t n, uint32_t ldx, uint32_t d, int metric,
                       const uint32_t *rep_ids, uint32_t R, uint32_t *owner, int threads) {
   set_threads(threads);

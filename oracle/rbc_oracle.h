@@ -77,6 +77,15 @@ void orc_oneshot_kquery(const float *x, uint32_t ldx, uint32_t d, int metric,
                         uint32_t ldmap, const uint32_t *counts, const float *q, uint32_t m,
                         uint32_t ldq, uint32_t k, uint32_t *ids, float *dists, uint32_t ldo,
                         int threads);
+/* orc_oneshot_kquery under a general computation plan (compPlan built from a charMatrix,
+ * rbc.cu:375-431): the query's group g (nearest rep) scans the list of every rep xg with
+ * plan[g * ldp + xg] != 0; the k best of the union of those lists as a multiset (a point
+ * in two planned lists is inserted twice, as planKNNKernel does). */
+void orc_oneshot_kquery_plan(const float *x, uint32_t ldx, uint32_t d, int metric,
+                             const uint32_t *rep_ids, uint32_t R, const uint32_t *map,
+                             uint32_t ldmap, const uint32_t *counts, const uint8_t *plan,
+                             uint32_t ldp, const float *q, uint32_t m, uint32_t ldq, uint32_t k,
+                             uint32_t *ids, float *dists, uint32_t ldo, int threads);
 
 /* Exact-RBC build pieces: owner[i] = nearest rep of x_i (FMA-order FP32, lowest rep
  * index on ties); radius[j] = max over owned points of the FP64 metric distance

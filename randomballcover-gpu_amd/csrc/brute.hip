@@ -96,6 +96,18 @@ int brute_k_device(const float *x, size_t ldx, uint32_t n, const float *q, size_
   return RBC_OK;
 }
 
+int merge_slots(const float *pd, const uint32_t *pi, uint32_t m, uint32_t K, uint32_t nslot, float *od,
+                uint32_t *oi, uint32_t ldo, hipStream_t st) {
+  if (m == 0) return RBC_OK;
+  if (K == 1) {
+    hipLaunchKernelGGL(merge_argmin, dim3(cdiv(m, 256)), dim3(256), 0, st, pd, pi, m, nslot, od, oi);
+  } else {
+    hipLaunchKernelGGL(merge_segments, dim3(cdiv(m, 4)), dim3(256), 0, st, pd, pi, m, K, nslot, od, oi, ldo);
+  }
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
 int nn_device(const float *x, size_t ldx, uint32_t n, const uint32_t *xid, const float *q,
               size_t ldq, uint32_t m, uint32_t d, int metric, float *od, uint32_t *oi,
               hipStream_t st) {

@@ -33,6 +33,30 @@ void kqueryRBC(const matrix q, const rbcStruct rbcS, intMatrix NNs, matrix NNdis
 
 void destroyRBC(rbcStruct *rbcS) { die_on(rbc_destroy(rbcS), "destroyRBC"); }
 
+void idIntersection(charMatrix cM) {
+  for (unint i = 0; i < cM.r && i < cM.c; ++i) cM.mat[size_t(i) * cM.ld + i] = 1;
+}
+
+void fullIntersection(charMatrix cM) {
+  for (unint i = 0; i < cM.r; ++i)
+    for (unint j = 0; j < cM.c; ++j) cM.mat[size_t(i) * cM.ld + j] = 1;
+}
+
+void queryRBCPlan(const matrix q, const rbcStruct rbcS, const charMatrix cM, unint *NNs, real *NNdists) {
+  if (cM.r < rbcS.dr.r || cM.c < rbcS.dr.r) die_on(RBC_ERR_ARG, "queryRBCPlan (computation matrix smaller than numReps)");
+  die_on(rbc_query_plan(q.mat, q.r, q.ld, &rbcS, reinterpret_cast<const uint8_t *>(cM.mat), cM.ld, NNs, NNdists,
+                        RBC_MEM_HOST, nullptr),
+         "queryRBCPlan");
+}
+
+void kqueryRBCPlan(const matrix q, const rbcStruct rbcS, const charMatrix cM, intMatrix NNs, matrix NNdists) {
+  if (NNs.ld != NNdists.ld) die_on(RBC_ERR_ARG, "kqueryRBCPlan (NNs.ld != NNdists.ld)");
+  if (cM.r < rbcS.dr.r || cM.c < rbcS.dr.r) die_on(RBC_ERR_ARG, "kqueryRBCPlan (computation matrix smaller than numReps)");
+  die_on(rbc_kquery_plan(q.mat, q.r, q.ld, &rbcS, reinterpret_cast<const uint8_t *>(cM.mat), cM.ld, NNs.mat,
+                         NNdists.mat, NNs.ld, RBC_MEM_HOST, nullptr),
+         "kqueryRBCPlan");
+}
+
 void bruteRangeCount(matrix x, matrix q, real *ranges, unint *cnts) {
   die_on(rbc_brute_range_count(x.mat, x.r, x.ld, q.mat, q.r, q.ld, x.c, rbc_get_metric(), ranges, cnts,
                                RBC_MEM_HOST, nullptr),

@@ -69,4 +69,10 @@ int nn_device(const float *x, size_t ldx, uint32_t n, const uint32_t *xid, const
               size_t ldq, uint32_t m, uint32_t d, int metric, float *od, uint32_t *oi,
               hipStream_t st);
 
+// Merges nslot per-slot results per query (slot s of query q at pd/pi[(s*m + q)*K], each
+// sorted, K entries) into the K best of their union (as a multiset, lowest id first among
+// equal distances), written to od/oi with row stride ldo (od may be null when K == 1).
+int merge_slots(const float *pd, const uint32_t *pi, uint32_t m, uint32_t K, uint32_t nslot, float *od,
+                uint32_t *oi, uint32_t ldo, hipStream_t st);
+
 }  // namespace rbc

@@ -10,7 +10,8 @@
 // in-block wave scans.  Lists come out in ascending point index, as in the reference.
 // Query (rbc.cu:19-123): nearest rep of each query, queries grouped by rep (the
 // buildQMap role, here a stable counting sort on the device), then every group scans
-// its rep's list through the xMap gather.
+// its rep's list through the xMap gather -- or, with a general computation plan, the
+// lists the plan names for it.
 #include <algorithm>
 #include <vector>
 
@@ -156,8 +157,85 @@ __global__ void row_totals(const uint32_t *off, uint32_t rows, uint32_t nseg, ui
 
 }  // namespace
 
+// General computation plan (compPlan, defs.h:86-92, built by initCompPlan from a charMatrix,
+// rbc.cu:375-431): the queries whose nearest rep is i scan the lists of every rep j with
+// plan[i * ldp + j] != 0, in ascending j (planNNKernel / planKNNKernel, kernels.cu:15-161).
+// The result is the K best of the union of those lists taken as a multiset: a point held
+// by two planned lists is seen twice, as in the reference, which keeps no record of the
+// points already merged.  Work items are (group, planned list, <= kTQ queries); the k-th
+// planned list of a group writes result slot k.  Slots are merged in passes of kPlanSlots
+// - 1 lists, slot 0 carrying the running result, so the scratch stays kPlanSlots x m x K
+// for any plan (fullIntersection, rbc.cu:305-312, included).
+constexpr uint32_t kPlanSlots = 8;
+
+static int oneshot_query_plan(const rbcStruct *s, const float *Qp, size_t ldQ, uint32_t m, uint32_t K,
+                       const uint32_t *qorder, const std::vector<uint32_t> &hqoff, const uint8_t *plan,
+                       uint32_t ldp, const std::vector<uint32_t> &hxo, const std::vector<uint32_t> &hxl,
+                       int metric, float *od, uint32_t *oi, hipStream_t st) {
+  const uint32_t d = s->dx.c, R = s->dr.r;
+  // Planned lists of each group that has queries.
+  std::vector<std::vector<uint32_t>> lists(R);
+  size_t most = 0;
+  for (uint32_t i = 0; i < R; ++i) {
+    if (hqoff[i + 1] == hqoff[i]) continue;
+    for (uint32_t j = 0; j < R; ++j)
+      if (plan[size_t(i) * ldp + j]) lists[i].push_back(j);
+    most = std::max(most, lists[i].size());
+  }
+  const uint32_t per = kPlanSlots - 1;
+  const size_t slot = size_t(m) * K;
+  DevBuf<float> pd;
+  DevBuf<uint32_t> pi;
+  RBC_TRY(pd.alloc(kPlanSlots * slot));
+  RBC_TRY(pi.alloc(kPlanSlots * slot));
+  RBC_TRY(fill_f32(pd.get(), FLT_MAX, slot, st));
+  RBC_TRY(fill_u32(pi.get(), kDummy, slot, st));
+  TileSrc src{s->dx.mat, s->dx.ld, s->dxMap.mat, nullptr, Qp, ldQ, qorder, d, (d + 3) & ~3u};
+  DevBuf<ScanItem> items;
+  for (size_t first = 0; first < most; first += per) {
+    std::vector<ScanItem> iv;
+    for (uint32_t i = 0; i < R; ++i) {
+      const uint32_t cnt = hqoff[i + 1] - hqoff[i];
+      for (size_t k = first; k < std::min(first + per, lists[i].size()); ++k) {
+        const uint32_t j = lists[i][k];
+        for (uint32_t t = 0; t < cnt; t += kTQ) {
+          ScanItem it;
+          it.x_begin = hxo[j];
+          it.x_count = hxl[j];
+          it.q_begin = hqoff[i] + t;
+          it.q_count = std::min<uint32_t>(kTQ, cnt - t);
+          it.aux = 1 + (uint32_t)(k - first);
+          iv.push_back(it);
+        }
+      }
+    }
+    RBC_TRY(fill_f32(pd.get() + slot, FLT_MAX, per * slot, st));
+    RBC_TRY(fill_u32(pi.get() + slot, kDummy, per * slot, st));
+    RBC_TRY(upload_items(iv, items, st));
+    if (K == 1) {
+      EpiArgMin epi;
+      epi.out_d = pd.get();
+      epi.out_i = pi.get();
+      epi.slot_stride = m;
+      RBC_TRY(launch_tiles_m(metric, src, items.get(), (uint32_t)iv.size(), epi, st));
+    } else {
+      EpiTopK epi{K, pd.get(), pi.get(), K, m};
+      RBC_TRY(launch_tiles_m(metric, src, items.get(), (uint32_t)iv.size(), epi, st));
+    }
+    // In place: each query's slots are read by one wave (one thread for K = 1) before it
+    // writes slot 0.
+    RBC_TRY(merge_slots(pd.get(), pi.get(), m, K, kPlanSlots, pd.get(), pi.get(), K, st));
+    // The items buffer is reused by the next pass.
+    RBC_HIP_TRY(hipStreamSynchronize(st));
+  }
+  RBC_HIP_TRY(hipMemcpyAsync(od, pd.get(), slot * 4, hipMemcpyDeviceToDevice, st));
+  RBC_HIP_TRY(hipMemcpyAsync(oi, pi.get(), slot * 4, hipMemcpyDeviceToDevice, st));
+  return RBC_OK;
+}
+
 int oneshot_query_common(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *s, uint32_t K,
-                         uint32_t *ids, float *dists, uint32_t ldo, int mem, hipStream_t st) {
+                         uint32_t *ids, float *dists, uint32_t ldo, int mem, hipStream_t st,
+                         const uint8_t *plan = nullptr, uint32_t ldp = 0) {
   RBC_ARG_CHECK(s && s->dx.mat && s->dr.mat && s->dxMap.mat && s->groupCount, "RBC not built");
   const uint32_t d = s->dx.c, R = s->dr.r;
   RBC_ARG_CHECK(ldq >= d, "leading dimension %u < dimension %u", ldq, d);
@@ -182,6 +260,21 @@ int oneshot_query_common(const float *q, uint32_t m, uint32_t ldq, const rbcStru
     hxo[r] = r * s->dxMap.ld;
     hxl[r] = std::min(s->groupCount[r], s->dxMap.ld);
   }
+  DevBuf<float> od;
+  DevBuf<uint32_t> oi;
+  RBC_TRY(od.alloc(size_t(m) * K));
+  RBC_TRY(oi.alloc(size_t(m) * K));
+  if (plan) {
+    std::vector<uint32_t> hqoff(R + 1);
+    RBC_HIP_TRY(hipMemcpyAsync(hqoff.data(), qoff.get(), (R + 1) * 4, hipMemcpyDeviceToHost, st));
+    RBC_HIP_TRY(hipStreamSynchronize(st));
+    RBC_TRY(oneshot_query_plan(s, Q.p, Q.ld, m, K, qorder.get(), hqoff, plan, ldp, hxo, hxl, metric, od.get(),
+                               oi.get(), st));
+    RBC_TRY(rows_out(ids, size_t(ldo) * 4, oi.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
+    if (dists) RBC_TRY(rows_out(dists, size_t(ldo) * 4, od.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
+    RBC_HIP_TRY(hipStreamSynchronize(st));
+    return RBC_OK;
+  }
   RBC_TRY(xoff.alloc(R));
   RBC_TRY(xlen.alloc(R));
   RBC_HIP_TRY(hipMemcpyAsync(xoff.get(), hxo.data(), R * 4, hipMemcpyHostToDevice, st));
@@ -191,10 +284,6 @@ int oneshot_query_common(const float *q, uint32_t m, uint32_t ldq, const rbcStru
   RBC_TRY(make_bucket_items(qoff.get(), xoff.get(), xlen.get(), R, items, &nitems, ws, 8, st));
   // Scan each group's list through the xMap gather (planKNNKernel / planNNKernel roles).
   TileSrc src{s->dx.mat, s->dx.ld, s->dxMap.mat, nullptr, Q.p, Q.ld, qorder.get(), d, (d + 3) & ~3u};
-  DevBuf<float> od;
-  DevBuf<uint32_t> oi;
-  RBC_TRY(od.alloc(size_t(m) * K));
-  RBC_TRY(oi.alloc(size_t(m) * K));
   if (K == 1) {
     RBC_TRY(fill_f32(od.get(), FLT_MAX, m, st));
     RBC_TRY(fill_u32(oi.get(), kDummy, m, st));
@@ -325,6 +414,27 @@ int rbc_kquery(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs, 
   if (m == 0) return RBC_OK;
   return oneshot_query_common(q, m, ldq, rbcs, KMAX, nn_ids, nn_dists, ldo, mem,
                               static_cast<hipStream_t>(stream));
+}
+
+int rbc_query_plan(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs, const uint8_t *plan,
+                   uint32_t ldp, uint32_t *nn_ids, float *nn_dists, int mem, void *stream) {
+  RBC_ARG_CHECK(rbcs && rbcs->dr.mat, "RBC not built");
+  RBC_ARG_CHECK(plan, "NULL plan");
+  RBC_ARG_CHECK(ldp >= rbcs->dr.r, "plan leading dimension %u < number of reps %u", ldp, rbcs->dr.r);
+  if (m == 0) return RBC_OK;
+  return oneshot_query_common(q, m, ldq, rbcs, 1, nn_ids, nn_dists, 1, mem, static_cast<hipStream_t>(stream), plan,
+                              ldp);
+}
+
+int rbc_kquery_plan(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *rbcs, const uint8_t *plan,
+                    uint32_t ldp, uint32_t *nn_ids, float *nn_dists, uint32_t ldo, int mem, void *stream) {
+  RBC_ARG_CHECK(rbcs && rbcs->dr.mat, "RBC not built");
+  RBC_ARG_CHECK(plan, "NULL plan");
+  RBC_ARG_CHECK(ldp >= rbcs->dr.r, "plan leading dimension %u < number of reps %u", ldp, rbcs->dr.r);
+  RBC_ARG_CHECK(ldo >= KMAX, "output leading dimension %u < KMAX", ldo);
+  if (m == 0) return RBC_OK;
+  return oneshot_query_common(q, m, ldq, rbcs, KMAX, nn_ids, nn_dists, ldo, mem, static_cast<hipStream_t>(stream),
+                              plan, ldp);
 }
 
 int rbc_destroy(rbcStruct *s) {

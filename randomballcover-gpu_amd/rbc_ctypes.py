@@ -80,6 +80,9 @@ _SIGS = {
                          ctypes.POINTER(RBCStruct)]),
     "rbc_query": (_int, [_vp, _u32, _u32, ctypes.POINTER(RBCStruct), _vp, _vp, _int, _vp]),
     "rbc_kquery": (_int, [_vp, _u32, _u32, ctypes.POINTER(RBCStruct), _vp, _vp, _u32, _int, _vp]),
+    "rbc_query_plan": (_int, [_vp, _u32, _u32, ctypes.POINTER(RBCStruct), _vp, _u32, _vp, _vp, _int, _vp]),
+    "rbc_kquery_plan": (_int, [_vp, _u32, _u32, ctypes.POINTER(RBCStruct), _vp, _u32, _vp, _vp, _u32, _int,
+                               _vp]),
     "rbc_destroy": (_int, [ctypes.POINTER(RBCStruct)]),
     "rbc_exact_build": (_int, [_vp, _u32, _u32, _u32, _u32, _int, _vp, _u64, _int,
                                ctypes.POINTER(_vp)]),
@@ -229,6 +232,28 @@ class OneShotRBC:
         ds = np.empty((m, KMAX), np.float32)
         _check(lib().rbc_kquery(qp, m, q.shape[1], ctypes.byref(self.rbcs), ids.ctypes.data,
                                 ds.ctypes.data, KMAX, MEM_HOST, None))
+        return ids, ds
+
+    def query_plan(self, q, plan):
+        """queryRBC under a computation matrix (num_reps x num_reps, nonzero = scan)."""
+        q, qp = _host(q, np.float32)
+        plan = np.ascontiguousarray(plan, dtype=np.uint8)
+        m = q.shape[0]
+        ids = np.empty(m, np.uint32)
+        ds = np.empty(m, np.float32)
+        _check(lib().rbc_query_plan(qp, m, q.shape[1], ctypes.byref(self.rbcs), plan.ctypes.data, plan.shape[1],
+                                    ids.ctypes.data, ds.ctypes.data, MEM_HOST, None))
+        return ids, ds
+
+    def kquery_plan(self, q, plan):
+        """kqueryRBC under a computation matrix (num_reps x num_reps, nonzero = scan)."""
+        q, qp = _host(q, np.float32)
+        plan = np.ascontiguousarray(plan, dtype=np.uint8)
+        m = q.shape[0]
+        ids = np.empty((m, KMAX), np.uint32)
+        ds = np.empty((m, KMAX), np.float32)
+        _check(lib().rbc_kquery_plan(qp, m, q.shape[1], ctypes.byref(self.rbcs), plan.ctypes.data, plan.shape[1],
+                                     ids.ctypes.data, ds.ctypes.data, KMAX, MEM_HOST, None))
         return ids, ds
 
     def close(self):

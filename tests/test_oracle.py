@@ -215,6 +215,45 @@ def test_oneshot_kquery_is_brute_force_over_the_nearest_list():
         np.testing.assert_array_equal(ds[i], sd[0])
 
 
+def test_oneshot_plan_identity_is_the_one_shot_query():
+    x = datagen.uniform(3000, 8, 0)
+    q = datagen.uniform(40, 8, 1)
+    reps = np.random.default_rng(4).choice(3000, 40, replace=False).astype(np.uint32)
+    mp, counts, _, _ = oracle.oneshot_build(x, reps, 200)
+    ids, ds = oracle.oneshot_kquery(x, reps, mp, counts, q, 32)
+    pids, pds = oracle.oneshot_kquery_plan(x, reps, mp, counts, np.eye(40, dtype=np.uint8), q, 32)
+    np.testing.assert_array_equal(pids, ids)
+    np.testing.assert_array_equal(pds, ds)
+
+
+def test_oneshot_plan_is_the_k_best_of_the_planned_lists_as_a_multiset():
+    # Sort-based restatement: every point of every planned list (repeats kept), ordered by
+    # (distance, id); the one-shot lists overlap, so repeats do occur.
+    x = datagen.uniform(2000, 6, 3)
+    q = datagen.uniform(30, 6, 4)
+    R = 24
+    reps = np.random.default_rng(5).choice(2000, R, replace=False).astype(np.uint32)
+    mp, counts, _, _ = oracle.oneshot_build(x, reps, 150)
+    plan = (np.random.default_rng(6).random((R, R)) < 0.2).astype(np.uint8)
+    plan[3] = 0  # a group with an empty plan row
+    ids, ds = oracle.oneshot_kquery_plan(x, reps, mp, counts, plan, q, 32)
+    near, _ = oracle.nn_f32(x[reps], q)
+    repeats = 0
+    for i in range(30):
+        pts = np.concatenate([mp[j, :counts[j]] for j in np.flatnonzero(plan[near[i]])] +
+                             [np.zeros(0, np.uint32)]).astype(np.uint32)
+        repeats += len(pts) - len(np.unique(pts))
+        dv = np.array([oracle.dist_f32_fma(x[j], q[i]) for j in pts], np.float32)
+        order = np.lexsort((pts, dv))[:32]
+        exp_i = np.full(32, 0xFFFFFFFF, np.uint32)
+        exp_d = np.full(32, np.finfo(np.float32).max, np.float32)
+        exp_i[:len(order)] = pts[order]
+        exp_d[:len(order)] = dv[order]
+        np.testing.assert_array_equal(ids[i], exp_i)
+        np.testing.assert_array_equal(ds[i], exp_d)
+    assert repeats > 0
+
+
 def test_exact_owners_and_radii():
     x = datagen.gmm(5000, 20, 0)
     reps = np.random.default_rng(6).choice(5000, 50, replace=False).astype(np.uint32)
