@@ -234,6 +234,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
                                                   const double *gam, const uint32_t *cert,
                                                   const float *radius,
                                                   const float *rr_den, double e, int metric,
+                                                  const uint32_t *nearP, uint32_t P,
                                                   uint32_t W, uint32_t *bits, uint32_t *list_cnt) {
   const uint32_t q0 = blockIdx.x * kPlanQ;
   const int l = lane_id();
@@ -268,6 +269,19 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
               const double Dq0 = (double)Dqr[size_t(q) * R + j0];
               const double num = Dqj * (1.0 - e) - Dq0 * (1.0 + e) - 2.0 * e * psi * psi;
               keep = !(num > 0.0) || num <= Gp * den * (1.0 + 1e-13);
+            }
+            // The same plane against the other reps near the query: an x owned by r_j
+            // satisfies |x-r_j|^2 <= |x-r_i|^2 + 2e psi^2 for EVERY rep r_i (its owner is
+            // the FP32 argmin over all reps), so each r_i gives a lower bound and the
+            // largest one decides.
+            for (uint32_t pi = 0; keep && pi < P; ++pi) {
+              const uint32_t ji = nearP[size_t(q) * P + pi];
+              if (ji >= R || ji == j || ji == j0) continue;
+              const double deni = (double)rr_den[size_t(ji) * R + j];
+              if (!(deni > 0.0)) continue;
+              const double Dqi = (double)Dqr[size_t(q) * R + ji];
+              const double num = Dqj * (1.0 - e) - Dqi * (1.0 + e) - 2.0 * e * psi * psi;
+              keep = !(num > 0.0) || num <= Gp * deni * (1.0 + 1e-13);
             }
           } else {
             const double lb = list_lower_bound(Dqj, 0.0, 0.0, psi, e, true, metric);
@@ -580,7 +594,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
   hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, Dqr, m, R, near, key1, gam, cert,
-                     h->radius.get(), h->rr_den.get(), e, h->metric, W, bits, lcnt);
+                     h->radius.get(), h->rr_den.get(), e, h->metric, nearP, nearP ? Pp : 0u, W, bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));
   RBC_TRY(ws.get(33, 4, &p));
