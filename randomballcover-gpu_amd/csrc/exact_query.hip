@@ -225,79 +225,95 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
 }
 
 // ---- 4. plan ------------------------------------------------------------------------
-// Pass A: keep bits (m x W words) and per-list query counts.  Block = 256 threads over
-// kPlanQ queries; thread t covers lists t, t+256, ...
-constexpr uint32_t kPlanQ = 32;
+// Pass A: keep bits (m x W words) and per-list query counts.  Block (x, y) = 4 waves over
+// lists 64y.. (lane = list) and queries kPlanCQ x.. (wave w takes queries w, w+4, ...).
+// Per (query, list) pair every load is independent of the others; the per-query values
+// are wave-uniform.
+constexpr uint32_t kPlanQ = 32;   // plan_scatter: queries per block
+constexpr uint32_t kPlanCQ = 16;  // plan_count: queries per block
 
 __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, uint32_t R,
-                                                  const uint32_t *near, const uint32_t *key1,
-                                                  const double *gam, const uint32_t *cert,
-                                                  const float *radius,
+                                                  const uint32_t *near,
+                                                  const uint32_t *key1, const double *gam,
+                                                  const uint32_t *cert, const float *radius,
                                                   const float *rr_den, double e, int metric,
                                                   const uint32_t *nearP, uint32_t P,
                                                   uint32_t W, uint32_t *bits, uint32_t *list_cnt) {
-  const uint32_t q0 = blockIdx.x * kPlanQ;
-  const int l = lane_id();
-  for (uint32_t jb = (threadIdx.x / kWave) * kWave; jb < R; jb += 256) {
-    const uint32_t j = jb + l;
-    uint32_t cnt = 0;
-    const double psi = j < R ? (double)radius[j] : 0.0;
-    for (uint32_t qi = 0; qi < kPlanQ && q0 + qi < m; ++qi) {
-      const uint32_t q = q0 + qi;
-      const uint32_t j0 = near[q];
-      const uint32_t j1 = key1 ? key1[q] : R;
-      const uint32_t cb = cert[q];
-      bool keep = false;
-      if (j < R) {
-        if (j == j0) {
-          keep = (cb & 1u) == 0;  // probed; rescan only if the probe was not complete
-        } else if (j == j1) {
-          keep = (cb & 2u) == 0;
-        } else {
-          const double Dqj = (double)Dqr[size_t(q) * R + j];
-          const double G = gam[q];
-          if (metric == 0) {
-            // list_lower_bound's test without square roots or divisions: the triangle
-            // bound sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as
-            // Dqj (1-e) <= (G' + psi)^2, and the plane bound num / den <= G' as
-            // num <= G' den with den rounded up; both keep at least what it keeps.
-            const double Gp = G * (1.0 + 2e-12);
-            const double rhs = Gp + psi;
-            keep = Dqj * (1.0 - e) <= rhs * rhs * (1.0 + 1e-13);
-            const double den = (double)rr_den[size_t(j0) * R + j];
-            if (keep && den > 0.0) {
-              const double Dq0 = (double)Dqr[size_t(q) * R + j0];
-              const double num = Dqj * (1.0 - e) - Dq0 * (1.0 + e) - 2.0 * e * psi * psi;
-              keep = !(num > 0.0) || num <= Gp * den * (1.0 + 1e-13);
-            }
-            // The same plane against the other reps near the query: an x owned by r_j
-            // satisfies |x-r_j|^2 <= |x-r_i|^2 + 2e psi^2 for EVERY rep r_i (its owner is
-            // the FP32 argmin over all reps), so each r_i gives a lower bound and the
-            // largest one decides.
-            for (uint32_t pi = 0; keep && pi < P; ++pi) {
-              const uint32_t ji = nearP[size_t(q) * P + pi];
-              if (ji >= R || ji == j || ji == j0) continue;
-              const double deni = (double)rr_den[size_t(ji) * R + j];
-              if (!(deni > 0.0)) continue;
-              const double Dqi = (double)Dqr[size_t(q) * R + ji];
-              const double num = Dqj * (1.0 - e) - Dqi * (1.0 + e) - 2.0 * e * psi * psi;
-              keep = !(num > 0.0) || num <= Gp * deni * (1.0 + 1e-13);
-            }
-          } else {
-            const double lb = list_lower_bound(Dqj, 0.0, 0.0, psi, e, true, metric);
-            keep = lb <= G;
+  __shared__ uint32_t red[4][kWave];
+  const int l = lane_id(), w = threadIdx.x / kWave;
+  const uint32_t jb = blockIdx.y * kWave, j = jb + l;
+  const double psi = j < R ? (double)radius[j] : 0.0;
+  const double slack = 2.0 * e * psi * psi;
+  uint32_t cnt = 0;
+  for (uint32_t qi = w; qi < kPlanCQ; qi += 4) {
+    const uint32_t q = blockIdx.x * kPlanCQ + qi;
+    if (q >= m) break;
+    const uint32_t j0 = near[q];
+    const uint32_t j1 = key1 ? key1[q] : R;
+    const uint32_t cb = cert[q];
+    bool keep = false;
+    if (j < R) {
+      if (j == j0) {
+        keep = (cb & 1u) == 0;  // probed; rescan only if the probe was not complete
+      } else if (j == j1) {
+        keep = (cb & 2u) == 0;
+      } else {
+        const double Dqj = (double)Dqr[size_t(q) * R + j];
+        const double G = gam[q];
+        if (metric == 0) {
+          // list_lower_bound's test without square roots or divisions: the triangle
+          // bound sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as
+          // Dqj (1-e) <= (G' + psi)^2, and each plane bound num / den <= G' as
+          // num <= G' den with den rounded up; both keep at least what it keeps.
+          const double Gp = G * (1.0 + 2e-12);
+          const double Gd = Gp * (1.0 + 1e-13);
+          const double rhs = Gp + psi;
+          const double Dlo = Dqj * (1.0 - e);
+          keep = Dlo <= rhs * rhs * (1.0 + 1e-13);
+          // The plane between r_j and r_i: an x owned by r_j satisfies
+          // |x-r_j|^2 <= |x-r_i|^2 + 2e psi^2 for EVERY rep r_i (its owner is the FP32
+          // argmin over all reps), so the nearest rep and each probed rep give a lower
+          // bound, and the largest decides.
+          // All loads first (independent of each other), then the tests.
+          uint32_t ji[1 + kProbeLists];
+          float Dqi[1 + kProbeLists], den[1 + kProbeLists];
+          ji[0] = j0;
+#pragma unroll
+          for (uint32_t pi = 0; pi < kProbeLists; ++pi) {
+            const uint32_t r = pi < P ? nearP[size_t(q) * P + pi] : R;
+            ji[1 + pi] = r < R && r != j0 ? r : R;
           }
+#pragma unroll
+          for (uint32_t pi = 0; pi <= kProbeLists; ++pi) {
+            const bool ok = ji[pi] < R && ji[pi] != j;
+            Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
+            den[pi] = ok ? rr_den[size_t(ji[pi]) * R + j] : 0.f;
+          }
+#pragma unroll
+          for (uint32_t pi = 0; pi <= kProbeLists; ++pi) {
+            const double dn = (double)den[pi];
+            const double num = Dlo - (double)Dqi[pi] * (1.0 + e) - slack;
+            if (dn > 0.0 && num > 0.0 && num > Gd * dn) keep = false;
+          }
+        } else {
+          const double lb = list_lower_bound(Dqj, 0.0, 0.0, psi, e, true, metric);
+          keep = lb <= G;
         }
       }
-      cnt += keep;
-      uint64_t b = __ballot(keep);
-      if (l == 0) {
-        uint32_t w = jb / 32;
-        if (w < W) bits[size_t(q) * W + w] = (uint32_t)b;
-        if (w + 1 < W) bits[size_t(q) * W + w + 1] = (uint32_t)(b >> 32);
-      }
     }
-    if (j < R && cnt) atomicAdd(&list_cnt[j], cnt);
+    cnt += keep;
+    const uint64_t b = __ballot(keep);
+    if (l == 0) {
+      const uint32_t wd = jb / 32;
+      if (wd < W) bits[size_t(q) * W + wd] = (uint32_t)b;
+      if (wd + 1 < W) bits[size_t(q) * W + wd + 1] = (uint32_t)(b >> 32);
+    }
+  }
+  red[w][l] = cnt;
+  __syncthreads();
+  if (w == 0) {
+    const uint32_t tot = red[0][l] + red[1][l] + red[2][l] + red[3][l];
+    if (j < R && tot) atomicAdd(&list_cnt[j], tot);
   }
 }
 
@@ -593,8 +609,9 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint32_t *cursor = static_cast<uint32_t *>(p);
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
-  hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, Dqr, m, R, near, key1, gam, cert,
-                     h->radius.get(), h->rr_den.get(), e, h->metric, nearP, nearP ? Pp : 0u, W, bits, lcnt);
+  hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanCQ), cdiv(R, kWave)), dim3(256), 0, st, Dqr, m, R, near,
+                     key1, gam, cert, h->radius.get(), h->rr_den.get(), e, h->metric, nearP, nearP ? Pp : 0u, W,
+                     bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));
   RBC_TRY(ws.get(33, 4, &p));
