@@ -41,7 +41,8 @@ namespace {
 // and an upper bound on the k-th smallest rep distance (k <= 64); optionally the reps of
 // the P <= 64 smallest lane minima, the lists the BF16 probe scans.
 __global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uint32_t R, uint32_t k,
-                                                 uint32_t *near, float *kth, uint32_t P, uint32_t *nearP) {
+                                                 uint32_t *near, float *kth, uint32_t P, uint32_t *nearP,
+                                                 uint32_t P2, uint32_t *planeR) {
   const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
   if (q >= m) return;
   const int l = lane_id();
@@ -66,6 +67,8 @@ __global__ __launch_bounds__(256) void row_stats(const float *D, uint32_t m, uin
   if (l == (int)k - 1) kth[q] = li != kDummy ? lm : FLT_MAX;
   // The P smallest lane minima: P distinct reps near the query (the nearest among them).
   if (nearP && l < (int)P) nearP[size_t(q) * P + l] = li;
+  // The P2 smallest: the reps whose Voronoi planes bound the lists in the plan.
+  if (planeR && l < (int)P2) planeR[size_t(q) * P2 + l] = li;
 }
 
 // ---- 3b. bounds from the probe ----------------------------------------------------
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
                                                   const uint32_t *key1, const double *gam,
                                                   const uint32_t *cert, const float *radius,
                                                   const float *rr_den, double e, int metric,
-                                                  const uint32_t *nearP, uint32_t P,
+                                                  const uint32_t *planeR, uint32_t P,
                                                   uint32_t W, uint32_t *bits, uint32_t *list_cnt) {
   __shared__ uint32_t red[4][kWave];
   const int l = lane_id(), w = threadIdx.x / kWave;
@@ -275,22 +278,22 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
           // argmin over all reps), so the nearest rep and each probed rep give a lower
           // bound, and the largest decides.
           // All loads first (independent of each other), then the tests.
-          uint32_t ji[1 + kProbeLists];
-          float Dqi[1 + kProbeLists], den[1 + kProbeLists];
+          uint32_t ji[1 + kPlaneReps];
+          float Dqi[1 + kPlaneReps], den[1 + kPlaneReps];
           ji[0] = j0;
 #pragma unroll
-          for (uint32_t pi = 0; pi < kProbeLists; ++pi) {
-            const uint32_t r = pi < P ? nearP[size_t(q) * P + pi] : R;
+          for (uint32_t pi = 0; pi < kPlaneReps; ++pi) {
+            const uint32_t r = pi < P ? planeR[size_t(q) * P + pi] : R;
             ji[1 + pi] = r < R && r != j0 ? r : R;
           }
 #pragma unroll
-          for (uint32_t pi = 0; pi <= kProbeLists; ++pi) {
+          for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) {
             const bool ok = ji[pi] < R && ji[pi] != j;
             Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
             den[pi] = ok ? rr_den[size_t(ji[pi]) * R + j] : 0.f;
           }
 #pragma unroll
-          for (uint32_t pi = 0; pi <= kProbeLists; ++pi) {
+          for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) {
             const double dn = (double)den[pi];
             const double num = Dlo - (double)Dqi[pi] * (1.0 + e) - slack;
             if (dn > 0.0 && num > 0.0 && num > Gd * dn) keep = false;
@@ -469,14 +472,17 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint32_t *near = static_cast<uint32_t *>(p);
   RBC_TRY(ws.get(22, size_t(m) * 4, &p));
   float *rep_kth = static_cast<float *>(p);
-  uint32_t *nearP = nullptr;
+  uint32_t *nearP = nullptr, *planeR = nullptr;
+  const uint32_t P2 = bprobe ? std::min<uint32_t>(kPlaneReps, R) : 0;
   if (bprobe) {
     RBC_TRY(ws.get(51, size_t(m) * Pp * 4, &p));
     nearP = static_cast<uint32_t *>(p);
+    RBC_TRY(ws.get(56, size_t(m) * P2 * 4, &p));
+    planeR = static_cast<uint32_t *>(p);
   }
   RBC_TRY(fill_f32(rep_kth, FLT_MAX, m, st));
   hipLaunchKernelGGL(row_stats, dim3(cdiv(m, 4)), dim3(256), 0, st, Dqr, m, R, std::min(k, R), near,
-                     rep_kth, Pp, nearP);
+                     rep_kth, Pp, nearP, P2, planeR);
   RBC_HIP_TRY(hipGetLastError());
   if (R < k) RBC_TRY(fill_f32(rep_kth, FLT_MAX, m, st));
   tm.mark();
@@ -610,7 +616,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
   hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanCQ), cdiv(R, kWave)), dim3(256), 0, st, Dqr, m, R, near,
-                     key1, gam, cert, h->radius.get(), h->rr_den.get(), e, h->metric, nearP, nearP ? Pp : 0u, W,
+                     key1, gam, cert, h->radius.get(), h->rr_den.get(), e, h->metric, planeR, P2, W,
                      bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));

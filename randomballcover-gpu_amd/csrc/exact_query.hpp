@@ -39,6 +39,8 @@ int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t
                hipStream_t st);
 // Lists probed per query by the BF16 probe.
 constexpr uint32_t kProbeLists = 8;
+// Reps per query whose Voronoi planes bound the lists in the exact plan (<= 64).
+constexpr uint32_t kPlaneReps = 12;
 // Queries per work item of the BF16 scan (its items must be cut at this size).
 constexpr uint32_t kBf16ScanQ = 256;
 // Queries per work item of the BF16 probe.
