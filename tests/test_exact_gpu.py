@@ -98,6 +98,28 @@ def test_exact_bf16_matches_fp64_oracle(gen, n, m, d, R, k):
     np.testing.assert_array_equal(ds, fd)
 
 
+@pytest.mark.parametrize("n,R,m,k", [(40, 1, 5, 3), (300, 5, 40, 10), (100, 7, 1, 64), (65, 11, 130, 1),
+                                     (2000, 13, 257, 20)])
+def test_exact_bf16_edge_shapes(n, R, m, k):
+    # Fewer reps than the probe's lists and the plan's plane reps, k above the probe's K',
+    # item counts that are not multiples of the tiles.
+    x = datagen.lowrank(n, 128, 0)
+    q = datagen.lowrank(m, 128, 1)
+    idx = rbc.ExactRBC(x, R, seed=2)
+    ids, ds = idx.kquery(q, k, precision=rbc.PREC_BF16)
+    check_against_fp64(x, q, ids, ds, k)
+
+
+def test_exact_bf16_coincident_reps():
+    # Rows repeated many times: some reps coincide (zero rep-rep distance, no plane).
+    base = datagen.lowrank(300, 128, 3)
+    x = np.concatenate([base] * 8)
+    q = np.concatenate([base[:30], datagen.lowrank(30, 128, 4)])
+    idx = rbc.ExactRBC(x, 200, seed=5)
+    ids, ds = idx.kquery(q, 12, precision=rbc.PREC_BF16)
+    check_against_fp64(x, q, ids, ds, 12)
+
+
 def test_exact_bf16_duplicates():
     base = datagen.lowrank(4000, 128, 0)
     x = np.concatenate([base, base[::-1]])
