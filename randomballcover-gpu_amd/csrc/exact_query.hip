@@ -290,14 +290,24 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
           for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) {
             const bool ok = ji[pi] < R && ji[pi] != j;
             Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
-            den[pi] = ok ? rr_den[size_t(ji[pi]) * R + j] : 0.f;
+            const float dn = ok ? rr_den[size_t(ji[pi]) * R + j] : 0.f;
+            den[pi] = dn > 0.f ? dn : INFINITY;  // no plane: never prunes
           }
+          // The plane test num > G' den in FP32, with every rounding pushed towards
+          // keeping: A <= Dlo - slack (rounded down), and fl(Gf den + B_i) >= G' den +
+          // Dqi (1+e), since Gf >= G' (1 + 8u) and B_i = fl(Dqi fl(1 + e + 8u)) >=
+          // Dqi (1+e)(1 + 6u) (u = 2^-24).  A > that implies the exact test.
+          const double Ad = Dlo - slack;
+          float A = (float)Ad;
+          if ((double)A > Ad) A = nextafterf(A, -INFINITY);
+          const double Gu = Gd * (1.0 + 0x1p-21);
+          float Gf = (float)Gu;
+          if ((double)Gf < Gu) Gf = nextafterf(Gf, INFINITY);
+          const float cb1 = (float)(1.0 + e + 0x1p-21);
+          bool far = false;
 #pragma unroll
-          for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) {
-            const double dn = (double)den[pi];
-            const double num = Dlo - (double)Dqi[pi] * (1.0 + e) - slack;
-            if (dn > 0.0 && num > 0.0 && num > Gd * dn) keep = false;
-          }
+          for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) far |= A > __builtin_fmaf(Gf, den[pi], Dqi[pi] * cb1);
+          if (far) keep = false;
         } else {
           const double lb = list_lower_bound(Dqj, 0.0, 0.0, psi, e, true, metric);
           keep = lb <= G;
