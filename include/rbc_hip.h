@@ -207,6 +207,10 @@ int rbc_mlp_loss_grad(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx,
                       const uint32_t *labels, float *loss, float *grads);
 /* Logits (host, rows x n_out) of host inputs. */
 int rbc_mlp_logits(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, float *logits);
+/* The proposal of each host input row: the index of its largest logit, lowest index
+ * among equal logits (UINT_MAX if every logit is NaN) -- the list the exact query adds
+ * to its probe. */
+int rbc_mlp_propose(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, uint32_t *out);
 /* Adam training (beta 0.9/0.999) for `epochs` passes over shuffled batches; epoch_loss
  * (may be NULL) receives the loss of each epoch's last batch. */
 int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels,

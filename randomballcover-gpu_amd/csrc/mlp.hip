@@ -41,7 +41,18 @@ struct GemmArgs {
   int M, N, K;
   int gelu;            // apply GELU for H
   float alpha;         // result = alpha * A B + bias (0 means 1)
+  unsigned long long *amax;  // if set: per-row argmax keys (argmax_key) instead of C and H
 };
+
+// Row argmax as one 64-bit maximum: ordered value bits above the complemented column, so
+// the largest value wins and, among equal values, the lowest column; NaN (and no value at
+// all) is key 0.
+__device__ __forceinline__ unsigned long long argmax_key(float v, uint32_t j) {
+  if (v != v) return 0ull;
+  uint32_t u = __float_as_uint(v == 0.f ? 0.f : v);
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return ((unsigned long long)u << 32) | (0xFFFFFFFFu - j);
+}
 
 constexpr int kGT = 64, kGK = 32, kLdsPad = 40;
 
@@ -99,6 +110,29 @@ __global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
     }
   }
   const int rb = (w / 2) * 32, cb = (w % 2) * 32;
+  if (g.amax) {
+    // The 16 lanes l/16 = c hold rows rb + bi*16 + 4c + r; each reduces its two columns,
+    // then the 16 reduce among themselves and one of them folds the row into amax.
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + rb + bi * 16 + (l / 16) * 4 + r;
+        unsigned long long best = 0ull;
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          const int j = j0 + cb + bj * 16 + l % 16;
+          if (i < g.M && j < g.N) {
+            const float v = (g.alpha != 0.f ? g.alpha : 1.f) * acc[bi][bj][r] + (g.bias ? g.bias[j] : 0.f);
+            best = max(best, argmax_key(v, (uint32_t)j));
+          }
+        }
+#pragma unroll
+        for (int sh = 1; sh < 16; sh <<= 1) best = max(best, __shfl_xor(best, sh, 64));
+        if (l % 16 == 0 && i < g.M && best) atomicMax(&g.amax[i], best);
+      }
+    return;
+  }
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
@@ -197,40 +231,22 @@ __global__ void to_half(const float *x, int64_t n, _Float16 *y) {
 }
 
 // Gathers rows idx[0..B) of X (FP32, ldx) into an FP16 [B][d] batch.
+// FP16 copy of rows idx[0..B) of X (rows 0..B when idx is null).
 __global__ void gather_half(const float *X, int64_t ldx, const uint32_t *idx, int B, int d, _Float16 *out) {
   int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= int64_t(B) * d) return;
   int r = int(e / d), c = int(e % d);
-  out[e] = (_Float16)X[int64_t(idx[r]) * ldx + c];
+  out[e] = (_Float16)X[int64_t(idx ? idx[r] : uint32_t(r)) * ldx + c];
+}
+
+__global__ void argmax_decode(const unsigned long long *key, uint32_t B, uint32_t *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < B) out[i] = key[i] ? 0xFFFFFFFFu - (uint32_t)key[i] : 0xFFFFFFFFu;
 }
 
 __global__ void gather_u32(const uint32_t *src, const uint32_t *idx, int B, uint32_t *out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < B) out[i] = src[idx[i]];
-}
-
-// Row argmax (lowest index on ties), one wave per row.
-__global__ __launch_bounds__(256) void row_argmax(const float *Z, int B, int N, uint32_t *out) {
-  const int row = blockIdx.x * 4 + threadIdx.x / 64, l = threadIdx.x % 64;
-  if (row >= B) return;
-  float bv = -INFINITY;
-  uint32_t bi = 0xFFFFFFFFu;
-  for (int j = l; j < N; j += 64) {
-    float v = Z[int64_t(row) * N + j];
-    if (v > bv) {
-      bv = v;
-      bi = j;
-    }
-  }
-  for (int s = 32; s > 0; s >>= 1) {
-    float ov = __shfl_xor(bv, s, 64);
-    uint32_t oi = __shfl_xor(bi, s, 64);
-    if (ov > bv || (ov == bv && oi < bi)) {
-      bv = ov;
-      bi = oi;
-    }
-  }
-  if (l == 0) out[row] = bi;
 }
 
 }  // namespace
@@ -251,6 +267,7 @@ struct rbc_mlp {
   rbc::DevBuf<_Float16> x16, h1, h2, dz3, dz2, dz1;
   rbc::DevBuf<float> z1, z2, z3, dh2, dh1, loss;
   rbc::DevBuf<uint32_t> lab, idx;
+  rbc::DevBuf<unsigned long long> amax;  // inference: per-row argmax keys
 
   int ensure(uint32_t B) {
     if (B <= cap) return RBC_OK;
@@ -268,6 +285,7 @@ struct rbc_mlp {
     RBC_TRY(dh1.alloc(size_t(B) * H1));
     RBC_TRY(lab.alloc(B));
     RBC_TRY(idx.alloc(B));
+    RBC_TRY(amax.alloc(B));
     RBC_TRY(loss.alloc(1));
     return RBC_OK;
   }
@@ -283,7 +301,7 @@ namespace rbc {
 namespace {
 
 // Forward pass on the FP16 batch in m->x16 (B rows); logits to m->z3 (or `logits`).
-int mlp_forward(rbc_mlp *m, uint32_t B, float *logits, hipStream_t st) {
+int mlp_forward(rbc_mlp *m, uint32_t B, float *logits, hipStream_t st, unsigned long long *amax = nullptr) {
   GemmArgs g{};
   g.A = m->x16.get(); g.a_rs = m->D; g.a_cs = 1;
   g.B = m->W16(0); g.b_rs = 1; g.b_cs = m->D;
@@ -299,8 +317,8 @@ int mlp_forward(rbc_mlp *m, uint32_t B, float *logits, hipStream_t st) {
   g = GemmArgs{};
   g.A = m->h2.get(); g.a_rs = m->H2; g.a_cs = 1;
   g.B = m->W16(2); g.b_rs = 1; g.b_cs = m->H2;
-  g.C = logits ? logits : m->z3.get(); g.ldc = m->C; g.bias = m->Bv(2);
-  g.M = B; g.N = m->C; g.K = m->H2; g.gelu = 0;
+  g.C = amax ? nullptr : logits ? logits : m->z3.get(); g.ldc = m->C; g.bias = m->Bv(2);
+  g.M = B; g.N = m->C; g.K = m->H2; g.gelu = 0; g.amax = amax;
   RBC_TRY((gemm<_Float16, _Float16>(g, st)));
   return RBC_OK;
 }
@@ -349,25 +367,21 @@ int mlp_backward(rbc_mlp *m, uint32_t B, hipStream_t st) {
 }  // namespace
 
 // Logits -> top-1 list per row.
+// Stream-ordered: out is complete when the work queued on st is.
 int mlp_propose(rbc_mlp *m, const float *q, size_t ldq, uint32_t rows, uint32_t *out, hipStream_t st) {
-  const uint32_t chunk = 16384;
+  const uint32_t chunk = 65536;
   RBC_TRY(m->ensure(std::min(rows, chunk)));
-  DevBuf<uint32_t> iota;
-  RBC_TRY(iota.alloc(chunk));
-  std::vector<uint32_t> hi(chunk);
-  for (uint32_t i = 0; i < chunk; ++i) hi[i] = i;
-  RBC_HIP_TRY(hipMemcpyAsync(iota.get(), hi.data(), chunk * 4, hipMemcpyHostToDevice, st));
   for (uint32_t r0 = 0; r0 < rows; r0 += chunk) {
     uint32_t B = std::min(chunk, rows - r0);
     int64_t ne = int64_t(B) * m->D;
     hipLaunchKernelGGL(gather_half, dim3((uint32_t)cdivz(ne, 256)), dim3(256), 0, st, q + size_t(r0) * ldq,
-                       (int64_t)ldq, iota.get(), (int)B, (int)m->D, m->x16.get());
+                       (int64_t)ldq, (const uint32_t *)nullptr, (int)B, (int)m->D, m->x16.get());
     RBC_HIP_TRY(hipGetLastError());
-    RBC_TRY(mlp_forward(m, B, nullptr, st));
-    hipLaunchKernelGGL(row_argmax, dim3(cdiv(B, 4)), dim3(256), 0, st, m->z3.get(), (int)B, (int)m->C, out + r0);
+    RBC_TRY(fill_u32(reinterpret_cast<uint32_t *>(m->amax.get()), 0u, 2 * size_t(B), st));
+    RBC_TRY(mlp_forward(m, B, nullptr, st, m->amax.get()));
+    hipLaunchKernelGGL(argmax_decode, dim3(cdiv(B, 256)), dim3(256), 0, st, m->amax.get(), B, out + r0);
     RBC_HIP_TRY(hipGetLastError());
   }
-  RBC_HIP_TRY(hipStreamSynchronize(st));
   return RBC_OK;
 }
 
@@ -571,6 +585,19 @@ int rbc_mlp_logits(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, floa
   RBC_TRY(mlp_forward(m, rows, zd.get(), nullptr));
   RBC_HIP_TRY(hipDeviceSynchronize());
   RBC_HIP_TRY(hipMemcpy(logits, zd.get(), size_t(rows) * m->C * 4, hipMemcpyDeviceToHost));
+  return RBC_OK;
+}
+
+int rbc_mlp_propose(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, uint32_t *out) {
+  RBC_ARG_CHECK(m && x && out && rows, "bad argument");
+  RBC_ARG_CHECK(ldx >= m->D, "ldx < input width");
+  DevBuf<float> xd;
+  DevBuf<uint32_t> od;
+  RBC_TRY(xd.alloc(size_t(rows) * ldx));
+  RBC_TRY(od.alloc(rows));
+  RBC_HIP_TRY(hipMemcpy(xd.get(), x, size_t(rows) * ldx * 4, hipMemcpyHostToDevice));
+  RBC_TRY(mlp_propose(m, xd.get(), ldx, rows, od.get(), nullptr));
+  RBC_HIP_TRY(hipMemcpy(out, od.get(), size_t(rows) * 4, hipMemcpyDeviceToHost));
   return RBC_OK;
 }
 

@@ -103,6 +103,7 @@ _SIGS = {
     "rbc_mlp_load": (_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "rbc_mlp_loss_grad": (_int, [_vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float), _vp]),
     "rbc_mlp_logits": (_int, [_vp, _vp, _u32, _u32, _vp]),
+    "rbc_mlp_propose": (_int, [_vp, _vp, _u32, _u32, _vp]),
     "rbc_mlp_train": (_int, [_vp, _vp, _u32, _u32, _vp, _int, _u32, _u32, ctypes.c_float, _u64, _vp]),
     "rbc_batched_build": (_int, [_u32, _vp, _vp, _vp, _vp, _vp, _int, _u64, _int, ctypes.POINTER(_vp)]),
     "rbc_batched_kquery": (_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _int, _vp, _vp]),
@@ -403,6 +404,13 @@ class ProposalMLP:
         x, xp = _host(x, np.float32)
         out = np.empty((x.shape[0], self.dims[3]), np.float32)
         _check(lib().rbc_mlp_logits(self.h, xp, x.shape[0], x.shape[1], out.ctypes.data))
+        return out
+
+    def propose(self, x):
+        """Index of each row's largest logit (lowest index among equals)."""
+        x, xp = _host(x, np.float32)
+        out = np.empty(x.shape[0], np.uint32)
+        _check(lib().rbc_mlp_propose(self.h, xp, x.shape[0], x.shape[1], out.ctypes.data))
         return out
 
     def train(self, x, labels, epochs=5, batch=1024, lr=1e-3, seed=0):

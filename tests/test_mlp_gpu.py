@@ -36,6 +36,20 @@ def test_logits_match_fp64_oracle():
     assert np.abs(z - oz).max() <= 1e-2 * np.abs(oz).max()
 
 
+@pytest.mark.parametrize("rows,n_out", [(70000, 2048), (300, 300), (1, 65)])
+def test_proposals_are_the_logit_argmax(rows, n_out):
+    # The fused GEMM + argmax of the query path against the argmax of the logits (lowest
+    # index among equal logits), over more rows than one inference chunk holds.
+    net = rbc.ProposalMLP(128, n_out, seed=4)
+    x = datagen.lowrank(rows, 128, 6)
+    p = net.propose(x)
+    z = net.logits(x)
+    np.testing.assert_array_equal(p, np.argmax(z, axis=1).astype(np.uint32))
+    # Equal logits: a network with all-zero weights proposes list 0 for every row.
+    net.set_params(np.zeros_like(net.get_params()))
+    assert (net.propose(x[:100]) == 0).all()
+
+
 def test_training_reduces_loss_and_proposal_gain():
     n, d, R, k = 1 << 17, 96, 256, 10
     x = datagen.gmm_dirichlet(n, d, 0)
