@@ -421,6 +421,9 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
     hx[0] = s * kVP + (l % 16) < np ? 0.5f * n0 : INFINITY;
     hx[1] = s * kVP + 16 + (l % 16) < np ? 0.5f * n1 : INFINITY;
     f32x4 acc[4][2];
+    // While a wave issues its MFMAs it goes first: the SIMD's other waves fill the gaps
+    // between its MFMAs instead of delaying them.
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int pb = 0; pb < 2; ++pb) {
       bf16x8 B[KS];
@@ -434,6 +437,7 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
         for (int qb = 0; qb < 4; ++qb)
           acc[qb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][ks], B[ks], acc[qb][pb], 0, 0, 0);
     }
+    __builtin_amdgcn_s_setprio(0);
     // Pre-filter on the float bit patterns as signed integers: for a threshold h > 0 that
     // order agrees with the float order (negative values and negative NaNs compare below,
     // positive NaNs above -- the exact test below rejects them); h == 0 (a zero-norm
