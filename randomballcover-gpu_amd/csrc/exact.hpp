@@ -39,6 +39,15 @@ struct rbc_exact {
   uint32_t cand_cap[2] = {0, 0};
   // Learned list proposal (not owned); NULL = probe only the nearest rep's list.
   struct rbc_mlp *proposal = nullptr;
+  // Second stream (default priority) on which the proposal's inference runs beside the
+  // q -> reps distances, and the events that order it; created on first use.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  ~rbc_exact() {
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (side) (void)hipStreamDestroy(side);
+  }
 };
 
 namespace rbc {

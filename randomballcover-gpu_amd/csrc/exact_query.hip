@@ -458,6 +458,23 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   const double e = fma_rel_err(h->dpad);
   const uint32_t Kp = std::min<uint32_t>(RBC_MAX_K, k + 8);
   void *p;
+  // 2'. The proposal network's inference needs only the queries: it runs on the side
+  // stream while the main stream computes the q -> reps distances, and proposal_slot
+  // waits for it.
+  uint32_t *prop = nullptr;
+  if (h->proposal && precision == RBC_PREC_BF16 && k <= 32) {
+    RBC_TRY(ws.get(43, size_t(m) * 4, &p));
+    prop = static_cast<uint32_t *>(p);
+    if (!h->side) {
+      RBC_HIP_TRY(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+      RBC_HIP_TRY(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
+      RBC_HIP_TRY(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
+    }
+    RBC_HIP_TRY(hipEventRecord(h->ev_in, st));
+    RBC_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_in, 0));
+    RBC_TRY(mlp_propose(h->proposal, qd, ldq, m, prop, h->side));
+    RBC_HIP_TRY(hipEventRecord(h->ev_out, h->side));
+  }
   // 1. q -> reps
   RBC_TRY(ws.get(20, size_t(m) * R * 4, &p));
   float *Dqr = static_cast<float *>(p);
@@ -543,10 +560,8 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint32_t *probe_i = nullptr, *probe_i1 = nullptr, *key1 = nullptr;
   if (bprobe) {
     // Entries e = q * Pp + s (query q, probe slot s), keyed by list, sorted list-major.
-    if (h->proposal) {
-      RBC_TRY(ws.get(43, size_t(m) * 4, &p));
-      uint32_t *prop = static_cast<uint32_t *>(p);
-      RBC_TRY(mlp_propose(h->proposal, qd, ldq, m, prop, st));
+    if (prop) {
+      RBC_HIP_TRY(hipStreamWaitEvent(st, h->ev_out, 0));
       hipLaunchKernelGGL(proposal_slot, dim3(cdiv(m, 256)), dim3(256), 0, st, prop, m, Pp, R, nearP);
       RBC_HIP_TRY(hipGetLastError());
     }
