@@ -92,9 +92,9 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
 // three steps ahead, so no VGPRs hold data in flight.  Each wave waits for its own loads of
 // the step (s_waitcnt vmcnt with the count of younger loads) before the step's barrier.
 // A stage holds 32 rows of 256 bytes; the 16-byte chunk c of row r sits at chunk c ^ (r%16),
-// which keeps the 64-lane reads (16 rows x 4 chunks) free of bank conflicts.  The compiler
-// would order every LDS read after any LDS DMA in flight, so the ring is read by one asm
-// block per step that waits for its own reads.
+// which keeps the 64-lane reads (16 rows x 4 chunks) free of bank conflicts.  The LDS DMA
+// is issued from asm: a compiler-visible one would make the compiler drain it before
+// every LDS read.
 // Candidates go to an LDS list first (an LDS atomic does not wait for the loads in flight)
 // and are appended to the per-query buffers after the last step; entries beyond the LDS
 // list's capacity are appended directly.
@@ -109,7 +109,9 @@ constexpr int kRowB = 288;
 constexpr int kStages = 3;
 
 typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(3))) const float lds_float;
 typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i32x4;
+typedef __attribute__((address_space(3))) const i32x4 lds_i32x4;
 
 __device__ __forceinline__ int max3i(int a, int b, int c) { return max(max(a, b), c); }
 
@@ -151,29 +153,6 @@ __device__ __forceinline__ void dma_step(const void *xs, const void *ns, uint32_
 template <int N>
 __device__ __forceinline__ void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
-}
-
-// One step's ring reads: eight 16-byte B fragments and the two half-norms of the lane's
-// points, then a wait for them (the asm's outputs are complete when it ends).
-__device__ __forceinline__ void ring_read(const uint32_t (&ad)[8], uint32_t ah, i32x4 (&b)[8], float &h0,
-                                          float &h1) {
-  asm volatile(
-      "ds_read_b128 %0, %10\n\t"
-      "ds_read_b128 %1, %11\n\t"
-      "ds_read_b128 %2, %12\n\t"
-      "ds_read_b128 %3, %13\n\t"
-      "ds_read_b128 %4, %14\n\t"
-      "ds_read_b128 %5, %15\n\t"
-      "ds_read_b128 %6, %16\n\t"
-      "ds_read_b128 %7, %17\n\t"
-      "ds_read_b32 %8, %18\n\t"
-      "ds_read_b32 %9, %18 offset:64\n\t"
-      "s_waitcnt lgkmcnt(0)"
-      : "=&v"(b[0]), "=&v"(b[1]), "=&v"(b[2]), "=&v"(b[3]), "=&v"(b[4]), "=&v"(b[5]), "=&v"(b[6]),
-        "=&v"(b[7]), "=&v"(h0), "=&v"(h1)
-      : "v"(ad[0]), "v"(ad[1]), "v"(ad[2]), "v"(ad[3]), "v"(ad[4]), "v"(ad[5]), "v"(ad[6]), "v"(ad[7]),
-        "v"(ah)
-      : "memory");
 }
 
 // Splits the list-major scan items into kXCD contiguous ranges of about equal work and
@@ -416,7 +395,13 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
     for (int i = 0; i < 8; ++i) ad[i] = ad0[i] + so;
     i32x4 braw[8];
     float n0, n1;
-    ring_read(ad, ah0 + ho, braw, n0, n1);
+    // Plain LDS reads: the compiler does not see the ring's LDS DMA (it is issued from
+    // asm), so it neither drains those loads before the reads nor moves the reads above
+    // the barrier (whose asm clobbers memory); it waits for the reads with lgkmcnt.
+#pragma unroll
+    for (int i = 0; i < 8; ++i) braw[i] = *reinterpret_cast<const lds_i32x4 *>((uintptr_t)ad[i]);
+    n0 = *reinterpret_cast<const lds_float *>((uintptr_t)(ah0 + ho));
+    n1 = *reinterpret_cast<const lds_float *>((uintptr_t)(ah0 + ho + 64));
     float hx[2];
     hx[0] = s * kVP + (l % 16) < np ? 0.5f * n0 : INFINITY;
     hx[1] = s * kVP + 16 + (l % 16) < np ? 0.5f * n1 : INFINITY;
