@@ -11,10 +11,12 @@
 //
 // Scan work item = (one ownership list, up to 256 queries that need it).  The four waves
 // of a workgroup keep 64 queries each as MFMA A operands in VGPRs and share the list's
-// points, 32 per step, through a three-stage LDS ring filled two steps ahead.  Items are
-// list-major and remapped so that each XCD takes a contiguous range: the items of a list
-// run on one XCD and re-read the list from that XCD's L2.  The probe (top-K' of one
-// list per query) uses 64-query items on the same LDS ring with per-lane register top-K'.
+// points, 32 per step, through a four-stage LDS ring filled three steps ahead by direct
+// global->LDS loads.  The grid is persistent; the list-major items are split into one
+// contiguous range per XCD (of equal work), so the items of a list run on one XCD at the
+// same time and re-read the list from that XCD's L2.  The probe (top-K' of one list per
+// (query, list) entry) uses 64-entry items on a three-stage register-staged LDS ring with
+// per-lane register top-K'.
 #include "exact_query.hpp"
 
 namespace rbc {

@@ -1,9 +1,10 @@
 // Exact k-NN query over an rbc_exact index.
 //
-// Pipeline (one stream):
+// Pipeline (one stream; the proposal network's inference runs beside step 1 on a second):
 //  1. q->reps   : FP32 distances of every query to every representative.
 //  2. row stats : nearest rep of each query and an upper bound on its k-th smallest rep
-//                 distance (reps are database points, so that bounds the k-th NN distance).
+//                 distance (reps are database points, so that bounds the k-th NN distance);
+//                 BF16: also the reps of its smallest lane minima (probe lists, plane reps).
 //  3. probe     : an upper bound gamma on the k-th NN distance (and a cut-off tau for
 //                 FP32 candidates) from K' = k+8 database points near the query.  FP32:
 //                 queries grouped by nearest rep scan that rep's list (and the proposal's
@@ -13,10 +14,11 @@
 //                 tiles; the query's best K' approximate pairs are re-ranked in FP32.
 //  4. plan      : per (query, list) lower bounds on the distance to any point of the
 //                 list -- the triangle inequality with the list radius
-//                 (d(q,r_j) - psi_j) and, for L2, the Voronoi plane between the list's
-//                 rep and the query's nearest rep -- computed in FP64 with explicit FP32
-//                 error bounds; lists whose bound exceeds gamma are pruned.  The
-//                 surviving (list, query) pairs are compacted list-major.
+//                 (d(q,r_j) - psi_j) and, for L2, the Voronoi planes between the list's
+//                 rep and the query's nearest rep (BF16: and its plane reps) -- with
+//                 explicit FP32 error bounds and directed rounding; lists whose bound
+//                 exceeds gamma are pruned.  The surviving (list, query) pairs are
+//                 compacted list-major.
 //  5. scan      : every (list, query tile) item computes all distances and appends the
 //                 ones that can be within gamma to per-query candidate buffers.
 //  6. select    : one wave per query re-ranks the candidates with tree-ordered FP32 sums,
