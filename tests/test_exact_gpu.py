@@ -10,7 +10,7 @@ from conftest import datagen, rbc
 
 pytestmark = pytest.mark.gpu
 
-REL_TOL = 1e-6  # distances vs FP64
+REL_TOL = 1e-6  # distances vs FP64, relative
 
 
 def check_against_fp64(x, q, ids, ds, k, metric=0):
