@@ -38,6 +38,62 @@ namespace rbc {
 
 namespace {
 
+// ---- 1. q -> reps ---------------------------------------------------------------------
+// All query-rep distances (metric value, FP32): per pair, acc = 0 and one fma_term per
+// coordinate in coordinate order -- the values of the tile engine's EpiDense, so the
+// error bound fma_rel_err(dpad) holds.  128 x 128 tiles, 8 x 8 per thread (rows
+// ty*4.. and 64+ty*4.. of the query tile, columns tx*4.. and 64+tx*4.. of the rep tile),
+// 32 coordinates at a time staged transposed in LDS; staging threads walk the rows, so
+// the transposed stores are free of bank conflicts.
+constexpr int kQRT = 128, kQRC = 32;
+
+template <int METRIC>
+__global__ __launch_bounds__(256) void qr_dist(const float *Q, size_t ldq, uint32_t m, const float *Rp, size_t ldr,
+                                               uint32_t R, uint32_t d, float *D) {
+  __shared__ __attribute__((aligned(16))) float Qs[kQRC][kQRT];
+  __shared__ __attribute__((aligned(16))) float Rs[kQRC][kQRT];
+  const int t = threadIdx.x, ty = t / 16, tx = t % 16;
+  const uint32_t q0 = blockIdx.x * kQRT, r0 = blockIdx.y * kQRT;
+  float acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = 0.f;
+  for (uint32_t c0 = 0; c0 < d; c0 += kQRC) {
+    __syncthreads();
+#pragma unroll 4
+    for (int i = t; i < kQRT * kQRC; i += 256) {
+      const uint32_t row = i % kQRT, c = c0 + i / kQRT;
+      Qs[i / kQRT][row] = q0 + row < m && c < d ? Q[size_t(q0 + row) * ldq + c] : 0.f;
+      Rs[i / kQRT][row] = r0 + row < R && c < d ? Rp[size_t(r0 + row) * ldr + c] : 0.f;
+    }
+    __syncthreads();
+    const uint32_t cn = min((uint32_t)kQRC, d - c0);
+    for (uint32_t c = 0; c < cn; ++c) {
+      const float4 qa = *reinterpret_cast<const float4 *>(&Qs[c][ty * 4]);
+      const float4 qb = *reinterpret_cast<const float4 *>(&Qs[c][64 + ty * 4]);
+      const float4 ra = *reinterpret_cast<const float4 *>(&Rs[c][tx * 4]);
+      const float4 rb = *reinterpret_cast<const float4 *>(&Rs[c][64 + tx * 4]);
+      const float qv[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+      const float rv[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = fma_term<METRIC>(rv[j], qv[i], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t q = q0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + i - 4);
+    if (q >= m) continue;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const uint32_t r = r0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + j - 4);
+      if (r < R) D[size_t(q) * R + r] = acc[i][j];
+    }
+  }
+}
+
 // ---- 2. row stats ----------------------------------------------------------------
 // One wave per query over its row of R distances: nearest rep (lowest index on ties)
 // and an upper bound on the k-th smallest rep distance (k <= 64); optionally the reps of
@@ -481,14 +537,12 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_TRY(ws.get(20, size_t(m) * R * 4, &p));
   float *Dqr = static_cast<float *>(p);
   {
-    std::vector<ScanItem> iv;
-    for (uint32_t q0 = 0; q0 < m; q0 += kTQ) iv.push_back(ScanItem{0, R, q0, std::min((uint32_t)kTQ, m - q0), 0});
-    DevBuf<ScanItem> items;
-    RBC_TRY(upload_items(iv, items, st));
-    TileSrc src{h->reps.get(), h->ld, nullptr, nullptr, qd, ldq, nullptr, d, h->dpad};
-    EpiDense epi{Dqr, R};
-    RBC_TRY(launch_tiles_m(h->metric, src, items.get(), (uint32_t)iv.size(), epi, st));
-    RBC_HIP_TRY(hipStreamSynchronize(st));
+    const dim3 g(cdiv(m, kQRT), cdiv(R, kQRT));
+    if (h->metric == RBC_L2)
+      hipLaunchKernelGGL(qr_dist<0>, g, dim3(256), 0, st, qd, ldq, m, h->reps.get(), (size_t)h->ld, R, d, Dqr);
+    else
+      hipLaunchKernelGGL(qr_dist<1>, g, dim3(256), 0, st, qd, ldq, m, h->reps.get(), (size_t)h->ld, R, d, Dqr);
+    RBC_HIP_TRY(hipGetLastError());
   }
   const bool bf16 = precision == RBC_PREC_BF16;
   // The BF16 probe keeps at most 32 candidates, so it serves k <= 32.  It scans the Pp
