@@ -270,18 +270,14 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
     wave_bitonic_sort(v, id);
     wave_merge_lower(tv, ti, v, id);
   }
-  uint32_t myid = l < (int)Kp ? ti : kDummy;
-  float myv = FLT_MAX;
-  for (uint32_t cc = 0; cc < Kp; ++cc) {
-    uint32_t id = __shfl(myid, cc, kWave);
-    if (id == kDummy) continue;
-    float v = tree_dist_row<METRIC, NPL>(xp + size_t(inv[id]) * ldx, qv, d);
-    if (l == (int)cc) myv = v;
-  }
-  wave_bitonic_sort(myv, myid);
+  // Lane l < K' holds the l-th best; the re-rank gathers their rows eight at a time.
+  const uint32_t myid = l < (int)Kp ? ti : kDummy;
+  float rv = FLT_MAX;
+  uint32_t ri = kDummy;
+  rerank_candidates<METRIC, NPL>(rv, ri, Kp, [&](uint32_t) { return myid; }, xp, ldx, inv, qv, d);
   if (l < (int)Kp) {
-    pi[size_t(q) * Kp + l] = myid;
-    pd[size_t(q) * Kp + l] = myv;
+    pi[size_t(q) * Kp + l] = ri;
+    pd[size_t(q) * Kp + l] = rv;
   }
 }
 
