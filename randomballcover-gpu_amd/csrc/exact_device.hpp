@@ -29,23 +29,6 @@ __device__ void wave_sort_runs(uint32_t &run, double &key, uint32_t &id, float &
     }
 }
 
-template <int METRIC, int NPL>
-__device__ float tree_dist_row(const float *xrow, const float (&qv)[NPL], uint32_t d) {
-  const int l = lane_id();
-  float t[NPL];
-#pragma unroll
-  for (int j = 0; j < NPL; ++j) {
-    uint32_t c = l + kWave * j;
-    t[j] = c < d ? tree_term<METRIC>(xrow[c], qv[j]) : 0.f;
-  }
-  // Strides >= 64 inside the lane: t[j] covers coordinate l + 64 j.
-#pragma unroll
-  for (int s = NPL / 2; s >= 1; s >>= 1)
-#pragma unroll
-    for (int j = 0; j < s; ++j) t[j] = __fadd_rn(t[j], t[j + s]);
-  return wave_tree_sum(t[0]);
-}
-
 template <int METRIC>
 __device__ double fp64_dist(const float *xrow, const float *qrow, uint32_t d) {
   double acc = 0.0;
