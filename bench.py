@@ -16,6 +16,12 @@ duration on the query stream.  ``traffic`` is its HBM bytes per launch from two
 ``rocprofv3 --pmc`` passes (FETCH_SIZE, WRITE_SIZE) of a child run of the same query,
 each scaled by what the counter reads for a 1 GiB device copy.
 
+``query_roofline`` is the whole step against the sum of its stages' algorithmic bounds
+(q -> reps FLOPs at the FP32 peak, proposal and probe and scan FLOPs at the BF16/FP16 MFMA
+peak, re-rank gathers and query/result bytes at the HBM peak).  ``measured_peaks`` are
+the device's own BF16 MFMA and FP32 FMA rates (``rbc_measure_peaks``) and the
+bandwidth of a 4 GiB device copy, next to the published figures used for ``frac``.
+
 ``cpu_baseline`` is the oracle's FP32 brute force (the reference's bruteK arithmetic) on
 a bounded query sample, on the host's cores.
 
@@ -68,6 +74,46 @@ def copy_calibration(mib):
     b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes": a.numel()}))
+
+
+def measured_peaks(rbc, torch, mib=4096, reps=5):
+    """The device's BF16 MFMA and FP32 FMA rates and the bandwidth of a device copy."""
+    peaks = rbc.measure_peaks()
+    a = torch.empty(mib << 20, dtype=torch.uint8, device="cuda:0")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(reps):
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del a, b
+    torch.cuda.empty_cache()
+    peaks["hbm_copy_GBps"] = 2.0 * (mib << 20) / (best * 1e-3) / 1e9
+    return {k: round(v, 1) for k, v in peaks.items()}
+
+
+def query_roofline(st, m, d, k, num_reps, ms_step, mlp_dims):
+    """Whole-step roofline: the step's time against the sum of its stages' bounds."""
+    mfma = MFMA_BF16_PEAK_TFLOPS * 1e12
+    bound = {
+        "qreps": 2.0 * d * m * num_reps / (FP32_PEAK_TFLOPS * 1e12),
+        "proposal": (2.0 * m * sum(a * b for a, b in zip(mlp_dims[:-1], mlp_dims[1:])) / mfma
+                     if mlp_dims else 0.0),
+        "probe": 2.0 * d * st["pairs_probe"] / mfma,
+        "scan": 2.0 * d * st["pairs_scan"] / mfma,
+        "rerank": 4.0 * d * st["pairs_rerank"] / (HBM_PEAK_GBS * 1e9),
+        "io": (4.0 * d * m + 8.0 * k * m) / (HBM_PEAK_GBS * 1e9),
+    }
+    total_ms = 1e3 * sum(bound.values())
+    return {"bound_ms": round(total_ms, 3), "ms_per_step": round(ms_step, 3),
+            "frac": round(total_ms / ms_step, 4),
+            "stage_bound_ms": {kk: round(1e3 * v, 4) for kk, v in bound.items()}}
 
 
 def train_proposal(rbc, datagen, idx, workload, d, num_reps, count, epochs, seed=0):
@@ -211,6 +257,9 @@ def main():
     rbc = _load("rbc_ctypes", os.path.join(PKG, "rbc_ctypes.py"))
     datagen = _load("rbc_datagen", os.path.join(PKG, "datagen.py"))
     rbc.lib()
+    peaks = measured_peaks(rbc, torch)
+    log(f"measured peaks: BF16 MFMA {peaks['bf16_mfma_tflops']:,.0f} TFLOP/s, FP32 FMA "
+        f"{peaks['fp32_fma_tflops']:,.1f} TFLOP/s, device copy {peaks['hbm_copy_GBps']:,.0f} GB/s")
 
     t0 = time.time()
     x, q, num_reps, k, xd, qd, idx, build_s = build_index(args, rbc, datagen, torch)
@@ -293,11 +342,18 @@ def main():
                 "flops_per_launch": flops, "pairs_per_launch": st["pairs_scan"],
                 "alg_bytes_per_launch": st["bytes_scan"],
                 "alg_GBps": round(st["bytes_scan"] / (scan_ms * 1e-3) / 1e9, 1),
-                "kernel_ms": round(scan_ms, 3)}
+                "kernel_ms": round(scan_ms, 3),
+                "frac_of_measured_peak": round(tflops / peaks["bf16_mfma_tflops"], 4)}
     else:
         roof = {"bound": "valu", "achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
                 "kernel": "tile_kernel<EpiEmit>", "kernel_ms": round(scan_ms, 3)}
+
+    qroof = None
+    if prec == rbc.PREC_BF16:
+        qroof = query_roofline(st, m, d, k, num_reps, ms_step, net.dims if use_mlp else None)
+        log(f"whole step: {qroof['bound_ms']:.2f} ms of stage bounds in {ms_step:.2f} ms "
+            f"({100 * qroof['frac']:.1f} %)")
 
     if not args.no_fp32 and prec == rbc.PREC_BF16:
         idx.set_proposal(None)
@@ -381,6 +437,8 @@ def main():
                    "pairs_scan_frac_of_brute": round((st["pairs_scan"] + st["pairs_probe"]) / (n * m), 4)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "query_roofline": qroof,
+        "measured_peaks": peaks,
         "stages_ms": {s: round(st[f"ms_{s}"], 3) for s in ("qreps", "probe", "plan", "scan", "select")},
         **extra,
     }

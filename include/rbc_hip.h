@@ -51,6 +51,12 @@ int rbc_set_device(int device);
 int rbc_set_metric(int metric);
 int rbc_get_metric(void);
 
+/* Measured compute peaks of the current device, for roofline reports: dense BF16 MFMA
+ * (v_mfma_f32_16x16x32_bf16) and FP32 FMA (v_fma_f32) throughput in TFLOP/s, each from
+ * a kernel of independent register accumulation chains timed over about 20 ms.  No
+ * counterpart in the reference (its benchmarks report wall time only). */
+int rbc_measure_peaks(double *bf16_mfma_tflops, double *fp32_fma_tflops, void *stream);
+
 /* ---------------------------------------------------------------- brute force -- */
 
 /* bruteK (brute.cu:65-88 -> knnWrap kernelWrap.cu:95-111 -> knnKernel kernels.cu:229-282):

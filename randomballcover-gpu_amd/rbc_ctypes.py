@@ -70,6 +70,7 @@ _SIGS = {
     "rbc_set_device": (_int, [_int]),
     "rbc_set_metric": (_int, [_int]),
     "rbc_get_metric": (_int, []),
+    "rbc_measure_peaks": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _vp]),
     "rbc_brute_k": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _u32, _vp, _vp, _u32, _int,
                            _vp]),
     "rbc_brute_search": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _vp, _vp, _int, _vp]),
@@ -135,6 +136,13 @@ def lib():
 def _check(status):
     if status != 0:
         raise RBCError(status, lib().rbc_last_error().decode())
+
+
+def measure_peaks():
+    """Measured dense BF16 MFMA and FP32 FMA throughput of the current device (TFLOP/s)."""
+    a, b = ctypes.c_double(), ctypes.c_double()
+    _check(lib().rbc_measure_peaks(ctypes.byref(a), ctypes.byref(b), None))
+    return {"bf16_mfma_tflops": a.value, "fp32_fma_tflops": b.value}
 
 
 def _host(a, dtype):

@@ -54,3 +54,6 @@ def test_argument_errors_do_not_touch_the_gpu():
     assert st == 1
     st = L.rbc_brute_k(None, 0, 4, None, 0, 4, 4, 0, 33, None, None, 33, 0, None)
     assert st == 1
+    # Null outputs of the peak probe.
+    assert L.rbc_measure_peaks(None, None, None) == 1
+    assert b"null" in L.rbc_last_error()
