@@ -497,15 +497,34 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
 // operands are swapped with respect to the scan: A = 16 points of the list (from a
 // three-stage LDS ring), B = 16 queries of the wave, so that lane l's accumulator holds
 // four points of ONE entry (column l%16).  The four lanes l, l^16, l^32, l^48 see disjoint
-// quarters of the list for their entry; each keeps its 8 smallest approximate distances
-// sorted in registers (strict '<': among equal values the earlier point stays ahead).
-// After the scan the four lists of an entry are merged by one thread into its K' <= 32
-// best, with their approximate values.  The result is the exact top-K' of the list unless
-// one quarter held more than 8 of them; the caller only needs K' close points of the list
-// (any K' distinct points bound the k-th NN distance from above).
+// quarters of the list for their entry; each keeps its 8 smallest keys sorted in
+// registers.  A key is the approximate distance's float bits with the low idx_bits bits
+// replaced by the point's slot in the lane's quarter (step, tile, row), so keys are unique
+// per lane, order by value first (to 23 - idx_bits mantissa bits) and then by slot, and a
+// key is inserted branch-free with one v_med3_u32 per kept slot.  After the scan the four
+// lists of an entry are merged by one thread into its K' <= 32 best.  The result is the
+// top-K' of the list by truncated value unless one quarter held more than 8 of them; the
+// caller only needs K' close points of the list (any K' distinct points bound the k-th NN
+// distance from above, and the caller re-ranks them exactly).
 constexpr int kPL = 8;                 // kept per lane
 constexpr int kPK = 4 * kPL;           // largest K'
 constexpr int kPQ = kBf16ProbeQ;       // queries per probe item
+constexpr uint32_t kKeyInf = 0x7F800000u;  // keys at or above this hold no point
+
+// Median of three unsigned words (one VALU instruction).
+__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// Inserts key into the ascending list t (the largest drops out): slot j takes the median of
+// (t[j-1], key, t[j]), which is t[j-1], key or t[j] as key falls below, between or above.
+__device__ __forceinline__ void insert_key(uint32_t (&t)[kPL], uint32_t key) {
+#pragma unroll
+  for (int j = kPL - 1; j > 0; --j) t[j] = med3_u32(t[j - 1], key, t[j]);
+  t[0] = min(t[0], key);
+}
 
 struct Bf16ProbeArgs {
   const ScanItem *items;
@@ -518,8 +537,9 @@ struct Bf16ProbeArgs {
   const uint32_t *perm;
   uint32_t ldb, Kp;
   uint32_t P;        // entries per query: entry e belongs to query e / P
+  uint32_t idx_bits; // low key bits holding a point's slot (8 slots per 32-point step)
   uint32_t *out_i;   // [entries][Kp] original ids, ascending approximate distance
-  float *out_v;      // [entries][Kp] their approximate squared distances D~
+  float *out_v;      // [entries][Kp] their approximate squared distances D~ (truncated)
 };
 
 template <int KS>
@@ -540,13 +560,10 @@ __global__ __launch_bounds__(256, 2) void bf16_probe_kernel(Bf16ProbeArgs a) {
   for (int s = 0; s < KS; ++s)
     Bq[s] = myrow != kDummy ? load8(a.qb + size_t(myrow) * a.ldb + s * 32 + koff) : bf16x8{};
   const float qn = myrow != kDummy ? a.qn[myrow] : 0.f;
-  float td[kPL];
-  uint32_t tp[kPL];
+  const uint32_t vmask = ~((1u << a.idx_bits) - 1u);
+  uint32_t tk[kPL];
 #pragma unroll
-  for (int j = 0; j < kPL; ++j) {
-    td[j] = FLT_MAX;
-    tp[j] = kDummy;
-  }
+  for (int j = 0; j < kPL; ++j) tk[j] = 0xFFFFFFFFu;
   const uint32_t x0 = it.x_begin, np = it.x_count;
   const uint32_t nsteps = (np + kVP - 1) / kVP;
   const uint32_t sp = t / 8, sc = (t % 8) * 32;
@@ -596,56 +613,48 @@ __global__ __launch_bounds__(256, 2) void bf16_probe_kernel(Bf16ProbeArgs a) {
       const float xv[4] = {xn4.x, xn4.y, xn4.z, xn4.w};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        float v = __builtin_fmaf(-2.f, acc[r], xv[r]) + qn;
-        if (v < td[kPL - 1]) {
-          uint32_t p = s * kVP + pt * 16 + (l / 16) * 4 + r;
-#pragma unroll
-          for (int j = 0; j < kPL; ++j) {
-            const bool lt = v < td[j];
-            const float ov = td[j];
-            const uint32_t op = tp[j];
-            td[j] = lt ? v : ov;
-            tp[j] = lt ? p : op;
-            v = lt ? ov : v;
-            p = lt ? op : p;
-          }
-        }
+        // Negative values (rounding) clamp to 0; padded points (xn = INF) give INF keys.
+        const float v = fmaxf(__builtin_fmaf(-2.f, acc[r], xv[r]) + qn, 0.f);
+        insert_key(tk, (__float_as_uint(v) & vmask) | (s * 8 + pt * 4 + r));
       }
     }
   }
   // Hand the per-lane lists to one thread per query through LDS (the ring is free now).
   __syncthreads();
-  float *ld = reinterpret_cast<float *>(&ring[0][0]);              // [kPQ][4][kPL]
-  uint32_t *lp = reinterpret_cast<uint32_t *>(&ring[1][0]);        // [kPQ][4][kPL]
+  uint32_t *lk = reinterpret_cast<uint32_t *>(&ring[0][0]);        // [kPQ][4][kPL]
   if (wave_live) {
     const int q = w * 16 + (l % 16), g = l / 16;
 #pragma unroll
-    for (int j = 0; j < kPL; ++j) {
-      ld[(q * 4 + g) * kPL + j] = td[j];
-      lp[(q * 4 + g) * kPL + j] = tp[j];
-    }
+    for (int j = 0; j < kPL; ++j) lk[(q * 4 + g) * kPL + j] = tk[j];
   }
   __syncthreads();
   if (t >= kPQ || qrow[t] == kDummy) return;
   const uint32_t row = qrow[t];  // the entry
+  const uint32_t smask = ~vmask;
   uint32_t head[4] = {0, 0, 0, 0};
   for (uint32_t j = 0; j < a.Kp; ++j) {
     int best = -1;
-    float bv = FLT_MAX;
-    uint32_t bp = kDummy;
+    uint32_t bk = 0xFFFFFFFFu;
     for (int g = 0; g < 4; ++g) {
       if (head[g] >= (uint32_t)kPL) continue;
-      const float v = ld[(t * 4 + g) * kPL + head[g]];
-      const uint32_t p = lp[(t * 4 + g) * kPL + head[g]];
-      if (p != kDummy && (best < 0 || pair_less(v, p, bv, bp))) {
+      const uint32_t key = lk[(t * 4 + g) * kPL + head[g]];
+      if (key < kKeyInf && key < bk) {
         best = g;
-        bv = v;
-        bp = p;
+        bk = key;
       }
     }
-    if (best >= 0) ++head[best];
-    a.out_i[size_t(row) * a.Kp + j] = bp != kDummy ? a.perm[x0 + bp] : kDummy;
-    a.out_v[size_t(row) * a.Kp + j] = bp != kDummy ? bv : FLT_MAX;
+    uint32_t id = kDummy;
+    float v = FLT_MAX;
+    if (best >= 0) {
+      ++head[best];
+      // Slot = step * 8 + tile * 4 + row; the lane's quarter is g = best.
+      const uint32_t slot = bk & smask;
+      const uint32_t p = (slot >> 3) * kVP + ((slot >> 2) & 1u) * 16 + best * 4 + (slot & 3u);
+      id = a.perm[x0 + p];
+      v = __uint_as_float(bk & vmask);
+    }
+    a.out_i[size_t(row) * a.Kp + j] = id;
+    a.out_v[size_t(row) * a.Kp + j] = v;
   }
 }
 
@@ -749,6 +758,17 @@ int bf16_probe(rbc_exact *h, const uint16_t *qb, const float *qn, const uint32_t
   a.ldb = h->ldb;
   a.Kp = Kp;
   a.P = P;
+  // Slot bits: 8 slots per 32-point step of the longest list.
+  uint32_t max_len = 0;
+  for (uint32_t r = 0; r < h->R; ++r) max_len = std::max(max_len, h->h_list_off[r + 1] - h->h_list_off[r]);
+  const uint32_t slots = std::max(1u, (max_len + kVP - 1) / kVP) * 8;
+  uint32_t bits = 3;
+  while ((1u << bits) < slots) ++bits;
+  if (bits > 23) {  // lists of more than 2^25 points
+    set_error("BF16 probe: a list of %u points is too long for its keys", max_len);
+    return RBC_ERR_LIMIT;
+  }
+  a.idx_bits = bits;
   a.out_i = out_i;
   a.out_v = out_v;
   switch (h->ldb / 32) {
