@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
 // Per (query, list) pair every load is independent of the others; the per-query values
 // are wave-uniform.
 constexpr uint32_t kPlanQ = 32;   // plan_scatter: queries per block
-constexpr uint32_t kPlanCQ = 16;  // plan_count: queries per block
+constexpr uint32_t kPlanCQ = 64;  // plan_count: queries per block (fewer list_cnt atomics)
 
 __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, uint32_t R,
                                                   const uint32_t *near,
@@ -327,41 +327,45 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
           const double rhs = Gp + psi;
           const double Dlo = Dqj * (1.0 - e);
           keep = Dlo <= rhs * rhs * (1.0 + 1e-13);
-          // The plane between r_j and r_i: an x owned by r_j satisfies
-          // |x-r_j|^2 <= |x-r_i|^2 + 2e psi^2 for EVERY rep r_i (its owner is the FP32
-          // argmin over all reps), so the nearest rep and each probed rep give a lower
-          // bound, and the largest decides.
-          // All loads first (independent of each other), then the tests.
-          uint32_t ji[1 + kPlaneReps];
-          float Dqi[1 + kPlaneReps], den[1 + kPlaneReps];
-          ji[0] = j0;
+          // The planes only matter for lists the triangle bound keeps; their rows of
+          // rr_den are not read for the others.
+          if (keep) {
+            // The plane between r_j and r_i: an x owned by r_j satisfies
+            // |x-r_j|^2 <= |x-r_i|^2 + 2e psi^2 for EVERY rep r_i (its owner is the FP32
+            // argmin over all reps), so the nearest rep and each probed rep give a lower
+            // bound, and the largest decides.
+            // All loads first (independent of each other), then the tests.
+            uint32_t ji[1 + kPlaneReps];
+            float Dqi[1 + kPlaneReps], den[1 + kPlaneReps];
+            ji[0] = j0;
 #pragma unroll
-          for (uint32_t pi = 0; pi < kPlaneReps; ++pi) {
-            const uint32_t r = pi < P ? planeR[size_t(q) * P + pi] : R;
-            ji[1 + pi] = r < R && r != j0 ? r : R;
+            for (uint32_t pi = 0; pi < kPlaneReps; ++pi) {
+              const uint32_t r = pi < P ? planeR[size_t(q) * P + pi] : R;
+              ji[1 + pi] = r < R && r != j0 ? r : R;
+            }
+#pragma unroll
+            for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) {
+              const bool ok = ji[pi] < R && ji[pi] != j;
+              Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
+              const float dn = ok ? rr_den[size_t(ji[pi]) * R + j] : 0.f;
+              den[pi] = dn > 0.f ? dn : INFINITY;  // no plane: never prunes
+            }
+            // The plane test num > G' den in FP32, with every rounding pushed towards
+            // keeping: A <= Dlo - slack (rounded down), and fl(Gf den + B_i) >= G' den +
+            // Dqi (1+e), since Gf >= G' (1 + 8u) and B_i = fl(Dqi fl(1 + e + 8u)) >=
+            // Dqi (1+e)(1 + 6u) (u = 2^-24).  A > that implies the exact test.
+            const double Ad = Dlo - slack;
+            float A = (float)Ad;
+            if ((double)A > Ad) A = nextafterf(A, -INFINITY);
+            const double Gu = Gd * (1.0 + 0x1p-21);
+            float Gf = (float)Gu;
+            if ((double)Gf < Gu) Gf = nextafterf(Gf, INFINITY);
+            const float cb1 = (float)(1.0 + e + 0x1p-21);
+            bool far = false;
+#pragma unroll
+            for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) far |= A > __builtin_fmaf(Gf, den[pi], Dqi[pi] * cb1);
+            if (far) keep = false;
           }
-#pragma unroll
-          for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) {
-            const bool ok = ji[pi] < R && ji[pi] != j;
-            Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
-            const float dn = ok ? rr_den[size_t(ji[pi]) * R + j] : 0.f;
-            den[pi] = dn > 0.f ? dn : INFINITY;  // no plane: never prunes
-          }
-          // The plane test num > G' den in FP32, with every rounding pushed towards
-          // keeping: A <= Dlo - slack (rounded down), and fl(Gf den + B_i) >= G' den +
-          // Dqi (1+e), since Gf >= G' (1 + 8u) and B_i = fl(Dqi fl(1 + e + 8u)) >=
-          // Dqi (1+e)(1 + 6u) (u = 2^-24).  A > that implies the exact test.
-          const double Ad = Dlo - slack;
-          float A = (float)Ad;
-          if ((double)A > Ad) A = nextafterf(A, -INFINITY);
-          const double Gu = Gd * (1.0 + 0x1p-21);
-          float Gf = (float)Gu;
-          if ((double)Gf < Gu) Gf = nextafterf(Gf, INFINITY);
-          const float cb1 = (float)(1.0 + e + 0x1p-21);
-          bool far = false;
-#pragma unroll
-          for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) far |= A > __builtin_fmaf(Gf, den[pi], Dqi[pi] * cb1);
-          if (far) keep = false;
         } else {
           const double lb = list_lower_bound(Dqj, 0.0, 0.0, psi, e, true, metric);
           keep = lb <= G;
