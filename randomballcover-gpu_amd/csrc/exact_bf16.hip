@@ -497,17 +497,19 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
 // operands are swapped with respect to the scan: A = 16 points of the list (from a
 // three-stage LDS ring), B = 16 queries of the wave, so that lane l's accumulator holds
 // four points of ONE entry (column l%16).  The four lanes l, l^16, l^32, l^48 see disjoint
-// quarters of the list for their entry; each keeps its 8 smallest keys sorted in
+// quarters of the list for their entry; each keeps its kPL smallest keys sorted in
 // registers.  A key is the approximate distance's float bits with the low idx_bits bits
 // replaced by the point's slot in the lane's quarter (step, tile, row), so keys are unique
 // per lane, order by value first (to 23 - idx_bits mantissa bits) and then by slot, and a
 // key is inserted branch-free with one v_med3_u32 per kept slot.  After the scan the four
-// lists of an entry are merged by one thread into its K' <= 32 best.  The result is the
-// top-K' of the list by truncated value unless one quarter held more than 8 of them; the
-// caller only needs K' close points of the list (any K' distinct points bound the k-th NN
-// distance from above, and the caller re-ranks them exactly).
-constexpr int kPL = 8;                 // kept per lane
-constexpr int kPK = 4 * kPL;           // largest K'
+// lists of an entry are merged by one thread into its min(K', 4 kPL) best (the remaining
+// of its K' slots are empty).  The caller merges the entries of a query, re-ranks the
+// best K' exactly and takes the k-th: any k distinct points bound the k-th NN distance
+// from above, so the probe only has to find close points, not the exact top-K' of each
+// list.  A query's k nearest neighbours are spread over several lists, and 4 per lane (16
+// per entry) scan 0.03 % more pairs than 8 per lane at a quarter less probe time.
+constexpr int kPL = 4;                 // kept per lane
+constexpr int kPK = 32;                // largest K'
 constexpr int kPQ = kBf16ProbeQ;       // queries per probe item
 constexpr uint32_t kKeyInf = 0x7F800000u;  // keys at or above this hold no point
 
