@@ -43,15 +43,27 @@ namespace {
 // coordinate in coordinate order -- the values of the tile engine's EpiDense, so the
 // error bound fma_rel_err(dpad) holds.  128 x 128 tiles, 8 x 8 per thread (rows
 // ty*4.. and 64+ty*4.. of the query tile, columns tx*4.. and 64+tx*4.. of the rep tile),
-// 32 coordinates at a time staged transposed in LDS; staging threads walk the rows, so
-// the transposed stores are free of bank conflicts.
-constexpr int kQRT = 128, kQRC = 32;
+// 32 coordinates at a time staged transposed in LDS.  Staging: 8 threads read one row's
+// 32 coordinates as 16-byte loads (a 128-byte run; the query tile falls back to 4-byte
+// loads when its rows are not 16-byte aligned), and write them transposed into rows
+// padded to 132 floats (at most 2-way bank conflicts on the stores).
+constexpr int kQRT = 128, kQRC = 32, kQRP = kQRT + 4;
 
-template <int METRIC>
+// Coordinates c..c+3 of a row (zero from d on); c is a multiple of 4 below d, and the row
+// holds at least the next multiple of 4 floats, so the load stays inside it.
+__device__ __forceinline__ float4 load_coords4(const float *row, uint32_t c, uint32_t d) {
+  float4 v = *reinterpret_cast<const float4 *>(row + c);
+  if (c + 1 >= d) v.y = 0.f;
+  if (c + 2 >= d) v.z = 0.f;
+  if (c + 3 >= d) v.w = 0.f;
+  return v;
+}
+
+template <int METRIC, bool VQ>
 __global__ __launch_bounds__(256) void qr_dist(const float *Q, size_t ldq, uint32_t m, const float *Rp, size_t ldr,
                                                uint32_t R, uint32_t d, float *D) {
-  __shared__ __attribute__((aligned(16))) float Qs[kQRC][kQRT];
-  __shared__ __attribute__((aligned(16))) float Rs[kQRC][kQRT];
+  __shared__ __attribute__((aligned(16))) float Qs[kQRC][kQRP];
+  __shared__ __attribute__((aligned(16))) float Rs[kQRC][kQRP];
   const int t = threadIdx.x, ty = t / 16, tx = t % 16;
   const uint32_t q0 = blockIdx.x * kQRT, r0 = blockIdx.y * kQRT;
   float acc[8][8];
@@ -59,13 +71,34 @@ __global__ __launch_bounds__(256) void qr_dist(const float *Q, size_t ldq, uint3
   for (int i = 0; i < 8; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[i][j] = 0.f;
+  const uint32_t srow = t / 8, scol = (t % 8) * 4;  // staging: rows srow + 32 s, coordinates scol..+3
   for (uint32_t c0 = 0; c0 < d; c0 += kQRC) {
     __syncthreads();
-#pragma unroll 4
-    for (int i = t; i < kQRT * kQRC; i += 256) {
-      const uint32_t row = i % kQRT, c = c0 + i / kQRT;
-      Qs[i / kQRT][row] = q0 + row < m && c < d ? Q[size_t(q0 + row) * ldq + c] : 0.f;
-      Rs[i / kQRT][row] = r0 + row < R && c < d ? Rp[size_t(r0 + row) * ldr + c] : 0.f;
+    const uint32_t c = c0 + scol;
+#pragma unroll
+    for (int s = 0; s < kQRT / 32; ++s) {
+      const uint32_t row = srow + 32 * s;
+      float4 qv = make_float4(0.f, 0.f, 0.f, 0.f), rv = qv;
+      if (q0 + row < m && c < d) {
+        const float *qr = Q + size_t(q0 + row) * ldq;
+        if constexpr (VQ) {
+          qv = load_coords4(qr, c, d);
+        } else {
+          qv.x = qr[c];
+          qv.y = c + 1 < d ? qr[c + 1] : 0.f;
+          qv.z = c + 2 < d ? qr[c + 2] : 0.f;
+          qv.w = c + 3 < d ? qr[c + 3] : 0.f;
+        }
+      }
+      if (r0 + row < R && c < d) rv = load_coords4(Rp + size_t(r0 + row) * ldr, c, d);
+      Qs[scol][row] = qv.x;
+      Qs[scol + 1][row] = qv.y;
+      Qs[scol + 2][row] = qv.z;
+      Qs[scol + 3][row] = qv.w;
+      Rs[scol][row] = rv.x;
+      Rs[scol + 1][row] = rv.y;
+      Rs[scol + 2][row] = rv.z;
+      Rs[scol + 3][row] = rv.w;
     }
     __syncthreads();
     const uint32_t cn = min((uint32_t)kQRC, d - c0);
@@ -538,10 +571,21 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   float *Dqr = static_cast<float *>(p);
   {
     const dim3 g(cdiv(m, kQRT), cdiv(R, kQRT));
-    if (h->metric == RBC_L2)
-      hipLaunchKernelGGL(qr_dist<0>, g, dim3(256), 0, st, qd, ldq, m, h->reps.get(), (size_t)h->ld, R, d, Dqr);
-    else
-      hipLaunchKernelGGL(qr_dist<1>, g, dim3(256), 0, st, qd, ldq, m, h->reps.get(), (size_t)h->ld, R, d, Dqr);
+    // 16-byte query loads need 16-byte aligned rows (the reps are padded to 16 floats).
+    const bool vq = ldq % 4 == 0 && reinterpret_cast<uintptr_t>(qd) % 16 == 0;
+    const float *rp = h->reps.get();
+    const size_t ldr = h->ld;
+    if (h->metric == RBC_L2) {
+      if (vq)
+        hipLaunchKernelGGL((qr_dist<0, true>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
+      else
+        hipLaunchKernelGGL((qr_dist<0, false>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
+    } else {
+      if (vq)
+        hipLaunchKernelGGL((qr_dist<1, true>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
+      else
+        hipLaunchKernelGGL((qr_dist<1, false>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
+    }
     RBC_HIP_TRY(hipGetLastError());
   }
   const bool bf16 = precision == RBC_PREC_BF16;

@@ -129,6 +129,26 @@ def test_exact_bf16_duplicates():
     check_against_fp64(x, q, ids, ds, 10)
 
 
+@pytest.mark.parametrize("precision,metric", [(rbc.PREC_FP32, 0), (rbc.PREC_BF16, 0), (rbc.PREC_FP32, 1)])
+def test_exact_pitched_device_queries_ignore_padding(precision, metric):
+    """Device queries with ldq > d, d not a multiple of 4 and ldq a multiple of 4 (the q->reps
+    distances then stage queries with 16-byte loads that reach past d), and 1e6 in the
+    padding columns: only the d coordinates enter any distance."""
+    import torch
+    n, d, m, k, ldq = 20000, 33, 3000, 7, 36
+    x = datagen.uniform(n, d, 0)
+    q = datagen.uniform(m, d, 1)
+    qp = np.full((m, ldq), 1e6, np.float32)
+    qp[:, :d] = q
+    idx = rbc.ExactRBC(x, 141, metric=metric, seed=3)
+    qd = torch.from_numpy(qp).cuda()[:, :d]
+    assert qd.stride(0) == ldq
+    gi = torch.empty((m, k), dtype=torch.int32, device="cuda")
+    gd = torch.empty((m, k), dtype=torch.float32, device="cuda")
+    idx.kquery_device(qd, k, gi, gd, precision=precision)
+    check_against_fp64(x, q, gi.cpu().numpy().view(np.uint32), gd.cpu().numpy(), k, metric)
+
+
 def test_exact_strided_host_buffers_match_device_path():
     """Host inputs with ldq > d and outputs with ldo > k (re-pitched copies) give the same
     answers as the device path, and the output padding columns are left untouched."""
