@@ -510,6 +510,7 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
 // per entry) scan 0.03 % more pairs than 8 per lane at a quarter less probe time.
 constexpr int kPL = 4;                 // kept per lane
 constexpr int kPK = 32;                // largest K'
+static_assert(4 * kPL == (int)kBf16ProbeKeep, "probe entries hold 4 lanes x kPL keys");
 constexpr int kPQ = kBf16ProbeQ;       // queries per probe item
 constexpr uint32_t kKeyInf = 0x7F800000u;  // keys at or above this hold no point
 

@@ -271,15 +271,15 @@ __global__ void proposal_slot(const uint32_t *prop, uint32_t m, uint32_t P, uint
   kq[P - 1] = p;
 }
 
-// Multi-list probe, per query (one wave): the P lists' K' best approximate pairs are merged
-// by approximate value, and the K' best of them are re-ranked with tree-ordered FP32
+// Multi-list probe, per query (one wave): the P lists' Ke best approximate pairs each are
+// merged by approximate value, and the K' best of them are re-ranked with tree-ordered FP32
 // distances into (pd, pi) ascending -- K' distinct database points, so the k-th of them
 // bounds the k-th NN distance from above.
 template <int METRIC, int NPL>
 __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, const float *xp, size_t ldx,
                                                    const uint32_t *inv, uint32_t m, uint32_t d, uint32_t P,
-                                                   uint32_t Kp, const float *pv, const uint32_t *pid,
-                                                   uint32_t *pi, float *pd) {
+                                                   uint32_t Ke, uint32_t Kp, const float *pv,
+                                                   const uint32_t *pid, uint32_t *pi, float *pd) {
   const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
   if (q >= m) return;
   const int l = lane_id();
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
   }
   float tv = FLT_MAX;
   uint32_t ti = kDummy;
-  const uint32_t total = P * Kp;
+  const uint32_t total = P * Ke;
   const float *vq = pv + size_t(q) * total;
   const uint32_t *iq = pid + size_t(q) * total;
   for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
@@ -477,19 +477,20 @@ __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
 // Exact FP32 (tree-order) distances of a probe's K' candidates, re-sorted by
 // (distance, id); one wave per query.
 
-int launch_probe_merge(const float *Q, size_t ldq, const rbc_exact *h, uint32_t m, uint32_t P, uint32_t Kp,
-                       const float *pv, const uint32_t *pid, uint32_t *pi, float *pd, hipStream_t st) {
+int launch_probe_merge(const float *Q, size_t ldq, const rbc_exact *h, uint32_t m, uint32_t P, uint32_t Ke,
+                       uint32_t Kp, const float *pv, const uint32_t *pid, uint32_t *pi, float *pd,
+                       hipStream_t st) {
   uint32_t T = tree_width(h->d);
   dim3 g(cdiv(m, 4)), b(256);
   if (T <= 64)
     hipLaunchKernelGGL((probe_merge<0, 1>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, P, Kp, pv, pid, pi, pd);
+                       h->d, P, Ke, Kp, pv, pid, pi, pd);
   else if (T == 128)
     hipLaunchKernelGGL((probe_merge<0, 2>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, P, Kp, pv, pid, pi, pd);
+                       h->d, P, Ke, Kp, pv, pid, pi, pd);
   else
     hipLaunchKernelGGL((probe_merge<0, 4>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, P, Kp, pv, pid, pi, pd);
+                       h->d, P, Ke, Kp, pv, pid, pi, pd);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
@@ -673,18 +674,20 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     RBC_TRY(counting_sort(nearP, ne, R, order, off, ws, 0, st));
     RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st,
                               kBf16ProbeQ));
-    RBC_TRY(ws.get(52, ne * KpUse * 4, &p));
+    // Each entry holds at most kBf16ProbeKeep candidates; the merge reads that many.
+    const uint32_t Ke = std::min(KpUse, kBf16ProbeKeep);
+    RBC_TRY(ws.get(52, ne * Ke * 4, &p));
     float *ev = static_cast<float *>(p);
-    RBC_TRY(ws.get(53, ne * KpUse * 4, &p));
+    RBC_TRY(ws.get(53, ne * Ke * 4, &p));
     uint32_t *eid = static_cast<uint32_t *>(p);
-    RBC_TRY(fill_f32(ev, FLT_MAX, ne * KpUse, st));
-    RBC_TRY(fill_u32(eid, kDummy, ne * KpUse, st));
-    RBC_TRY(bf16_probe(h, qb, qn, order, Pp, pitems.get(), npitems, KpUse, eid, ev, st));
+    RBC_TRY(fill_f32(ev, FLT_MAX, ne * Ke, st));
+    RBC_TRY(fill_u32(eid, kDummy, ne * Ke, st));
+    RBC_TRY(bf16_probe(h, qb, qn, order, Pp, pitems.get(), npitems, Ke, eid, ev, st));
     RBC_TRY(ws.get(25, size_t(m) * KpUse * 4, &p));
     probe_d = static_cast<float *>(p);
     RBC_TRY(ws.get(26, size_t(m) * KpUse * 4, &p));
     probe_i = static_cast<uint32_t *>(p);
-    RBC_TRY(launch_probe_merge(qd, ldq, h, m, Pp, KpUse, ev, eid, probe_i, probe_d, st));
+    RBC_TRY(launch_probe_merge(qd, ldq, h, m, Pp, Ke, KpUse, ev, eid, probe_i, probe_d, st));
   } else {
     RBC_TRY(run_probe(near, 23, 24, 25, 26, &probe_d, &probe_i));
   }

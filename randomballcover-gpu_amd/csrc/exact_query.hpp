@@ -45,6 +45,8 @@ constexpr uint32_t kPlaneReps = 12;
 constexpr uint32_t kBf16ScanQ = 256;
 // Queries per work item of the BF16 probe.
 constexpr uint32_t kBf16ProbeQ = 64;
+// Candidates the BF16 probe finds per (query, list) entry (4 lanes x 4 kept keys).
+constexpr uint32_t kBf16ProbeKeep = 16;
 // Relative error constant of the BF16 GEMM distance: gamma_K + 6u', u' = 2^-23.
 double bf16_gemm_err(const rbc_exact *h);
 // Scan items must come from make_bucket_items over list_qoff (R + 1 offsets of the
