@@ -66,49 +66,10 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return 0.5f * (1.f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * __expf(-0.5f * z * z);
 }
 
-// KC: both operands are FP16 and contiguous along k (a_cs == b_rs == 1, 16-byte aligned
-// rows), so every thread stages 8 consecutive k of one row with one 16-byte load.
-template <typename TA, typename TB, bool KC>
-__global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) _Float16 As[kGT][kLdsPad];
-  __shared__ __attribute__((aligned(16))) _Float16 Bs[kGT][kLdsPad];  // Bs[j][k]
-  const int t = threadIdx.x, w = t / 64, l = t % 64;
-  const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
-  const TA *A = static_cast<const TA *>(g.A);
-  const TB *B = static_cast<const TB *>(g.B);
-  f32x4 acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int lr = t / 4, lk = (t % 4) * 8;
-  for (int k0 = 0; k0 < g.K; k0 += kGK) {
-    __syncthreads();
-    if constexpr (KC) {
-      const int i = i0 + lr, j = j0 + lr, k = k0 + lk;
-      const f16x8 z = f16x8{};
-      *reinterpret_cast<f16x8 *>(&As[lr][lk]) =
-          (i < g.M && k < g.K) ? *reinterpret_cast<const f16x8 *>(A + int64_t(i) * g.a_rs + k) : z;
-      *reinterpret_cast<f16x8 *>(&Bs[lr][lk]) =
-          (j < g.N && k < g.K) ? *reinterpret_cast<const f16x8 *>(B + int64_t(j) * g.b_cs + k) : z;
-    } else {
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        int i = i0 + lr, k = k0 + lk + e, j = j0 + lr;
-        As[lr][lk + e] = (i < g.M && k < g.K) ? to_h(A[i * g.a_rs + k * g.a_cs]) : (_Float16)0.f;
-        Bs[lr][lk + e] = (j < g.N && k < g.K) ? to_h(B[k * g.b_rs + j * g.b_cs]) : (_Float16)0.f;
-      }
-    }
-    __syncthreads();
-    const int rb = (w / 2) * 32, cb = (w % 2) * 32;
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
-      f16x8 av = *reinterpret_cast<const f16x8 *>(&As[rb + bi * 16 + l % 16][(l / 16) * 8]);
-#pragma unroll
-      for (int bj = 0; bj < 2; ++bj) {
-        f16x8 bv = *reinterpret_cast<const f16x8 *>(&Bs[cb + bj * 16 + l % 16][(l / 16) * 8]);
-        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[bi][bj], 0, 0, 0);
-      }
-    }
-  }
+// Result of wave w's 32 x 32 quarter of the 64 x 64 tile at (i0, j0): lane l holds rows
+// rb + bi*16 + 4(l/16) + r, column cb + bj*16 + l%16 in acc[bi][bj][r].
+__device__ __forceinline__ void gemm_epilogue(const GemmArgs &g, const f32x4 (&acc)[2][2], int i0, int j0, int w,
+                                              int l) {
   const int rb = (w / 2) * 32, cb = (w % 2) * 32;
   if (g.amax) {
     // The 16 lanes l/16 = c hold rows rb + bi*16 + 4c + r; each reduces its two columns,
@@ -147,16 +108,207 @@ __global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
       }
 }
 
+// KC: both operands FP16 and contiguous along k (a_cs == b_rs == 1, 16-byte aligned rows;
+// the forward pass).  64-wide k chunks: every thread stages two 16-byte pieces of one row
+// of A and of B per chunk, and issues the next chunk's loads before the current chunk's
+// MFMAs so that their latency overlaps the math.  Row pitch 72 halves: the 16 rows of a
+// read pass fall on distinct banks.
+constexpr int kKC = 64, kKCPad = kKC + 8;
+
+__global__ __launch_bounds__(256) void gemm_f16_kc(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) _Float16 As[kGT][kKCPad];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[kGT][kKCPad];  // Bs[j][k]
+  const int t = threadIdx.x, w = t / 64, l = t % 64;
+  const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
+  const _Float16 *A = static_cast<const _Float16 *>(g.A);
+  const _Float16 *B = static_cast<const _Float16 *>(g.B);
+  const int lr = t / 4, lk = (t % 4) * 8;  // row; k offsets lk and lk + 32 of the chunk
+  const int i = i0 + lr, j = j0 + lr;
+  f16x8 ra[2], rb[2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int k = k0 + lk + 32 * e;
+      ra[e] = (i < g.M && k < g.K) ? *reinterpret_cast<const f16x8 *>(A + int64_t(i) * g.a_rs + k) : f16x8{};
+      rb[e] = (j < g.N && k < g.K) ? *reinterpret_cast<const f16x8 *>(B + int64_t(j) * g.b_cs + k) : f16x8{};
+    }
+  };
+  f32x4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int rb0 = (w / 2) * 32, cb0 = (w % 2) * 32;
+  gload(0);
+  for (int k0 = 0; k0 < g.K; k0 += kKC) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      *reinterpret_cast<f16x8 *>(&As[lr][lk + 32 * e]) = ra[e];
+      *reinterpret_cast<f16x8 *>(&Bs[lr][lk + 32 * e]) = rb[e];
+    }
+    __syncthreads();
+    if (k0 + kKC < g.K) gload(k0 + kKC);
+#pragma unroll
+    for (int ks = 0; ks < kKC / 32; ++ks)
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) {
+        const f16x8 av = *reinterpret_cast<const f16x8 *>(&As[rb0 + bi * 16 + l % 16][ks * 32 + (l / 16) * 8]);
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+          const f16x8 bv = *reinterpret_cast<const f16x8 *>(&Bs[cb0 + bj * 16 + l % 16][ks * 32 + (l / 16) * 8]);
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[bi][bj], 0, 0, 0);
+        }
+      }
+  }
+  gemm_epilogue(g, acc, i0, j0, w, l);
+}
+
+// Last layer with the row argmax fused (inference): only argmax_j (alpha A W^T + b)(i, j)
+// is kept, as an argmax_key per row.  A block owns 64 rows and walks all N columns in
+// 64-column tiles: wave w keeps rows (w/2)*32.. of A in registers (2 x KS MFMA fragments),
+// the tile of W (64 rows of W, K = 32 KS) is staged in LDS once per block and shared by the
+// 4 waves, the next tile's loads are issued before the current tile's MFMAs, and every
+// lane keeps the best key of each of its 8 rows.  Row pitch CPR + 1 chunks (CPR even): the
+// 16 rows of a read pass fall on distinct banks.
+template <int KS>
+__global__ __launch_bounds__(256) void gemm_f16_argmax(GemmArgs g) {
+  constexpr int K = 32 * KS, CPR = K / 8, PITCH = CPR + 1;
+  __shared__ f16x8 Ws[kGT * PITCH];
+  const int t = threadIdx.x, w = t / 64, l = t % 64;
+  const int i0 = blockIdx.x * kGT, rb = (w / 2) * 32, cb = (w % 2) * 32;
+  const _Float16 *A = static_cast<const _Float16 *>(g.A);
+  const _Float16 *W = static_cast<const _Float16 *>(g.B);  // column j of B = row j of W
+  const float alpha = g.alpha != 0.f ? g.alpha : 1.f;
+  f16x8 af[2][KS];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi) {
+    const int i = i0 + rb + bi * 16 + l % 16;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      af[bi][ks] = i < g.M ? *reinterpret_cast<const f16x8 *>(A + int64_t(i) * g.a_rs + ks * 32 + (l / 16) * 8)
+                           : f16x8{};
+  }
+  unsigned long long best[2][4];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) best[bi][r] = 0ull;
+  // Staging: chunk c = t + 256 e of the tile is chunk c % CPR of W row j0 + c / CPR.
+  f16x8 wr[KS];
+  auto wload = [&](int j0) {
+#pragma unroll
+    for (int e = 0; e < KS; ++e) {
+      const int c = t + 256 * e, j = j0 + c / CPR;
+      wr[e] = j < g.N ? *reinterpret_cast<const f16x8 *>(W + int64_t(j) * g.b_cs + (c % CPR) * 8) : f16x8{};
+    }
+  };
+  wload(0);
+  for (int j0 = 0; j0 < g.N; j0 += kGT) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < KS; ++e) {
+      const int c = t + 256 * e;
+      Ws[(c / CPR) * PITCH + c % CPR] = wr[e];
+    }
+    __syncthreads();
+    if (j0 + kGT < g.N) wload(j0 + kGT);
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const f16x8 bv = Ws[(cb + bj * 16 + l % 16) * PITCH + ks * 4 + l / 16];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[bi][ks], bv, acc[bi][bj], 0, 0, 0);
+      }
+    // Lane l holds rows rb + bi*16 + 4(l/16) + r of columns j0 + cb + bj*16 + l%16.
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int j = j0 + cb + bj * 16 + l % 16;
+      if (j < g.N) {
+        const float bias = g.bias ? g.bias[j] : 0.f;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) best[bi][r] = max(best[bi][r], argmax_key(alpha * acc[bi][bj][r] + bias, j));
+      }
+    }
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      unsigned long long b = best[bi][r];
+#pragma unroll
+      for (int sh = 1; sh < 16; sh <<= 1) b = max(b, __shfl_xor(b, sh, 64));
+      const int i = i0 + rb + bi * 16 + (l / 16) * 4 + r;
+      if (l % 16 == 0 && i < g.M && b) atomicMax(&g.amax[i], b);
+    }
+}
+
+// General operands (any element strides, FP32 or FP16): 32-wide k chunks staged element by
+// element.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) _Float16 As[kGT][kLdsPad];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[kGT][kLdsPad];  // Bs[j][k]
+  const int t = threadIdx.x, w = t / 64, l = t % 64;
+  const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
+  const TA *A = static_cast<const TA *>(g.A);
+  const TB *B = static_cast<const TB *>(g.B);
+  f32x4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int lr = t / 4, lk = (t % 4) * 8;
+  for (int k0 = 0; k0 < g.K; k0 += kGK) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      int i = i0 + lr, k = k0 + lk + e, j = j0 + lr;
+      As[lr][lk + e] = (i < g.M && k < g.K) ? to_h(A[i * g.a_rs + k * g.a_cs]) : (_Float16)0.f;
+      Bs[lr][lk + e] = (j < g.N && k < g.K) ? to_h(B[k * g.b_rs + j * g.b_cs]) : (_Float16)0.f;
+    }
+    __syncthreads();
+    const int rb = (w / 2) * 32, cb = (w % 2) * 32;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+      f16x8 av = *reinterpret_cast<const f16x8 *>(&As[rb + bi * 16 + l % 16][(l / 16) * 8]);
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        f16x8 bv = *reinterpret_cast<const f16x8 *>(&Bs[cb + bj * 16 + l % 16][(l / 16) * 8]);
+        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(av, bv, acc[bi][bj], 0, 0, 0);
+      }
+    }
+  }
+  gemm_epilogue(g, acc, i0, j0, w, l);
+}
+
 template <typename TA, typename TB>
 int gemm(const GemmArgs &g, hipStream_t st) {
   dim3 grid(cdiv(g.N, kGT), cdiv(g.M, kGT));
   const bool kc = sizeof(TA) == 2 && sizeof(TB) == 2 && g.a_cs == 1 && g.b_rs == 1 && g.K % 8 == 0 &&
                   g.a_rs % 8 == 0 && g.b_cs % 8 == 0 && reinterpret_cast<uintptr_t>(g.A) % 16 == 0 &&
                   reinterpret_cast<uintptr_t>(g.B) % 16 == 0;
-  if (kc)
-    hipLaunchKernelGGL((gemm_f16<TA, TB, true>), grid, dim3(256), 0, st, g);
+  if (kc && g.amax && g.K % 32 == 0 && g.K <= 256) {
+    const dim3 rows(cdiv(g.M, kGT));
+    switch (g.K / 32) {
+      case 1: hipLaunchKernelGGL(gemm_f16_argmax<1>, rows, dim3(256), 0, st, g); break;
+      case 2: hipLaunchKernelGGL(gemm_f16_argmax<2>, rows, dim3(256), 0, st, g); break;
+      case 3: hipLaunchKernelGGL(gemm_f16_argmax<3>, rows, dim3(256), 0, st, g); break;
+      case 4: hipLaunchKernelGGL(gemm_f16_argmax<4>, rows, dim3(256), 0, st, g); break;
+      case 5: hipLaunchKernelGGL(gemm_f16_argmax<5>, rows, dim3(256), 0, st, g); break;
+      case 6: hipLaunchKernelGGL(gemm_f16_argmax<6>, rows, dim3(256), 0, st, g); break;
+      case 7: hipLaunchKernelGGL(gemm_f16_argmax<7>, rows, dim3(256), 0, st, g); break;
+      default: hipLaunchKernelGGL(gemm_f16_argmax<8>, rows, dim3(256), 0, st, g); break;
+    }
+  } else if (kc)
+    hipLaunchKernelGGL(gemm_f16_kc, grid, dim3(256), 0, st, g);
   else
-    hipLaunchKernelGGL((gemm_f16<TA, TB, false>), grid, dim3(256), 0, st, g);
+    hipLaunchKernelGGL((gemm_f16<TA, TB>), grid, dim3(256), 0, st, g);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
