@@ -315,11 +315,11 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
 }
 
 // ---- 4. plan ------------------------------------------------------------------------
-// Pass A: keep bits (m x W words) and per-list query counts.  Block (x, y) = 4 waves over
-// lists 64y.. (lane = list) and queries kPlanCQ x.. (wave w takes queries w, w+4, ...).
-// Per (query, list) pair every load is independent of the others; the per-query values
-// are wave-uniform.
-constexpr uint32_t kPlanQ = 32;   // plan_scatter: queries per block
+// Pass A: keep bits, list-major (R x Wq words, bit b of word (j, t) = query 32 t + b
+// keeps list j), and per-list query counts.  Block (x, y) = 4 waves over lists 64y..
+// (lane = list) and queries kPlanCQ x.. (wave w takes queries 16w.. of them, and waves
+// 2h, 2h+1 make up word h).  Per (query, list) pair every load is independent of the
+// others; the per-query values are wave-uniform.
 constexpr uint32_t kPlanCQ = 64;  // plan_count: queries per block (fewer list_cnt atomics)
 
 __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, uint32_t R,
@@ -328,15 +328,16 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
                                                   const uint32_t *cert, const float *radius,
                                                   const float *rr_den, double e, int metric,
                                                   const uint32_t *planeR, uint32_t P,
-                                                  uint32_t W, uint32_t *bits, uint32_t *list_cnt) {
+                                                  uint32_t Wq, uint32_t *bits, uint32_t *list_cnt) {
+  static_assert(kPlanCQ == 64, "four waves of 16 queries make two 32-query words");
   __shared__ uint32_t red[4][kWave];
   const int l = lane_id(), w = threadIdx.x / kWave;
   const uint32_t jb = blockIdx.y * kWave, j = jb + l;
   const double psi = j < R ? (double)radius[j] : 0.0;
   const double slack = 2.0 * e * psi * psi;
-  uint32_t cnt = 0;
-  for (uint32_t qi = w; qi < kPlanCQ; qi += 4) {
-    const uint32_t q = blockIdx.x * kPlanCQ + qi;
+  uint32_t mask = 0;
+  for (uint32_t qi = 0; qi < 16; ++qi) {
+    const uint32_t q = blockIdx.x * kPlanCQ + 16 * w + qi;
     if (q >= m) break;
     const uint32_t j0 = near[q];
     const uint32_t j1 = key1 ? key1[q] : R;
@@ -405,35 +406,48 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
         }
       }
     }
-    cnt += keep;
-    const uint64_t b = __ballot(keep);
-    if (l == 0) {
-      const uint32_t wd = jb / 32;
-      if (wd < W) bits[size_t(q) * W + wd] = (uint32_t)b;
-      if (wd + 1 < W) bits[size_t(q) * W + wd + 1] = (uint32_t)(b >> 32);
-    }
+    mask |= uint32_t(keep) << qi;
   }
-  red[w][l] = cnt;
+  red[w][l] = mask;
   __syncthreads();
-  if (w == 0) {
-    const uint32_t tot = red[0][l] + red[1][l] + red[2][l] + red[3][l];
-    if (j < R && tot) atomicAdd(&list_cnt[j], tot);
+  if (w == 0 && j < R) {
+    const uint32_t w0 = red[0][l] | (red[1][l] << 16), w1 = red[2][l] | (red[3][l] << 16);
+    const uint32_t t0 = blockIdx.x * 2;
+    if (t0 < Wq) bits[size_t(j) * Wq + t0] = w0;
+    if (t0 + 1 < Wq) bits[size_t(j) * Wq + t0 + 1] = w1;
+    const uint32_t tot = __popc(w0) + __popc(w1);
+    if (tot) atomicAdd(&list_cnt[j], tot);
   }
 }
 
-// Pass C: scatter query ids list-major.  Same block shape as plan_count.
-__global__ __launch_bounds__(256) void plan_scatter(uint32_t m, uint32_t R, uint32_t W,
-                                                    const uint32_t *bits, const uint32_t *list_qoff,
-                                                    uint32_t *cursor, uint32_t *qlist) {
-  const uint32_t q0 = blockIdx.x * kPlanQ;
-  for (uint32_t j = threadIdx.x; j < R; j += 256) {
-    uint32_t cnt = 0;
-    for (uint32_t qi = 0; qi < kPlanQ && q0 + qi < m; ++qi)
-      cnt += (bits[size_t(q0 + qi) * W + j / 32] >> (j % 32)) & 1u;
-    if (!cnt) continue;
-    uint32_t base = list_qoff[j] + atomicAdd(&cursor[j], cnt);
-    for (uint32_t qi = 0; qi < kPlanQ && q0 + qi < m; ++qi)
-      if ((bits[size_t(q0 + qi) * W + j / 32] >> (j % 32)) & 1u) qlist[base++] = q0 + qi;
+// Pass C: scatter query ids list-major.  One wave per (list blockIdx.y, 64 words = 2048
+// queries): lane l reads word w0 + l of the list's row (one 256-byte segment), a wave
+// prefix sum of the popcounts places every lane's ids after those of the lanes before it,
+// and one atomic per wave reserves the wave's range of the list.
+__global__ __launch_bounds__(256) void plan_scatter(uint32_t Wq, const uint32_t *bits,
+                                                    const uint32_t *list_qoff, uint32_t *cursor,
+                                                    uint32_t *qlist) {
+  const uint32_t j = blockIdx.y;
+  const uint32_t w0 = (blockIdx.x * 4 + threadIdx.x / kWave) * kWave;
+  if (w0 >= Wq) return;
+  const int l = lane_id();
+  const uint32_t t = w0 + l;
+  uint32_t word = t < Wq ? bits[size_t(j) * Wq + t] : 0u;
+  const uint32_t c = __popc(word);
+  uint32_t incl = c;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o, kWave);
+    if (l >= o) incl += v;
+  }
+  const uint32_t total = __shfl(incl, kWave - 1, kWave);
+  if (total == 0) return;
+  uint32_t base = 0;
+  if (l == 0) base = list_qoff[j] + atomicAdd(&cursor[j], total);
+  uint32_t pos = __shfl(base, 0, kWave) + incl - c;
+  while (word) {
+    qlist[pos++] = t * 32 + (__ffs(word) - 1);
+    word &= word - 1;
   }
 }
 
@@ -734,8 +748,8 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   }
   tm.mark();
   // 4. plan
-  const uint32_t W = cdiv(R, 32);
-  RBC_TRY(ws.get(30, size_t(m) * W * 4, &p));
+  const uint32_t Wq = cdiv(m, 32);
+  RBC_TRY(ws.get(30, size_t(R) * Wq * 4, &p));
   uint32_t *bits = static_cast<uint32_t *>(p);
   RBC_TRY(ws.get(31, (size_t(R) + 1) * 4, &p));
   uint32_t *lcnt = static_cast<uint32_t *>(p);
@@ -744,7 +758,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
   hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanCQ), cdiv(R, kWave)), dim3(256), 0, st, Dqr, m, R, near,
-                     key1, gam, cert, h->radius.get(), h->rr_den.get(), e, h->metric, planeR, P2, W,
+                     key1, gam, cert, h->radius.get(), h->rr_den.get(), e, h->metric, planeR, P2, Wq,
                      bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));
@@ -754,7 +768,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_HIP_TRY(hipStreamSynchronize(st));
   RBC_TRY(ws.get(34, size_t(std::max(total_pairs, 1u)) * 4, &p));
   uint32_t *qlist = static_cast<uint32_t *>(p);
-  hipLaunchKernelGGL(plan_scatter, dim3(cdiv(m, kPlanQ)), dim3(256), 0, st, m, R, W, bits, lcnt, cursor,
+  hipLaunchKernelGGL(plan_scatter, dim3(cdiv(Wq, 4 * kWave), R), dim3(256), 0, st, Wq, bits, lcnt, cursor,
                      qlist);
   RBC_HIP_TRY(hipGetLastError());
   DevBuf<ScanItem> sitems;
