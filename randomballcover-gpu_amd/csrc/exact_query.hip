@@ -251,10 +251,26 @@ __global__ __launch_bounds__(256) void cand_stats(const uint32_t *cnt, uint32_t 
     sum += __shfl_xor(sum, o, kWave);
     over += (uint32_t)__shfl_xor((int)over, o, kWave);
   }
+  // The block's 4 waves combine in LDS, so that one thread per block touches the three
+  // shared counters.
+  __shared__ unsigned long long part[4][3];
+  const int w = threadIdx.x / kWave;
   if (lane_id() == 0) {
-    atomicMax(&out[0], (unsigned long long)mx);
-    atomicAdd(&out[1], sum);
-    atomicAdd(&out[2], (unsigned long long)over);
+    part[w][0] = mx;
+    part[w][1] = sum;
+    part[w][2] = over;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long bm = 0, bs = 0, bo = 0;
+    for (int i = 0; i < 4; ++i) {
+      bm = max(bm, part[i][0]);
+      bs += part[i][1];
+      bo += part[i][2];
+    }
+    atomicMax(&out[0], bm);
+    atomicAdd(&out[1], bs);
+    atomicAdd(&out[2], bo);
   }
 }
 
@@ -804,8 +820,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     RBC_TRY(ws.get(54, 3 * 8, &p));
     unsigned long long *cs = static_cast<unsigned long long *>(p);
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, cs, 3);
-    hipLaunchKernelGGL(cand_stats, dim3(std::min<uint32_t>(cdiv(m, 256), 1024)), dim3(256), 0, st, cand_n, m, cap,
-                       cs);
+    hipLaunchKernelGGL(cand_stats, dim3(std::min<uint32_t>(cdiv(m, 256), 64)), dim3(256), 0, st, cand_n, m, cap, cs);
     RBC_HIP_TRY(hipGetLastError());
     unsigned long long hcs[3];
     RBC_HIP_TRY(hipMemcpyAsync(hcs, cs, sizeof(hcs), hipMemcpyDeviceToHost, st));

@@ -102,33 +102,43 @@ __global__ __launch_bounds__(kSortNT) void sort_hist(const uint32_t *key, size_t
     hist[size_t(b) * nblocks + blockIdx.x] = cnt[b];
 }
 
+// Stable scatter of one block's kSortItems keys, 256 at a time.  Each lane finds the lanes
+// of its wave holding the same key with one ballot per key bit (kbits bits, plus the
+// valid flag); its rank is the number of such lanes below it.  The block's 4 waves then
+// take positions from the per-bucket counters in wave order, so equal keys keep their
+// input order.
 __global__ __launch_bounds__(kSortNT) void sort_scatter(const uint32_t *key, size_t n, uint32_t nb_buckets,
-                                                        uint32_t nblocks, const uint32_t *hist_off,
-                                                        uint32_t *order) {
-  extern __shared__ uint32_t run[];  // [nb_buckets] running positions, then [kSortNT] keys
-  uint32_t *ks = run + nb_buckets;
+                                                        uint32_t kbits, uint32_t nblocks,
+                                                        const uint32_t *hist_off, uint32_t *order) {
+  static_assert(kSortNT == 4 * 64, "four waves per block");
+  extern __shared__ uint32_t run[];  // [nb_buckets] running positions
   for (uint32_t b = threadIdx.x; b < nb_buckets; b += kSortNT)
     run[b] = hist_off[size_t(b) * nblocks + blockIdx.x];
   __syncthreads();
-  size_t i0 = size_t(blockIdx.x) * kSortItems;
-  const uint32_t t = threadIdx.x;
+  const size_t i0 = size_t(blockIdx.x) * kSortItems;
+  const uint32_t t = threadIdx.x, w = t / 64, l = t % 64;
+  const uint64_t below = l ? (~0ull >> (64 - l)) : 0ull;
   for (uint32_t c = 0; c < kSortItems; c += kSortNT) {
-    size_t i = i0 + c + t;
-    uint32_t k = i < n ? key[i] : 0xFFFFFFFFu;
-    if (k >= nb_buckets) k = 0xFFFFFFFFu;
-    ks[t] = k;
-    __syncthreads();
-    uint32_t rank = 0;
-    bool last = true;
-    if (k != 0xFFFFFFFFu) {
-      for (uint32_t j = 0; j < t; ++j) rank += ks[j] == k;
-      for (uint32_t j = t + 1; j < kSortNT; ++j)
-        if (ks[j] == k) { last = false; break; }
-      order[run[k] + rank] = (uint32_t)i;
+    const size_t i = i0 + c + t;
+    const uint32_t k = i < n ? key[i] : 0xFFFFFFFFu;
+    const bool valid = k < nb_buckets;
+    uint64_t same = __ballot(valid);  // only valid lanes' masks are used
+    for (uint32_t b = 0; b < kbits; ++b) {
+      const bool bit = (k >> b) & 1u;
+      const uint64_t set = __ballot(bit);
+      same &= bit ? set : ~set;
     }
-    __syncthreads();
-    if (k != 0xFFFFFFFFu && last) run[k] += rank + 1;
-    __syncthreads();
+    const uint32_t rank = (uint32_t)__popcll(same & below);
+    const bool leader = valid && (same & below) == 0;
+    for (uint32_t ww = 0; ww < 4; ++ww) {
+      if (w == ww && valid) {
+        const uint32_t base = run[k];
+        order[base + rank] = (uint32_t)i;
+        // Every lane of the wave has read run[k] before its leader's write (in-order LDS).
+        if (leader) run[k] = base + (uint32_t)__popcll(same);
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -203,9 +213,10 @@ int counting_sort(const uint32_t *key, size_t n, uint32_t nbuckets, uint32_t *or
                      n, nbuckets, nblocks, hist);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(hist, hist, hn, ws, slot + 1, st));
-  hipLaunchKernelGGL(sort_scatter, dim3(nblocks), dim3(kSortNT),
-                     (nbuckets + kSortNT) * sizeof(uint32_t), st, key, n, nbuckets, nblocks, hist,
-                     order);
+  uint32_t kbits = 0;
+  while ((1u << kbits) < nbuckets) ++kbits;
+  hipLaunchKernelGGL(sort_scatter, dim3(nblocks), dim3(kSortNT), nbuckets * sizeof(uint32_t), st, key, n,
+                     nbuckets, kbits, nblocks, hist, order);
   RBC_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(gather_offsets, dim3(cdiv(nbuckets + 1, 256)), dim3(256), 0, st, hist, nbuckets,
                      nblocks, offsets);
