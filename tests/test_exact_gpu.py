@@ -110,6 +110,19 @@ def test_exact_bf16_edge_shapes(n, R, m, k):
     check_against_fp64(x, q, ids, ds, k)
 
 
+def test_exact_bf16_very_long_lists():
+    # Two reps over 200,000 points: a list of more than 65,536 points, so the BF16 probe's
+    # keys spend 15 or more bits on the point slot and keep at most 8 of the value's
+    # mantissa bits; the probe then orders its candidates coarsely, and the results must
+    # still be exact.
+    x = datagen.lowrank(200000, 32, 7)
+    q = datagen.lowrank(300, 32, 8)
+    idx = rbc.ExactRBC(x, 2, seed=1)
+    assert idx.info()["max_list"] > 65536
+    ids, ds = idx.kquery(q, 10, precision=rbc.PREC_BF16)
+    check_against_fp64(x, q, ids, ds, 10)
+
+
 def test_exact_bf16_coincident_reps():
     # Rows repeated many times: some reps coincide (zero rep-rep distance, no plane).
     base = datagen.lowrank(300, 128, 3)
