@@ -14,9 +14,9 @@
 // points, 32 per step, through a four-stage LDS ring filled three steps ahead by direct
 // global->LDS loads.  The grid is persistent; the list-major items are split into one
 // contiguous range per XCD (of equal work), so the items of a list run on one XCD at the
-// same time and re-read the list from that XCD's L2.  The probe (top-K' of one list per
-// (query, list) entry) uses 64-entry items on a three-stage register-staged LDS ring with
-// per-lane register top-K'.
+// same time and re-read the list from that XCD's L2.  The probe (close points of one list
+// per (query, list) entry) uses 64-entry items on a three-stage register-staged LDS ring,
+// each lane keeping its best few packed (value, slot) keys in registers.
 #include "exact_query.hpp"
 
 namespace rbc {
