@@ -136,7 +136,8 @@ int rbc_destroy(rbcStruct *rbcs);
 typedef struct rbc_exact rbc_exact;
 
 #define RBC_PREC_FP32 0 /* difference-form FP32 distances on the vector ALUs */
-#define RBC_PREC_BF16 1 /* BF16 MFMA distance filter + exact FP32 re-rank */
+#define RBC_PREC_BF16 1 /* BF16 MFMA distance filter + exact FP32 re-rank (L2, d <= 128,
+                           lists of at most 2^25 points) */
 
 /* Build: rep_ids (num_reps distinct rows) or a seeded sample; every point is owned by
  * its nearest rep (lowest rep index on ties); lists are stored contiguously in
