@@ -547,7 +547,12 @@ struct Bf16ProbeArgs {
 
 template <int KS>
 __global__ __launch_bounds__(256, 2) void bf16_probe_kernel(Bf16ProbeArgs a) {
-  const ScanItem it = a.items[blockIdx.x];
+  // Workgroup b runs on XCD b % kXCD; each XCD takes a contiguous run of the list-major
+  // items, so the items of one list re-read it from the same L2.
+  const uint32_t per_xcd = (a.nitems + kXCD - 1) / kXCD;
+  const uint32_t item = (blockIdx.x % kXCD) * per_xcd + blockIdx.x / kXCD;
+  if (item >= a.nitems) return;
+  const ScanItem it = a.items[item];
   __shared__ uint32_t qrow[kPQ];
   __shared__ __attribute__((aligned(16))) unsigned char ring[kStages][kVP * kRowB];
   __shared__ __attribute__((aligned(16))) float xring[kStages][kVP];  // ||x~||^2 (INF past the end)
@@ -663,7 +668,7 @@ __global__ __launch_bounds__(256, 2) void bf16_probe_kernel(Bf16ProbeArgs a) {
 
 template <int KS>
 void launch_probe(const Bf16ProbeArgs &a, uint32_t nitems, hipStream_t st) {
-  hipLaunchKernelGGL((bf16_probe_kernel<KS>), dim3(nitems), dim3(256), 0, st, a);
+  hipLaunchKernelGGL((bf16_probe_kernel<KS>), dim3(kXCD * ((nitems + kXCD - 1) / kXCD)), dim3(256), 0, st, a);
 }
 
 }  // namespace
