@@ -311,12 +311,19 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
   const uint32_t total = P * Ke;
   const float *vq = pv + size_t(q) * total;
   const uint32_t *iq = pid + size_t(q) * total;
+  // Entries of 16 arrive sorted by (value, slot): the lanes of every other 16-run read
+  // theirs backwards, and the 32- and 64-wide stages finish the sort.  Equal values may
+  // be out of id order, which only changes which of equally close points are re-ranked.
+  const bool runs = Ke == 16;
   for (uint32_t b0 = 0; b0 < total; b0 += kWave) {
-    const uint32_t e = b0 + l;
+    const uint32_t e = runs && (l & 16) ? b0 + (l & ~15) + (15 - (l & 15)) : b0 + l;
     float v = e < total ? vq[e] : FLT_MAX;
     uint32_t id = e < total ? iq[e] : kDummy;
     if (id == kDummy) v = FLT_MAX;
-    wave_bitonic_sort(v, id);
+    if (runs)
+      wave_bitonic_sort_from16(v, id);
+    else
+      wave_bitonic_sort(v, id);
     wave_merge_lower(tv, ti, v, id);
   }
   // Lane l < K' holds the l-th best; the re-rank gathers their rows eight at a time.

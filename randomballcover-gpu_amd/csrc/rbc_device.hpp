@@ -47,6 +47,22 @@ __device__ __forceinline__ void wave_bitonic_sort(float &v, uint32_t &id) {
   }
 }
 
+// Sorts 64 pairs whose aligned 16-lane runs are already ordered, ascending where lane bit 4
+// is clear and descending where it is set (the state wave_bitonic_sort reaches after its
+// 16-wide stages): the remaining 32- and 64-wide stages of that network.
+__device__ __forceinline__ void wave_bitonic_sort_from16(float &v, uint32_t &id) {
+  const int l = lane_id();
+#pragma unroll
+  for (int k = 32; k <= kWave; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      bool up = (l & k) == 0;
+      bool lower = (l & j) == 0;
+      cx_stage(v, id, j, lower == up);
+    }
+  }
+}
+
 // Sorts a bitonic sequence held one per lane into ascending order.
 __device__ __forceinline__ void wave_bitonic_merge(float &v, uint32_t &id) {
   const int l = lane_id();
