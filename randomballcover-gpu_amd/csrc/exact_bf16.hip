@@ -4,19 +4,22 @@
 // v_mfma_f32_16x16x32_bf16 (FP32 accumulation) and emits every point whose TRUE
 // distance could still be within G, the probe's upper bound on the k-th NN distance.
 // With eta = |q - q~| + max_list |x - x~| (FP64, rounded up), |q - x| <= G implies
-// |q~ - x~| <= G + eta, which the kernel tests as q~.x~ + s >= |x~|^2/2 with s folded into
-// the accumulator's initial value and a margin of (gamma_K + 6u') times the absolute
-// terms (u' = 2^-23, so no rounding mode of the accumulator is assumed).  The test is
-// conservative; the exact FP32 re-rank decides.
+// |q~ - x~| <= G + eta, which the kernel tests as -|x~|^2/2 + q~.x~ >= -s: the
+// accumulator starts at -|x~|^2/2 and s carries a margin of (gamma_K + 6u') times the
+// absolute terms (u' = 2^-23, so no rounding mode of the accumulator is assumed)
+// (bf16_threshold).  The test is conservative; the exact FP32 re-rank decides.
 //
-// Scan work item = (one ownership list, up to 256 queries that need it).  The four waves
-// of a workgroup keep 64 queries each as MFMA A operands in VGPRs and share the list's
-// points, 32 per step, through a four-stage LDS ring filled three steps ahead by direct
-// global->LDS loads.  The grid is persistent; the list-major items are split into one
-// contiguous range per XCD (of equal work), so the items of a list run on one XCD at the
-// same time and re-read the list from that XCD's L2.  The probe (close points of one list
-// per (query, list) entry) uses 64-entry items on a three-stage register-staged LDS ring,
-// each lane keeping its best few packed (value, slot) keys in registers.
+// Scan work item = (one ownership list, up to 512 queries that need it).  The four waves
+// of a workgroup keep 128 queries each as MFMA B operands in VGPRs (two waves per SIMD)
+// and share the list's points, the A operands, 32 per step, through a four-stage LDS
+// ring filled three steps ahead by direct global->LDS loads.  Each ring read feeds 8 MFMAs,
+// and each lane's accumulators hold four points of ONE query per query block, so the
+// per-query threshold is one register per block.  The grid is persistent; the list-major
+// items are split into one contiguous range per XCD (of equal work), so the items of a
+// list run on one XCD at the same time and re-read the list from that XCD's L2.  The probe
+// (close points of one list per (query, list) entry) uses 64-entry items on a three-stage
+// register-staged LDS ring, each lane keeping its best few packed (value, slot) keys in
+// registers.
 #include "exact_query.hpp"
 
 namespace rbc {
@@ -88,19 +91,26 @@ __device__ __forceinline__ bf16x8 load8(const uint16_t *p) {
 }
 
 // ---- BF16 list scan ------------------------------------------------------------------
-// Work item = (one list, up to 256 queries).  Wave w keeps queries 64w..64w+63 as MFMA A
-// operands in VGPRs; the 32 points of a step are shared by all four waves through a
-// four-stage LDS ring that is filled by direct global->LDS loads (global_load_lds_dwordx4)
+// Work item = (one list, up to 512 queries).  Wave w keeps queries 128w..128w+127 as MFMA
+// B operands in VGPRs (8 blocks of 16); the 32 points of a step, the A operands (two
+// blocks of 16), are shared by all four waves through a four-stage LDS ring that is filled by direct global->LDS loads (global_load_lds_dwordx4)
 // three steps ahead, so no VGPRs hold data in flight.  Each wave waits for its own loads of
 // the step (s_waitcnt vmcnt with the count of younger loads) before the step's barrier.
 // A stage holds 32 rows of 256 bytes; the 16-byte chunk c of row r sits at chunk c ^ (r%16),
-// which keeps the 64-lane reads (16 rows x 4 chunks) free of bank conflicts.  The LDS DMA
+// which keeps the 64-lane reads (16 rows x 4 chunks) free of bank conflicts.  A ring read
+// (16 points x 32 coordinates) feeds 8 MFMAs, one per query block: twice as many as with 64
+// queries per wave, which halves the LDS reads, ring loads and barriers per FLOP.  The
+// output tile of a (point block, query block) MFMA gives lane l query l%16 and points
+// 4 (l/16) .. +3, so the accumulators start at those points' -|x~|^2/2 (one 16-byte LDS
+// read of the ring's norms) and the per-query threshold is one register.  The LDS DMA
 // is issued from asm: a compiler-visible one would make the compiler drain it before
 // every LDS read.
 // Candidates go to an LDS list first (an LDS atomic does not wait for the loads in flight)
 // and are appended to the per-query buffers after the last step; entries beyond the LDS
 // list's capacity are appended directly.
-constexpr int kVQ = kBf16ScanQ;  // queries per item
+constexpr int kQB = 8;           // 16-query blocks per wave (128 queries)
+constexpr int kVQ = 64 * kQB;    // queries per item (4 waves)
+static_assert(kVQ == (int)kBf16ScanQ, "scan items are cut at kBf16ScanQ queries");
 constexpr int kVP = 32;          // points per step
 constexpr int kSStages = 4;      // scan ring stages (prefetch distance 3)
 constexpr int kSStageB = kVP * 256;
@@ -178,7 +188,7 @@ __global__ __launch_bounds__(1024) void scan_split(const uint32_t *qoff, const u
       const uint32_t cnt = qoff[b + 1] - qoff[b];
       ni = (cnt + kVQ - 1) / kVQ;
       steps = (len[b] + kVP - 1) / kVP;
-      wt = (unsigned long long)steps * ((cnt + kWave - 1) / kWave + ni);
+      wt = (unsigned long long)steps * ((cnt + 16 * kQB - 1) / (16 * kQB) + ni);
     }
   };
   // Total work.
@@ -263,7 +273,7 @@ __global__ __launch_bounds__(1024) void scan_split(const uint32_t *qoff, const u
 }
 
 template <int KS>
-__global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
+__global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
   // Emission test per (query i, point x): MFMA(s_i + q~.x~) >= |x~|^2 / 2 with s_i the
   // threshold of the (query, list) pair (bf16_threshold), so the test never drops an x with
   // |q - x| <= G_i.
@@ -273,7 +283,7 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   __shared__ __attribute__((aligned(16))) float nring[kSStages][4][kWave];  // ||x~||^2, per wave copy
   __shared__ uint32_t cs_p[kCS];  // candidate: point index in the list
   __shared__ float cs_v[kCS];     // candidate: approximate squared distance minus |q~|^2
-  __shared__ uint8_t cs_q[kCS];   // candidate: query slot in the item
+  __shared__ uint16_t cs_q[kCS];   // candidate: query slot in the item
   __shared__ uint32_t cs_n;
   __shared__ uint32_t s_item;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
@@ -336,46 +346,58 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
   if (nsteps > 0) issue(0);
   if (nsteps > 1) issue(1);
   if (nsteps > 2) issue(2);
-  // Thread t = 64 w + l owns query slot t, so a wave already holds the rows of its own 64
-  // queries: the A operands are fetched through lane shuffles, in parallel with the
-  // threshold inputs, without waiting for the LDS copy of the rows.
-  const bool live = (uint32_t)t < it.q_count;
-  const uint32_t myq = live ? a.qids[it.q_begin + t] : kDummy;
-  const bool wave_live = (uint32_t)(w * 64) < it.q_count;
-  bf16x8 A[4][KS];
+  // Wave w owns query slots 128 w .. 128 w + 127 and lane l fetches the ids of slots
+  // 128 w + 64 h + l (h = 0, 1), so a wave already holds the rows of its own queries: the
+  // MFMA operands are fetched through lane shuffles, in parallel with the threshold
+  // inputs, without waiting for an LDS copy of the ids.
+  uint32_t myq[kQB / 4];
+#pragma unroll
+  for (int h = 0; h < kQB / 4; ++h) {
+    const uint32_t slot = 16 * kQB * w + 64 * h + l;
+    myq[h] = slot < it.q_count ? a.qids[it.q_begin + slot] : kDummy;
+  }
+  const bool wave_live = (uint32_t)(w * 16 * kQB) < it.q_count;
+  bf16x8 A[kQB][KS];
   const uint32_t koff = (l / 16) * 8;
 #pragma unroll
-  for (int qb = 0; qb < 4; ++qb) {
-    const uint32_t row = (uint32_t)__shfl((int)myq, qb * 16 + (l % 16), kWave);
+  for (int qb = 0; qb < kQB; ++qb) {
+    const uint32_t row = (uint32_t)__shfl((int)myq[qb / 4], (qb % 4) * 16 + (l % 16), kWave);
 #pragma unroll
     for (int s = 0; s < KS; ++s)
       A[qb][s] = row != kDummy ? load8(a.qb + size_t(row) * a.ldb + s * 32 + koff) : bf16x8{};
   }
-  qrow[t] = myq;
-  sh[t] = live ? bf16_threshold(a.gam[myq], a.qe[myq], a.qn[myq], a.lqe[it.aux], a.lnm[it.aux], a.c_gemm)
-               : -INFINITY;
+#pragma unroll
+  for (int h = 0; h < kQB / 4; ++h) {
+    const uint32_t slot = 16 * kQB * w + 64 * h + l;
+    qrow[slot] = myq[h];
+    sh[slot] = myq[h] != kDummy ? bf16_threshold(a.gam[myq[h]], a.qe[myq[h]], a.qn[myq[h]], a.lqe[it.aux],
+                                                  a.lnm[it.aux], a.c_gemm)
+                                : -INFINITY;
+  }
   if (t == 0) cs_n = 0;
   __syncthreads();
-  // The initial accumulators.
-  f32x4 S[4];
+  // Lane l's query of block qb is slot 128 w + 16 qb + l%16; it emits x when the
+  // accumulator reaches -s.  The pre-filter compares float bit patterns as signed integers,
+  // which orders them like the floats for a positive threshold; a threshold <= 0 lets
+  // every value through to the exact float test.
+  int thrI[kQB];
 #pragma unroll
-  for (int qb = 0; qb < 4; ++qb) {
-    const float4 s4 = *reinterpret_cast<const float4 *>(&sh[w * 64 + qb * 16 + (l / 16) * 4]);
-    S[qb] = f32x4{s4.x, s4.y, s4.z, s4.w};
+  for (int qb = 0; qb < kQB; ++qb) {
+    const float tq = -sh[w * 16 * kQB + qb * 16 + (l % 16)];
+    thrI[qb] = tq > 0.f ? __float_as_int(tq) : INT_MIN;
   }
-  // Per-lane read addresses (stage 0): B fragment (pb, ks) is row pb*16 + l%16, chunk
-  // l/16 + 4 ks; chunks at or past 4 KS are read but never used.
-  uint32_t ad0[8];
+  // Per-lane read addresses (stage 0): point fragment (pb, ks) is row pb*16 + l%16, chunk
+  // l/16 + 4 ks; rows 16..31 sit 4096 bytes after rows 0..15 with the same swizzle (r%16 is
+  // the same), so four addresses serve both point blocks.  Chunks at or past 4 KS are read
+  // but never used.
+  uint32_t ad0[4];
 #pragma unroll
-  for (int pb = 0; pb < 2; ++pb)
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const uint32_t r = pb * 16 + l % 16;
-      const uint32_t c = ((uint32_t)(l / 16) + 4 * ks) ^ (r & 15);
-      ad0[pb * 4 + ks] = lds_addr(ring) + r * 256 + c * 16;
-    }
-  const uint32_t ah0 = lds_addr(&nring[0][w][0]) + (l % 16) * 4;
-  // The A operands must have arrived before the first MFMA; waiting here (for everything,
+  for (int ks = 0; ks < 4; ++ks) {
+    const uint32_t r = l % 16;
+    const uint32_t c = ((uint32_t)(l / 16) + 4 * ks) ^ r;
+    ad0[ks] = lds_addr(ring) + r * 256 + c * 16;
+  }
+  // The query operands must have arrived before the first MFMA; waiting here (for everything,
   // the ring's first loads included) keeps the compiler from placing that wait inside the
   // loop, where it would also drain the ring loads in flight.
   wait_vm<0>();
@@ -391,78 +413,64 @@ __global__ __launch_bounds__(256, 3) void bf16_scan_kernel(Bf16ScanArgs a) {
     asm volatile("s_barrier" ::: "memory");
     if (s + 3 < nsteps) issue(s + 3);
     if (!wave_live) continue;
-    const uint32_t so = (s % kSStages) * kSStageB, ho = (s % kSStages) * (4 * kWave * 4);
-    uint32_t ad[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) ad[i] = ad0[i] + so;
-    i32x4 braw[8];
-    float n0, n1;
+    const uint32_t so = (s % kSStages) * kSStageB;
     // Plain LDS reads: the compiler does not see the ring's LDS DMA (it is issued from
     // asm), so it neither drains those loads before the reads nor moves the reads above
     // the barrier (whose asm clobbers memory); it waits for the reads with lgkmcnt.
-#pragma unroll
-    for (int i = 0; i < 8; ++i) braw[i] = *reinterpret_cast<const lds_i32x4 *>((uintptr_t)ad[i]);
-    n0 = *reinterpret_cast<const lds_float *>((uintptr_t)(ah0 + ho));
-    n1 = *reinterpret_cast<const lds_float *>((uintptr_t)(ah0 + ho + 64));
-    float hx[2];
-    hx[0] = s * kVP + (l % 16) < np ? 0.5f * n0 : INFINITY;
-    hx[1] = s * kVP + 16 + (l % 16) < np ? 0.5f * n1 : INFINITY;
-    f32x4 acc[4][2];
-    // While a wave issues its MFMAs it goes first: the SIMD's other waves fill the gaps
-    // between its MFMAs instead of delaying them.
-    __builtin_amdgcn_s_setprio(1);
+    // Rows past the list end hold copies of its last row; the exact test drops them.
 #pragma unroll
     for (int pb = 0; pb < 2; ++pb) {
-      bf16x8 B[KS];
+      i32x4 braw[4];
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) B[ks] = __builtin_bit_cast(bf16x8, braw[pb * 4 + ks]);
+      for (int i = 0; i < 4; ++i)
+        braw[i] = *reinterpret_cast<const lds_i32x4 *>((uintptr_t)(ad0[i] + so + pb * 4096));
+      // -|x~|^2/2 of the chunk's rows (l/16)*4 .. +3 as the initial accumulator.
+      const float4 n4 = *reinterpret_cast<const float4 *>(&nring[s % kSStages][w][pb * 16 + (l / 16) * 4]);
+      const f32x4 init = f32x4{-0.5f * n4.x, -0.5f * n4.y, -0.5f * n4.z, -0.5f * n4.w};
+      f32x4 acc[kQB];
+      // While a wave issues its MFMAs it goes first: the SIMD's other wave fills the gaps
+      // between its MFMAs instead of delaying them.
+      __builtin_amdgcn_s_setprio(1);
+      {
+        bf16x8 B[KS];
 #pragma unroll
-      for (int qb = 0; qb < 4; ++qb) acc[qb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][0], B[0], S[qb], 0, 0, 0);
+        for (int ks = 0; ks < KS; ++ks) B[ks] = __builtin_bit_cast(bf16x8, braw[ks]);
 #pragma unroll
-      for (int ks = 1; ks < KS; ++ks)
+        for (int qb = 0; qb < kQB; ++qb) acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[0], A[qb][0], init, 0, 0, 0);
 #pragma unroll
-        for (int qb = 0; qb < 4; ++qb)
-          acc[qb][pb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[qb][ks], B[ks], acc[qb][pb], 0, 0, 0);
-    }
-    __builtin_amdgcn_s_setprio(0);
-    // Pre-filter on the float bit patterns as signed integers: for a threshold h > 0 that
-    // order agrees with the float order (negative values and negative NaNs compare below,
-    // positive NaNs above -- the exact test below rejects them); h == 0 (a zero-norm
-    // point) always passes the pre-filter.  One flag per group of 4 accumulators (one
-    // query block, one point block), so that a step with a hit only tests the groups that
-    // have one.
-    bool gh[2][4];
-    bool hit = false;
+        for (int ks = 1; ks < KS; ++ks)
 #pragma unroll
-    for (int pb = 0; pb < 2; ++pb) {
-      const int thr = hx[pb] > 0.f ? __float_as_int(hx[pb]) : INT_MIN;
-#pragma unroll
-      for (int qb = 0; qb < 4; ++qb) {
-        const int mx = max(max3i(__float_as_int(acc[qb][pb][0]), __float_as_int(acc[qb][pb][1]),
-                                 __float_as_int(acc[qb][pb][2])),
-                           __float_as_int(acc[qb][pb][3]));
-        gh[pb][qb] = mx >= thr;
-        hit = hit || gh[pb][qb];
+          for (int qb = 0; qb < kQB; ++qb)
+            acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[ks], A[qb][ks], acc[qb], 0, 0, 0);
       }
-    }
-    if (__any(hit)) {
+      __builtin_amdgcn_s_setprio(0);
+      // One pre-filter flag per query block, so that a chunk with a hit only tests the
+      // blocks that have one.
+      bool g[kQB];
+      bool hit = false;
 #pragma unroll
-      for (int pb = 0; pb < 2; ++pb) {
-        const uint32_t p = s * kVP + pb * 16 + (l % 16);
+      for (int qb = 0; qb < kQB; ++qb) {
+        g[qb] = max(max3i(__float_as_int(acc[qb][0]), __float_as_int(acc[qb][1]), __float_as_int(acc[qb][2])),
+                    __float_as_int(acc[qb][3])) >= thrI[qb];
+        hit = hit || g[qb];
+      }
+      if (__any(hit)) {
+        const uint32_t p0 = s * kVP + pb * 16 + (l / 16) * 4;
 #pragma unroll
-        for (int qb = 0; qb < 4; ++qb) {
-          if (!gh[pb][qb]) continue;
-          const int qi0 = w * 64 + qb * 16 + (l / 16) * 4;
+        for (int qb = 0; qb < kQB; ++qb) {
+          if (!g[qb]) continue;
+          const int qi = w * 16 * kQB + qb * 16 + (l % 16);
+          const float th = -sh[qi];
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            if (acc[qb][pb][r] >= hx[pb] && p < np) {
-              const int qi = qi0 + r;
-              const float v = 2.f * (hx[pb] - (acc[qb][pb][r] - sh[qi]));
+            const uint32_t p = p0 + r;
+            if (acc[qb][r] >= th && p < np) {
+              const float v = -2.f * acc[qb][r];
               const uint32_t slot = atomicAdd(&cs_n, 1u);
               if (slot < (uint32_t)kCS) {
                 cs_p[slot] = p;
                 cs_v[slot] = v;
-                cs_q[slot] = (uint8_t)qi;
+                cs_q[slot] = (uint16_t)qi;
               } else {
                 const uint32_t row = qrow[qi];
                 const uint32_t gs = atomicAdd(&a.cand_n[row], 1u);

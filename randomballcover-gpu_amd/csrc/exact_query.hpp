@@ -42,7 +42,7 @@ constexpr uint32_t kProbeLists = 8;
 // Reps per query whose Voronoi planes bound the lists in the exact plan (<= 64).
 constexpr uint32_t kPlaneReps = 12;
 // Queries per work item of the BF16 scan (its items must be cut at this size).
-constexpr uint32_t kBf16ScanQ = 256;
+constexpr uint32_t kBf16ScanQ = 512;
 // Queries per work item of the BF16 probe.
 constexpr uint32_t kBf16ProbeQ = 64;
 // Candidates the BF16 probe finds per (query, list) entry (4 lanes x 4 kept keys).
@@ -56,12 +56,15 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
               uint32_t *sched, float *cand_d, uint32_t *cand_i, uint32_t *cand_n, uint32_t cap,
               hipStream_t st);
 
-// Emission threshold of the BF16 list scan for one (query, list) pair.  The scan emits x
-// when MFMA(s + q~.x~) >= |x~|^2 / 2 with the accumulator initialised to the returned s =
-// (T - |q~|^2)/2 + margin, T = (G + eta)^2, eta = |q - q~| + max_list |x - x~|: any x with
-// |q - x| <= G has |q~ - x~| <= G + eta, i.e. q~.x~ + (T - |q~|^2)/2 >= |x~|^2/2 in exact
-// arithmetic; the margin covers the FP32 accumulation (c_gemm = gamma_K + 6u' of the
-// absolute terms) and the rounding of the stored norms.  Rounded up to float.
+// Emission threshold of the BF16 list scan for one (query, list) pair.  The scan's
+// accumulator starts at -|x~|^2/2 (the stored FP32 norm halved, exactly) and adds the K
+// products q~_i x~_i; it emits x when the accumulator reaches -s, with the returned s =
+// (T - |q~|^2)/2 + margin rounded up, T = (G + eta)^2, eta = |q - q~| + max_list |x - x~|.
+// Any x with |q - x| <= G has |q~ - x~| <= G + eta, i.e. q~.x~ - |x~|^2/2 >= -(T - |q~|^2)/2
+// in exact arithmetic.  The accumulator's error is at most c_gemm (gamma_K + 6u' of the
+// absolute terms) times |x~|^2/2 + |q~||x~| <= nxm + nq/2, and the stored norm is within
+// u' nxm of |x~|^2 (u' = 2^-23); the margin c_gemm (|s0| + nq + nxm) + u' (nq + nxm) covers
+// both with room to spare.
 __device__ __forceinline__ float bf16_threshold(double G, float qe, float qn, float lqe, float lnm,
                                                 double c_gemm) {
   const double eta = (double)qe + (double)lqe;
@@ -70,7 +73,7 @@ __device__ __forceinline__ float bf16_threshold(double G, float qe, float qn, fl
   const double T0 = (G + eta) * (G + eta);
   const double s0 = 0.5 * (T0 - nq);
   const double up = 0x1p-23;
-  const double margin = c_gemm * (fabs(s0) + 0.5 * (nq + nxm)) + up * (nq + nxm) + 1e-30;
+  const double margin = c_gemm * (fabs(s0) + nq + nxm) + up * (nq + nxm) + 1e-30;
   const double hh = (s0 + margin) * (1.0 + 1e-9);
   float hs = hh >= (double)FLT_MAX ? INFINITY : (float)hh;
   if ((double)hs < hh) hs = nextafterf(hs, INFINITY);
