@@ -507,13 +507,13 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
   }
   if (mtot == 0) return RBC_OK;
   RBC_TRY(b->dinst.ensure(count));
-  RBC_HIP_TRY(hipMemcpyAsync(b->dinst.get(), hi.data(), count * sizeof(BInst), hipMemcpyHostToDevice, st));
+  RBC_TRY(upload_host(b->dinst.get(), hi.data(), count * sizeof(BInst), st));
   b->mtot = mtot;
   RBC_TRY(b->qinst.ensure(mtot));
   RBC_TRY(b->cand_i.ensure(size_t(mtot) * Kp));
   RBC_TRY(b->cand_d.ensure(size_t(mtot) * Kp));
   RBC_TRY(b->cert.ensure(mtot));
-  RBC_HIP_TRY(hipMemcpyAsync(b->qinst.get(), qin.data(), mtot * 4, hipMemcpyHostToDevice, st));
+  RBC_TRY(upload_host(b->qinst.get(), qin.data(), mtot * 4, st));
 
   // Autotune the tile size once per batch shape.
   std::vector<uint32_t> key(mv);

@@ -127,29 +127,33 @@ int rows_out(void *dst, size_t dpitch, const void *src, size_t spitch, size_t wi
   return RBC_OK;
 }
 
+int upload_host(void *dst, const void *src, size_t bytes, hipStream_t st) {
+  if (!bytes) return RBC_OK;
+  RBC_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  return RBC_OK;
+}
+
 int copy_out(void *dst, const void *src, size_t bytes, int mem, hipStream_t st) {
   if (!bytes) return RBC_OK;
   RBC_HIP_TRY(hipMemcpyAsync(dst, src, bytes,
                              mem == RBC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                              st));
+  // The caller reads a host destination as soon as the call returns.
+  if (mem != RBC_MEM_DEVICE) RBC_HIP_TRY(hipStreamSynchronize(st));
   return RBC_OK;
 }
 
 int copy_in(void *dst, const void *src, size_t bytes, int mem, hipStream_t st) {
   if (!bytes) return RBC_OK;
-  RBC_HIP_TRY(hipMemcpyAsync(dst, src, bytes,
-                             mem == RBC_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                             st));
+  if (mem != RBC_MEM_DEVICE) return upload_host(dst, src, bytes, st);
+  RBC_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
   return RBC_OK;
 }
 
 int upload_items(const std::vector<ScanItem> &v, DevBuf<ScanItem> &buf, hipStream_t st) {
   RBC_TRY(buf.ensure(v.size()));
-  if (!v.empty()) {
-    RBC_HIP_TRY(hipMemcpyAsync(buf.get(), v.data(), v.size() * sizeof(ScanItem),
-                               hipMemcpyHostToDevice, st));
-  }
-  return RBC_OK;
+  return upload_host(buf.get(), v.data(), v.size() * sizeof(ScanItem), st);
 }
 
 }  // namespace rbc

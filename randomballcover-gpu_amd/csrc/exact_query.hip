@@ -552,6 +552,44 @@ int launch_select(const SelectArgs &a, int metric, hipStream_t st) {
 
 // ---- host orchestration ---------------------------------------------------------------
 
+// Workspace slots of the query.  Each primitive call gets its own window of kPrimSlots
+// slots from 0: an exclusive scan takes 2 slots per recursion level (at most 3 levels
+// below 2^33 elements), a counting sort or an item generation one more, so a window never
+// reaches the next one, and the query's own buffers start at kQ above all windows.
+constexpr size_t kPrimSlots = 10;
+constexpr size_t kSlotSort = 0, kSlotProbeItems = 10, kSlotPlanScan = 20, kSlotScanItems = 30, kQ = 64;
+static_assert(kPrimSlots >= 1 + 2 * 3 && kSlotScanItems + kPrimSlots <= kQ, "workspace slot windows overlap");
+
+// Creates the side stream and its two events together (all or none).
+int ensure_side_stream(rbc_exact *h) {
+  if (h->side && h->ev_in && h->ev_out) return RBC_OK;
+  hipStream_t s = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&a, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    if (b) (void)hipEventDestroy(b);
+    if (a) (void)hipEventDestroy(a);
+    if (s) (void)hipStreamDestroy(s);
+    set_error("creating the proposal's side stream failed: %s", hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? RBC_ERR_NOMEM : RBC_ERR_HIP;
+  }
+  h->side = s;
+  h->ev_in = a;
+  h->ev_out = b;
+  return RBC_OK;
+}
+
+// Waits for the side stream when the query returns (its work is done by then on success;
+// on an error return this keeps it from outliving the call).
+struct SideStreamGuard {
+  hipStream_t s = nullptr;
+  ~SideStreamGuard() {
+    if (s) (void)hipStreamSynchronize(s);
+  }
+};
+
 struct Timer {
   hipEvent_t ev[8];
   int n = 0;
@@ -591,21 +629,21 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   // stream while the main stream computes the q -> reps distances, and proposal_slot
   // waits for it.
   uint32_t *prop = nullptr;
+  // An early error return must not leave the proposal's inference running on the side
+  // stream over workspace that a later call reuses.
+  SideStreamGuard side_guard;
   if (h->proposal && precision == RBC_PREC_BF16 && k <= 32) {
-    RBC_TRY(ws.get(43, size_t(m) * 4, &p));
+    RBC_TRY(ws.get(kQ + 23, size_t(m) * 4, &p));
     prop = static_cast<uint32_t *>(p);
-    if (!h->side) {
-      RBC_HIP_TRY(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
-      RBC_HIP_TRY(hipEventCreateWithFlags(&h->ev_in, hipEventDisableTiming));
-      RBC_HIP_TRY(hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming));
-    }
+    RBC_TRY(ensure_side_stream(h));
     RBC_HIP_TRY(hipEventRecord(h->ev_in, st));
     RBC_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_in, 0));
+    side_guard.s = h->side;
     RBC_TRY(mlp_propose(h->proposal, qd, ldq, m, prop, h->side));
     RBC_HIP_TRY(hipEventRecord(h->ev_out, h->side));
   }
   // 1. q -> reps
-  RBC_TRY(ws.get(20, size_t(m) * R * 4, &p));
+  RBC_TRY(ws.get(kQ + 0, size_t(m) * R * 4, &p));
   float *Dqr = static_cast<float *>(p);
   {
     const dim3 g(cdiv(m, kQRT), cdiv(R, kQRT));
@@ -633,16 +671,16 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   const uint32_t KpUse = bprobe ? std::min<uint32_t>(32, k + 8) : Kp;
   const uint32_t Pp = bprobe ? std::min<uint32_t>(kProbeLists, R) : 0;
   // 2. row stats
-  RBC_TRY(ws.get(21, size_t(m) * 4, &p));
+  RBC_TRY(ws.get(kQ + 1, size_t(m) * 4, &p));
   uint32_t *near = static_cast<uint32_t *>(p);
-  RBC_TRY(ws.get(22, size_t(m) * 4, &p));
+  RBC_TRY(ws.get(kQ + 2, size_t(m) * 4, &p));
   float *rep_kth = static_cast<float *>(p);
   uint32_t *nearP = nullptr, *planeR = nullptr;
   const uint32_t P2 = bprobe ? std::min<uint32_t>(kPlaneReps, R) : 0;
   if (bprobe) {
-    RBC_TRY(ws.get(51, size_t(m) * Pp * 4, &p));
+    RBC_TRY(ws.get(kQ + 31, size_t(m) * Pp * 4, &p));
     nearP = static_cast<uint32_t *>(p);
-    RBC_TRY(ws.get(56, size_t(m) * P2 * 4, &p));
+    RBC_TRY(ws.get(kQ + 36, size_t(m) * P2 * 4, &p));
     planeR = static_cast<uint32_t *>(p);
   }
   RBC_TRY(fill_f32(rep_kth, FLT_MAX, m, st));
@@ -660,11 +698,11 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint16_t *qb = nullptr;
   float *qn = nullptr, *qe = nullptr;
   if (bf16) {
-    RBC_TRY(ws.get(40, size_t(m) * h->ldb * 2, &p));
+    RBC_TRY(ws.get(kQ + 20, size_t(m) * h->ldb * 2, &p));
     qb = static_cast<uint16_t *>(p);
-    RBC_TRY(ws.get(41, size_t(m) * 4, &p));
+    RBC_TRY(ws.get(kQ + 21, size_t(m) * 4, &p));
     qn = static_cast<float *>(p);
-    RBC_TRY(ws.get(42, size_t(m) * 4, &p));
+    RBC_TRY(ws.get(kQ + 22, size_t(m) * 4, &p));
     qe = static_cast<float *>(p);
     RBC_TRY(bf16_prepare_queries(h, qd, ldq, m, qb, qn, qe, st));
   }
@@ -678,9 +716,9 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     uint32_t *order = static_cast<uint32_t *>(pp);
     RBC_TRY(ws.get(slot_off, (size_t(R) + 1) * 4, &pp));
     uint32_t *off = static_cast<uint32_t *>(pp);
-    RBC_TRY(counting_sort(keys, m, R, order, off, ws, 0, st));
-    RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st,
-                              (uint32_t)kTQ));
+    RBC_TRY(counting_sort(keys, m, R, order, off, ws, kSlotSort, st));
+    RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws,
+                              kSlotProbeItems, st, (uint32_t)kTQ));
     RBC_TRY(ws.get(slot_d, size_t(m) * KpUse * 4, &pp));
     float *pd = static_cast<float *>(pp);
     RBC_TRY(ws.get(slot_i, size_t(m) * KpUse * 4, &pp));
@@ -704,48 +742,49 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
       RBC_HIP_TRY(hipGetLastError());
     }
     const size_t ne = size_t(m) * Pp;
-    RBC_TRY(ws.get(23, ne * 4, &p));
+    RBC_TRY(ws.get(kQ + 3, ne * 4, &p));
     uint32_t *order = static_cast<uint32_t *>(p);
-    RBC_TRY(ws.get(24, (size_t(R) + 1) * 4, &p));
+    RBC_TRY(ws.get(kQ + 4, (size_t(R) + 1) * 4, &p));
     uint32_t *off = static_cast<uint32_t *>(p);
-    RBC_TRY(counting_sort(nearP, ne, R, order, off, ws, 0, st));
-    RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws, 8, st,
+    RBC_TRY(counting_sort(nearP, ne, R, order, off, ws, kSlotSort, st));
+    RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws,
+                              kSlotProbeItems, st,
                               kBf16ProbeQ));
     // Each entry holds at most kBf16ProbeKeep candidates; the merge reads that many.
     const uint32_t Ke = std::min(KpUse, kBf16ProbeKeep);
-    RBC_TRY(ws.get(52, ne * Ke * 4, &p));
+    RBC_TRY(ws.get(kQ + 32, ne * Ke * 4, &p));
     float *ev = static_cast<float *>(p);
-    RBC_TRY(ws.get(53, ne * Ke * 4, &p));
+    RBC_TRY(ws.get(kQ + 33, ne * Ke * 4, &p));
     uint32_t *eid = static_cast<uint32_t *>(p);
     RBC_TRY(fill_f32(ev, FLT_MAX, ne * Ke, st));
     RBC_TRY(fill_u32(eid, kDummy, ne * Ke, st));
     RBC_TRY(bf16_probe(h, qb, qn, order, Pp, pitems.get(), npitems, Ke, eid, ev, st));
-    RBC_TRY(ws.get(25, size_t(m) * KpUse * 4, &p));
+    RBC_TRY(ws.get(kQ + 5, size_t(m) * KpUse * 4, &p));
     probe_d = static_cast<float *>(p);
-    RBC_TRY(ws.get(26, size_t(m) * KpUse * 4, &p));
+    RBC_TRY(ws.get(kQ + 6, size_t(m) * KpUse * 4, &p));
     probe_i = static_cast<uint32_t *>(p);
     RBC_TRY(launch_probe_merge(qd, ldq, h, m, Pp, Ke, KpUse, ev, eid, probe_i, probe_d, st));
   } else {
-    RBC_TRY(run_probe(near, 23, 24, 25, 26, &probe_d, &probe_i));
+    RBC_TRY(run_probe(near, kQ + 3, kQ + 4, kQ + 5, kQ + 6, &probe_d, &probe_i));
   }
   // 3a. FP32 second probe: the proposal network's top-1 list where it differs from the
   // nearest rep's.
   if (h->proposal && !bprobe) {
-    RBC_TRY(ws.get(43, size_t(m) * 4, &p));
+    RBC_TRY(ws.get(kQ + 23, size_t(m) * 4, &p));
     uint32_t *prop = static_cast<uint32_t *>(p);
     RBC_TRY(mlp_propose(h->proposal, qd, ldq, m, prop, st));
-    RBC_TRY(ws.get(44, size_t(m) * 4, &p));
+    RBC_TRY(ws.get(kQ + 24, size_t(m) * 4, &p));
     key1 = static_cast<uint32_t *>(p);
     hipLaunchKernelGGL(second_probe_key, dim3(cdiv(m, 256)), dim3(256), 0, st, prop, near, m, R, key1);
     RBC_HIP_TRY(hipGetLastError());
-    RBC_TRY(run_probe(key1, 45, 46, 47, 48, &probe_d1, &probe_i1));
+    RBC_TRY(run_probe(key1, kQ + 25, kQ + 26, kQ + 27, kQ + 28, &probe_d1, &probe_i1));
   }
   // 3b. bounds
-  RBC_TRY(ws.get(27, size_t(m) * 4, &p));
+  RBC_TRY(ws.get(kQ + 7, size_t(m) * 4, &p));
   float *tau = static_cast<float *>(p);
-  RBC_TRY(ws.get(28, size_t(m) * 8, &p));
+  RBC_TRY(ws.get(kQ + 8, size_t(m) * 8, &p));
   double *gam = static_cast<double *>(p);
-  RBC_TRY(ws.get(29, size_t(m) * 4, &p));
+  RBC_TRY(ws.get(kQ + 9, size_t(m) * 4, &p));
   uint32_t *cert = static_cast<uint32_t *>(p);
   {
     BoundArgs ba{};
@@ -772,11 +811,11 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   tm.mark();
   // 4. plan
   const uint32_t Wq = cdiv(m, 32);
-  RBC_TRY(ws.get(30, size_t(R) * Wq * 4, &p));
+  RBC_TRY(ws.get(kQ + 10, size_t(R) * Wq * 4, &p));
   uint32_t *bits = static_cast<uint32_t *>(p);
-  RBC_TRY(ws.get(31, (size_t(R) + 1) * 4, &p));
+  RBC_TRY(ws.get(kQ + 11, (size_t(R) + 1) * 4, &p));
   uint32_t *lcnt = static_cast<uint32_t *>(p);
-  RBC_TRY(ws.get(32, (size_t(R) + 1) * 4, &p));
+  RBC_TRY(ws.get(kQ + 12, (size_t(R) + 1) * 4, &p));
   uint32_t *cursor = static_cast<uint32_t *>(p);
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
@@ -784,37 +823,37 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
                      key1, gam, cert, h->radius.get(), h->rr_den.get(), e, h->metric, planeR, P2, Wq,
                      bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
-  RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, 12, st));
-  RBC_TRY(ws.get(33, 4, &p));
+  RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, kSlotPlanScan, st));
+  RBC_TRY(ws.get(kQ + 13, 4, &p));
   uint32_t total_pairs = 0;
   RBC_HIP_TRY(hipMemcpyAsync(&total_pairs, lcnt + R, 4, hipMemcpyDeviceToHost, st));
   RBC_HIP_TRY(hipStreamSynchronize(st));
-  RBC_TRY(ws.get(34, size_t(std::max(total_pairs, 1u)) * 4, &p));
+  RBC_TRY(ws.get(kQ + 14, size_t(std::max(total_pairs, 1u)) * 4, &p));
   uint32_t *qlist = static_cast<uint32_t *>(p);
   hipLaunchKernelGGL(plan_scatter, dim3(cdiv(Wq, 4 * kWave), R), dim3(256), 0, st, Wq, bits, lcnt, cursor,
                      qlist);
   RBC_HIP_TRY(hipGetLastError());
   DevBuf<ScanItem> sitems;
   uint32_t nsitems = 0;
-  RBC_TRY(make_bucket_items(lcnt, h->list_off.get(), h->list_len.get(), R, sitems, &nsitems, ws, 16, st,
-                            bf16 ? kBf16ScanQ : (uint32_t)kTQ));
+  RBC_TRY(make_bucket_items(lcnt, h->list_off.get(), h->list_len.get(), R, sitems, &nsitems, ws,
+                            kSlotScanItems, st, bf16 ? kBf16ScanQ : (uint32_t)kTQ));
   tm.mark();
   // 5. scan with candidate emission; re-run with a larger buffer on overflow.
   uint32_t &cap_seen = h->cand_cap[bf16 ? 1 : 0];
   uint32_t cap = std::max<uint32_t>({256u, 4 * k, cap_seen});
-  RBC_TRY(ws.get(36, size_t(m) * 4, &p));
+  RBC_TRY(ws.get(kQ + 16, size_t(m) * 4, &p));
   uint32_t *cand_n = static_cast<uint32_t *>(p);
   float *cand_d = nullptr;
   uint32_t *cand_i = nullptr;
   double fallbacks = 0, rerank_total = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
-    RBC_TRY(ws.get(37, size_t(m) * cap * 4, &p));
+    RBC_TRY(ws.get(kQ + 17, size_t(m) * cap * 4, &p));
     cand_d = static_cast<float *>(p);
-    RBC_TRY(ws.get(38, size_t(m) * cap * 4, &p));
+    RBC_TRY(ws.get(kQ + 18, size_t(m) * cap * 4, &p));
     cand_i = static_cast<uint32_t *>(p);
     RBC_TRY(fill_u32(cand_n, 0, m, st));
     if (precision == RBC_PREC_BF16) {
-      RBC_TRY(ws.get(55, 32 * 4, &p));
+      RBC_TRY(ws.get(kQ + 35, 32 * 4, &p));
       RBC_TRY(bf16_scan(h, qb, qn, qe, gam, qlist, lcnt, sitems.get(), nsitems, static_cast<uint32_t *>(p),
                         cand_d, cand_i, cand_n, cap, st));
     } else {
@@ -824,7 +863,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     }
     // Largest candidate count, total and overflows, reduced on the device (a pageable copy
     // of all m counts would cost more than the reduction).
-    RBC_TRY(ws.get(54, 3 * 8, &p));
+    RBC_TRY(ws.get(kQ + 34, 3 * 8, &p));
     unsigned long long *cs = static_cast<unsigned long long *>(p);
     hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, st, cs, 3);
     hipLaunchKernelGGL(cand_stats, dim3(std::min<uint32_t>(cdiv(m, 256), 64)), dim3(256), 0, st, cand_n, m, cap, cs);
@@ -845,7 +884,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   }
   tm.mark();
   // 6. select
-  RBC_TRY(ws.get(39, 4 * 8, &p));
+  RBC_TRY(ws.get(kQ + 19, 4 * 8, &p));
   unsigned long long *sdev = static_cast<unsigned long long *>(p);
   RBC_HIP_TRY(hipMemsetAsync(sdev, 0, 4 * 8, st));
   SelectArgs sa;
@@ -888,11 +927,11 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     // Pairs evaluated by the probe and the scan, from the per-list query counts.
     std::vector<uint32_t> hq(R + 1), hl(R + 1), hq1(R + 1, 0);
     void *pq0;
-    RBC_TRY(ws.get(24, (size_t(R) + 1) * 4, &pq0));
+    RBC_TRY(ws.get(kQ + 4, (size_t(R) + 1) * 4, &pq0));
     RBC_HIP_TRY(hipMemcpy(hq.data(), pq0, (R + 1) * 4, hipMemcpyDeviceToHost));
     if (h->proposal && !bprobe) {
       void *pq1;
-      RBC_TRY(ws.get(46, (size_t(R) + 1) * 4, &pq1));
+      RBC_TRY(ws.get(kQ + 26, (size_t(R) + 1) * 4, &pq1));
       RBC_HIP_TRY(hipMemcpy(hq1.data(), pq1, (R + 1) * 4, hipMemcpyDeviceToHost));
     }
     RBC_HIP_TRY(hipMemcpy(hl.data(), lcnt, (R + 1) * 4, hipMemcpyDeviceToHost));

@@ -50,9 +50,13 @@ struct DevMat {
   int stage(const float *x, uint32_t rows, uint32_t d, uint32_t ldx, int mem, hipStream_t st);
 };
 
-// Contiguous copies between the device and the caller's buffer (host or device).
+// Contiguous copies between the device and the caller's buffer (host or device).  With a
+// host buffer both return after the copy is done, so the buffer may be reused (copy_in)
+// or read (copy_out) at once; device-to-device copies stay asynchronous on st.
 int copy_out(void *dst, const void *src, size_t bytes, int mem, hipStream_t st);
 int copy_in(void *dst, const void *src, size_t bytes, int mem, hipStream_t st);
+// Host-to-device copy that returns once the host buffer may be modified or freed.
+int upload_host(void *dst, const void *src, size_t bytes, hipStream_t st);
 
 // Row-strided copies: `rows` rows of `width` bytes, source pitch `spitch`, destination
 // pitch `dpitch` (bytes).  The host end of a transfer is always moved as one contiguous

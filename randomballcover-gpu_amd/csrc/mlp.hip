@@ -771,7 +771,10 @@ int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   for (uint32_t ep = 0; ep < epochs; ++ep) {
     std::shuffle(hp.begin(), hp.end(), rng);
-    RBC_HIP_TRY(hipMemcpyAsync(perm.get(), hp.data(), size_t(rows) * 4, hipMemcpyHostToDevice, st));
+    // Synchronous upload: the next epoch's shuffle rewrites hp, so the copy must be done
+    // before this loop moves on (the previous epoch's kernels that read perm are ordered
+    // before it on the stream).
+    RBC_TRY(upload_host(perm.get(), hp.data(), size_t(rows) * 4, st));
     uint32_t nb = 0;
     for (uint32_t r0 = 0; r0 + batch <= rows; r0 += batch, ++nb) {
       const uint32_t B = batch;

@@ -277,8 +277,8 @@ int oneshot_query_common(const float *q, uint32_t m, uint32_t ldq, const rbcStru
   }
   RBC_TRY(xoff.alloc(R));
   RBC_TRY(xlen.alloc(R));
-  RBC_HIP_TRY(hipMemcpyAsync(xoff.get(), hxo.data(), R * 4, hipMemcpyHostToDevice, st));
-  RBC_HIP_TRY(hipMemcpyAsync(xlen.get(), hxl.data(), R * 4, hipMemcpyHostToDevice, st));
+  RBC_TRY(upload_host(xoff.get(), hxo.data(), R * 4, st));
+  RBC_TRY(upload_host(xlen.get(), hxl.data(), R * 4, st));
   DevBuf<ScanItem> items;
   uint32_t nitems = 0;
   RBC_TRY(make_bucket_items(qoff.get(), xoff.get(), xlen.get(), R, items, &nitems, ws, 8, st));
@@ -331,7 +331,7 @@ int rbc_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32_t num
   RBC_TRY(dr.alloc(size_t(pR) * pd));
   RBC_TRY(fill_f32(dr.get(), 0.f, size_t(pR) * pd, st));
   RBC_TRY(drep.alloc(R));
-  RBC_HIP_TRY(hipMemcpyAsync(drep.get(), reps.data(), R * 4, hipMemcpyHostToDevice, st));
+  RBC_TRY(upload_host(drep.get(), reps.data(), R * 4, st));
   for (uint32_t r = 0; r < R; ++r)
     RBC_HIP_TRY(hipMemcpyAsync(dr.get() + size_t(r) * pd, dx.get() + size_t(reps[r]) * pd, size_t(pd) * 4,
                                hipMemcpyDeviceToDevice, st));
