@@ -68,6 +68,22 @@ int rbc_brute_k(const float *x, uint32_t n, uint32_t ldx, const float *q, uint32
                 uint32_t ldq, uint32_t d, int metric, uint32_t k, uint32_t *nn_ids,
                 float *nn_dists, uint32_t ldo, int mem, void *stream);
 
+/* Allocation-free variant of rbc_brute_k, with no counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc/hipFree and no allocating
+ * routine: its work items and per-segment partial lists live in a fixed 64 MiB device
+ * array that every GPU the library runs on holds until the process ends.  x, q, nn_ids and
+ * nn_dists must be device pointers; ldx and ldq multiples of 4.  Results equal
+ * rbc_brute_k's.  Sizes whose partial lists cannot fit even at one segment (m * k above
+ * about 8 million) are rejected with RBC_ERR_ARG; split the queries.  The call is
+ * asynchronous: it returns once its work is queued on `stream`.  All calls on one GPU
+ * share that GPU's array, so calls that overlap in time there (on different streams or
+ * threads) race; calls queued on one stream are ordered.  No scratch word needs to start at zero: each is
+ * written before it is read within the call. */
+int rbc_brute_k_noalloc(const float *x, uint32_t n, uint32_t ldx, const float *q, uint32_t m,
+                        uint32_t ldq, uint32_t d, int metric, uint32_t k, uint32_t *nn_ids,
+                        float *nn_dists, uint32_t ldo, void *stream);
+
 /* bruteSearch (brute.cu:43-62 -> nnWrap -> nnKernel kernels.cu:165-222): 1-NN ids and
  * distances; ties go to the lowest index.  nn_dists may be NULL. */
 int rbc_brute_search(const float *x, uint32_t n, uint32_t ldx, const float *q, uint32_t m,

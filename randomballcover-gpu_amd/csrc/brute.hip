@@ -4,6 +4,7 @@
 // fill the device even for a few queries; per-segment results are merged by one wave
 // per query.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "launch.hpp"
@@ -71,6 +72,52 @@ int check_common(const float *x, uint32_t n, uint32_t ldx, const float *q, uint3
   RBC_ARG_CHECK((x || n == 0) && (q || m == 0), "NULL matrix");
   return RBC_OK;
 }
+
+// Work items of a brute-force launch on the device: item s * qtiles + t covers queries
+// [64 t, 64 t + 64) against segment s (the order grid_items produces on the host).
+__global__ void grid_items_k(uint32_t m, uint32_t n, uint32_t seg_len, uint32_t nseg, uint32_t qtiles,
+                             ScanItem *items) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg * qtiles) return;
+  const uint32_t s = i / qtiles, q0 = (i % qtiles) * kTQ;
+  ScanItem it;
+  it.x_begin = s * seg_len;
+  it.x_count = n ? min(seg_len, n - s * seg_len) : 0;
+  it.q_begin = q0;
+  it.q_count = min((uint32_t)kTQ, m - q0);
+  it.aux = s;
+  items[i] = it;
+}
+
+// ---- allocation-free brute force --------------------------------------------------------
+// Scratch of rbc_brute_k_noalloc: the work items and the per-segment partial top-k lists.
+// Each device holds its own copy of this array from module load until the process ends.
+// Host code never passes the array's name to a kernel (that is a host-side stand-in, not a
+// device address): it uses the address hipGetSymbolAddress returns for the current device.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of the current device's scratch array, looked up once per device.
+int scratch_address(void **out) {
+  int dev = 0;
+  RBC_HIP_TRY(hipGetDevice(&dev));
+  RBC_ARG_CHECK(dev >= 0 && dev < kMaxDevices, "device %d is beyond the %d the library tracks", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *p = nullptr;
+    RBC_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    RBC_ARG_CHECK(p != nullptr, "scratch array has no device address");
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return RBC_OK;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace
 
@@ -213,6 +260,61 @@ int rbc_brute_range_count(const float *x, uint32_t n, uint32_t ldx, const float 
   RBC_TRY(launch_tiles_m(metric, src, items.get(), (uint32_t)iv.size(), epi, st));
   RBC_TRY(copy_out(counts, cnt.get(), size_t(m) * 4, mem, st));
   RBC_HIP_TRY(hipStreamSynchronize(st));
+  return RBC_OK;
+}
+
+int rbc_brute_k_noalloc(const float *x, uint32_t n, uint32_t ldx, const float *q, uint32_t m, uint32_t ldq,
+                        uint32_t d, int metric, uint32_t k, uint32_t *nn_ids, float *nn_dists, uint32_t ldo,
+                        void *stream) {
+  RBC_TRY(check_common(x, n, ldx, q, m, ldq, d, metric));
+  RBC_ARG_CHECK(k >= 1 && k <= KMAX, "k=%u outside [1, %d]", k, KMAX);
+  RBC_ARG_CHECK(ldo >= k, "output leading dimension %u < k=%u", ldo, k);
+  RBC_ARG_CHECK(nn_ids && nn_dists, "NULL output");
+  // The tile engine reads rows with 16-byte loads; the allocating entry point re-pitches
+  // other layouts into a copy, which this one cannot make.
+  RBC_ARG_CHECK(ldx % 4 == 0 && ldq % 4 == 0, "leading dimensions must be multiples of 4 (ldx=%u, ldq=%u)",
+                ldx, ldq);
+  if (m == 0) return RBC_OK;
+  if (n) RBC_TRY(check_device_ptr(x, "x"));
+  RBC_TRY(check_device_ptr(q, "q"));
+  RBC_TRY(check_device_ptr(nn_ids, "nn_ids"));
+  RBC_TRY(check_device_ptr(nn_dists, "nn_dists"));
+  // Segments as the allocating entry point cuts them, fewer if their partial lists would
+  // not fit the scratch array.
+  const uint32_t qtiles = cdiv(m, kTQ);
+  const size_t part = size_t(m) * k * 4;  // one segment's partial distances (ids: the same)
+  uint32_t seg = seg_len_for(m, n);
+  uint32_t nseg = n ? cdiv(n, seg) : 1;
+  auto need = [&](uint32_t ns) { return align256(size_t(ns) * qtiles * sizeof(ScanItem)) + 2 * align256(ns * part); };
+  if (need(nseg) > kScratchBytes) {
+    const size_t fit = kScratchBytes / (2 * part + 512);
+    RBC_ARG_CHECK(fit >= 1 && need(1) <= kScratchBytes,
+                  "m=%u queries at k=%u need %zu bytes of scratch; the allocation-free entry point has %zu "
+                  "(split the queries)",
+                  m, k, need(1), kScratchBytes);
+    nseg = (uint32_t)std::min<size_t>(nseg, fit);
+    while (nseg > 1 && need(nseg) > kScratchBytes) --nseg;
+    seg = std::max((uint32_t)kTP, cdiv(cdiv(n, nseg), kTP) * kTP);
+    nseg = n ? cdiv(n, seg) : 1;
+  }
+  void *base = nullptr;
+  RBC_TRY(scratch_address(&base));
+  unsigned char *sp = static_cast<unsigned char *>(base);
+  ScanItem *items = reinterpret_cast<ScanItem *>(sp);
+  float *pd = reinterpret_cast<float *>(sp + align256(size_t(nseg) * qtiles * sizeof(ScanItem)));
+  uint32_t *pi = reinterpret_cast<uint32_t *>(reinterpret_cast<unsigned char *>(pd) + align256(nseg * part));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const uint32_t nitems = nseg * qtiles;
+  hipLaunchKernelGGL(grid_items_k, dim3(cdiv(nitems, 256)), dim3(256), 0, st, m, n, seg, nseg, qtiles, items);
+  RBC_HIP_TRY(hipGetLastError());
+  // Every slot of the partial lists is written by the tile epilogue before the merge reads
+  // it, so nothing in the scratch array has to start at zero.
+  TileSrc src{x, ldx, nullptr, nullptr, q, ldq, nullptr, d, (d + 3) & ~3u};
+  EpiTopK epi{k, pd, pi, k, m};
+  RBC_TRY(launch_tiles_m(metric, src, items, nitems, epi, st));
+  hipLaunchKernelGGL(merge_segments, dim3(cdiv(m, 4)), dim3(256), 0, st, pd, pi, m, k, nseg, nn_dists, nn_ids,
+                     ldo);
+  RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
 

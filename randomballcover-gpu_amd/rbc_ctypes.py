@@ -73,6 +73,8 @@ _SIGS = {
     "rbc_measure_peaks": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _vp]),
     "rbc_brute_k": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _u32, _vp, _vp, _u32, _int,
                            _vp]),
+    "rbc_brute_k_noalloc": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _u32, _vp, _vp, _u32,
+                                    _vp]),
     "rbc_brute_search": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _vp, _vp, _int, _vp]),
     "rbc_brute_range_count": (_int, [_vp, _u32, _u32, _vp, _u32, _u32, _u32, _int, _vp, _vp, _int,
                                      _vp]),
@@ -170,6 +172,15 @@ def bruteK(x, q, k=KMAX, metric=L2):
     _check(lib().rbc_brute_k(xp, x.shape[0], x.shape[1], qp, m, q.shape[1], d, metric, k,
                              ids.ctypes.data, ds.ctypes.data, k, MEM_HOST, None))
     return ids, ds
+
+
+def bruteK_noalloc(x, q, k, ids, ds, metric=L2, stream=None):
+    """rbc_brute_k_noalloc on device tensors (x [n,d], q [m,d] float32, row strides multiples of
+    4; ids [m,>=k] int32/uint32 and ds [m,>=k] float32 outputs): the allocation-free bruteK.
+    Asynchronous on `stream` (a raw hipStream_t or None for the default stream)."""
+    m, d = q.shape
+    _check(lib().rbc_brute_k_noalloc(_ptr(x), x.shape[0], x.stride(0), _ptr(q), m, q.stride(0), d, metric,
+                                     k, _ptr(ids), _ptr(ds), ids.stride(0), stream))
 
 
 def bruteSearch(x, q, metric=L2):
