@@ -208,9 +208,11 @@ int rbc_exact_load(const char *path, rbc_exact **out);
 /* ------------------------------------------------------- learned list proposal -- */
 
 /* A small MLP d_in -> h1 -> h2 -> n_out (GELU, FP16 weights and activations, FP32
- * accumulation) trained on the GPU by the library's own kernels.  For an index it maps
- * a query to the ownership list most likely to hold its nearest neighbour (n_out =
- * num_reps).  No reference counterpart. */
+ * accumulation) trained on the GPU by the library's own kernels.  For an index (n_out =
+ * num_reps) it scores the ownership lists by how likely they hold a query's nearest
+ * neighbours; trained on the lists that hold the exact k NNs of training queries (soft
+ * targets), its ordering decides which lists the exact query probes first.  No reference
+ * counterpart. */
 typedef struct rbc_mlp rbc_mlp;
 
 int rbc_mlp_create(uint32_t d_in, uint32_t h1, uint32_t h2, uint32_t n_out, uint64_t seed,
@@ -228,6 +230,12 @@ int rbc_mlp_load(const char *path, rbc_mlp **out);
  * parameters (host, same layout as the parameters); no update. */
 int rbc_mlp_loss_grad(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx,
                       const uint32_t *labels, float *loss, float *grads);
+/* The same with labels_per_row (1..64) labels per row, labels[rows][labels_per_row]: row
+ * i's target is uniform over its labels below n_out (a label >= n_out is no label; a row
+ * without any contributes nothing).  labels_per_row = 1 is rbc_mlp_loss_grad. */
+int rbc_mlp_loss_grad_multi(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx,
+                            const uint32_t *labels, uint32_t labels_per_row, float *loss,
+                            float *grads);
 /* Logits (host, rows x n_out) of host inputs. */
 int rbc_mlp_logits(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, float *logits);
 /* The proposal of each host input row: the index of its largest logit, lowest index
@@ -239,9 +247,27 @@ int rbc_mlp_propose(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, uin
 int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels,
                   int mem, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
                   float *epoch_loss);
-/* Use (or, with NULL, stop using) the network's top-1 list as a second first-probe of
- * the exact k-query.  The network is not owned by the index. */
+/* Training on soft targets (labels[rows][labels_per_row], as rbc_mlp_loss_grad_multi),
+ * otherwise as rbc_mlp_train; labels_per_row = 1 is rbc_mlp_train. */
+int rbc_mlp_train_multi(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx,
+                        const uint32_t *labels, uint32_t labels_per_row, int mem, uint32_t epochs,
+                        uint32_t batch, float lr, uint64_t seed, float *epoch_loss);
+/* The P (1..16) largest logits of each host input row, descending (lowest index among
+ * equal logits), as lists[rows][P], and in counts[rows] how many of them it takes for
+ * their softmax probabilities to sum to at least `mass` (0 if all P fall short) -- the
+ * probe set the exact query takes from the network. */
+int rbc_mlp_propose_lists(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, uint32_t P,
+                          float mass, uint32_t *lists, uint32_t *counts);
+/* Use (or, with NULL, stop using) the network to choose the lists the exact k-query probes
+ * first.  BF16 filter: a query probes the network's best lists, as many as it takes for
+ * their softmax mass to reach the proposal mass (default 0.9), at most 8; a query whose
+ * 8 best fall short probes the lists of its 8 nearest representatives, as without the
+ * network.  FP32 filter: the network's top-1 list is a second probe.  Results do not
+ * depend on it (the probe only sets the pruning bound).  The network is not owned by the
+ * index. */
 int rbc_exact_set_proposal(rbc_exact *h, rbc_mlp *mlp);
+/* The proposal mass of the BF16 probe set, in (0, 1]. */
+int rbc_exact_set_proposal_mass(rbc_exact *h, float mass);
 
 /* ------------------------------------------------------------ batched exact search -- */
 

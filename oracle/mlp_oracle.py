@@ -1,7 +1,8 @@
 """FP64 CPU oracle of the proposal network -- TEST INFRASTRUCTURE ONLY.
 
 Forward and backward of d -> H1 -> H2 -> C with GELU (erf form) and mean softmax
-cross-entropy, in float64 numpy.  With round16=True the inputs, weights and the two
+cross-entropy against soft targets (uniform over each row's labels; one label = one-hot),
+in float64 numpy, and the list proposal taken from the logits (propose_lists).  With round16=True the inputs, weights and the two
 hidden activations are rounded to FP16 first, as the GPU kernels store them, so the
 remaining difference to the GPU is FP16 rounding of the back-propagated terms and FP32
 accumulation.  Parameter layout: W1[H1][D], b1, W2[H2][H1], b2, W3[C][H2], b3
@@ -52,7 +53,21 @@ def forward(params, x, dims, round16=True):
     return {"x": x, "z1": z1, "h1": h1, "z2": z2, "h2": h2, "z3": z3, "W": (W1, W2, W3)}
 
 
+def targets(labels, B, C):
+    """Soft targets: row i uniform over its labels below C (labels [B] or [B, L]; a label
+    >= C is no label, a row without any has an all-zero target)."""
+    lab = np.asarray(labels, np.int64).reshape(B, -1)
+    t = np.zeros((B, C))
+    for i in range(B):
+        v = [int(a) for a in lab[i] if 0 <= a < C]
+        for a in v:
+            t[i, a] += 1.0 / len(v)
+    return t
+
+
 def loss_grad(params, x, labels, dims, round16=True):
+    """Mean soft-target cross-entropy sum_j -t_ij log p_ij over the B rows and its gradient
+    (labels [B] is the one-hot case)."""
     D, H1, H2, C = dims
     f = forward(params, x, dims, round16)
     B = x.shape[0]
@@ -61,9 +76,11 @@ def loss_grad(params, x, labels, dims, round16=True):
     e = np.exp(z3 - mx)
     se = e.sum(1, keepdims=True)
     p = e / se
-    loss = float(np.mean(np.log(se[:, 0]) + mx[:, 0] - z3[np.arange(B), labels]))
-    dz3 = p.copy()
-    dz3[np.arange(B), labels] -= 1.0
+    t = targets(labels, B, C)
+    has = t.sum(1) > 0
+    logp = z3 - mx - np.log(se)
+    loss = float(-(t * logp).sum() / B)
+    dz3 = np.where(has[:, None], p - t, 0.0)
     W1, W2, W3 = f["W"]
     gW3 = dz3.T @ f["h2"] / B
     gb3 = dz3.sum(0) / B
@@ -91,3 +108,18 @@ def finite_difference(params, x, labels, dims, idx, h=1e-6):
         lm, _ = loss_grad(pm, x, labels, dims, round16=False)
         out.append((lp - lm) / (2 * h))
     return np.array(out)
+
+
+def propose_lists(logits, P, mass):
+    """The P largest logits of each row (descending, lowest index among equal values) and
+    how many of them it takes for their softmax probabilities to sum to at least `mass`
+    (0 if all P fall short), in float64."""
+    z = np.asarray(logits, np.float64)
+    B, C = z.shape
+    order = np.lexsort((np.tile(np.arange(C), (B, 1)), -z), axis=1)[:, :P]
+    p = np.exp(z - z.max(1, keepdims=True))
+    p /= p.sum(1, keepdims=True)
+    cum = np.cumsum(np.take_along_axis(p, order, 1), 1)
+    reach = cum >= mass
+    counts = np.where(reach.any(1), reach.argmax(1) + 1, 0)
+    return order.astype(np.uint32), counts.astype(np.uint32), cum

@@ -37,8 +37,12 @@ struct rbc_exact {
   // Candidate-buffer capacity per query and distance filter (FP32, BF16), grown to 1.25x
   // the largest count seen, so that repeated queries on one index do not re-run the scan.
   uint32_t cand_cap[2] = {0, 0};
-  // Learned list proposal (not owned); NULL = probe only the nearest rep's list.
+  // Learned list proposal (not owned); NULL = the geometric probe set only.  BF16 filter:
+  // a query probes the network's top lists until their softmax mass reaches prop_mass
+  // (at most kProbeLists of them); a query whose top lists fall short keeps the
+  // geometric set.  FP32 filter: the network's top-1 list is a second probe.
   struct rbc_mlp *proposal = nullptr;
+  float prop_mass = 0.9f;
   // Second stream (default priority) on which the proposal's inference runs beside the
   // q -> reps distances, and the events that order it; created on first use.
   hipStream_t side = nullptr;
@@ -72,6 +76,10 @@ inline uint32_t tree_width(uint32_t d) {
 
 // Top-1 list of the proposal network for each query row.
 int mlp_propose(struct rbc_mlp *m, const float *q, size_t ldq, uint32_t rows, uint32_t *out, hipStream_t st);
+// The P (<= 16) best lists of each query row by logit, descending, into lists[rows][P], and
+// in count[rows] how many of them first reach softmax mass tau (0 if all P fall short).
+int mlp_propose_lists(struct rbc_mlp *m, const float *q, size_t ldq, uint32_t rows, uint32_t P, float tau,
+                      uint32_t *lists, uint32_t *count, hipStream_t st);
 int mlp_dims(const struct rbc_mlp *m, uint32_t *d_in, uint32_t *n_out);
 
 int nn_device(const float *x, size_t ldx, uint32_t n, const uint32_t *xid, const float *q,

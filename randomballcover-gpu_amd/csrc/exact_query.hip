@@ -274,17 +274,19 @@ __global__ __launch_bounds__(256) void cand_stats(const uint32_t *cnt, uint32_t 
   }
 }
 
-// Multi-list probe: the proposal network's list takes the last of a query's P probe slots
-// unless one of the other slots already holds it.
-__global__ void proposal_slot(const uint32_t *prop, uint32_t m, uint32_t P, uint32_t R, uint32_t *keys) {
-  uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+// Multi-list probe: a query whose proposal reached the mass threshold within its P lists
+// (count c > 0) probes the network's c best lists instead of its geometric set; the other
+// slots get key R (no list, dropped by the counting sort).
+__global__ void proposal_lists(const uint32_t *plists, const uint32_t *pcnt, uint32_t m, uint32_t P, uint32_t R,
+                               uint32_t *keys) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m) return;
-  const uint32_t p = prop[q];
-  if (p >= R) return;
-  uint32_t *kq = keys + size_t(q) * P;
-  for (uint32_t s = 0; s + 1 < P; ++s)
-    if (kq[s] == p) return;
-  kq[P - 1] = p;
+  const uint32_t c = pcnt[q];
+  if (c == 0) return;
+  for (uint32_t s = 0; s < P; ++s) {
+    const uint32_t j = plists[size_t(q) * P + s];
+    keys[size_t(q) * P + s] = s < c && j < R ? j : R;
+  }
 }
 
 // Multi-list probe, per query (one wave): the P lists' Ke best approximate pairs each are
@@ -628,18 +630,21 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   // 2'. The proposal network's inference needs only the queries: it runs on the side
   // stream while the main stream computes the q -> reps distances, and proposal_slot
   // waits for it.
-  uint32_t *prop = nullptr;
+  uint32_t *plists = nullptr, *pcnt = nullptr;
   // An early error return must not leave the proposal's inference running on the side
   // stream over workspace that a later call reuses.
   SideStreamGuard side_guard;
+  const uint32_t Pprop = std::min<uint32_t>(kProbeLists, R);
   if (h->proposal && precision == RBC_PREC_BF16 && k <= 32) {
-    RBC_TRY(ws.get(kQ + 23, size_t(m) * 4, &p));
-    prop = static_cast<uint32_t *>(p);
+    RBC_TRY(ws.get(kQ + 23, size_t(m) * Pprop * 4, &p));
+    plists = static_cast<uint32_t *>(p);
+    RBC_TRY(ws.get(kQ + 37, size_t(m) * 4, &p));
+    pcnt = static_cast<uint32_t *>(p);
     RBC_TRY(ensure_side_stream(h));
     RBC_HIP_TRY(hipEventRecord(h->ev_in, st));
     RBC_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_in, 0));
     side_guard.s = h->side;
-    RBC_TRY(mlp_propose(h->proposal, qd, ldq, m, prop, h->side));
+    RBC_TRY(mlp_propose_lists(h->proposal, qd, ldq, m, Pprop, h->prop_mass, plists, pcnt, h->side));
     RBC_HIP_TRY(hipEventRecord(h->ev_out, h->side));
   }
   // 1. q -> reps
@@ -736,9 +741,9 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   uint32_t *probe_i = nullptr, *probe_i1 = nullptr, *key1 = nullptr;
   if (bprobe) {
     // Entries e = q * Pp + s (query q, probe slot s), keyed by list, sorted list-major.
-    if (prop) {
+    if (plists) {
       RBC_HIP_TRY(hipStreamWaitEvent(st, h->ev_out, 0));
-      hipLaunchKernelGGL(proposal_slot, dim3(cdiv(m, 256)), dim3(256), 0, st, prop, m, Pp, R, nearP);
+      hipLaunchKernelGGL(proposal_lists, dim3(cdiv(m, 256)), dim3(256), 0, st, plists, pcnt, m, Pp, R, nearP);
       RBC_HIP_TRY(hipGetLastError());
     }
     const size_t ne = size_t(m) * Pp;
@@ -968,6 +973,13 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
 }  // namespace rbc
 
 using namespace rbc;
+
+extern "C" int rbc_exact_set_proposal_mass(rbc_exact *h, float mass) {
+  RBC_ARG_CHECK(h, "NULL handle");
+  RBC_ARG_CHECK(mass > 0.f && mass <= 1.f, "proposal mass %g outside (0, 1]", (double)mass);
+  h->prop_mass = mass;
+  return RBC_OK;
+}
 
 extern "C" int rbc_exact_set_proposal(rbc_exact *h, rbc_mlp *mlp) {
   RBC_ARG_CHECK(h, "NULL handle");

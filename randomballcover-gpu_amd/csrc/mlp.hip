@@ -250,6 +250,187 @@ __global__ __launch_bounds__(256) void gemm_f16_argmax(GemmArgs g) {
     }
 }
 
+// Last layer with the top-P lists fused (inference): for every row i, the Pr <= P largest
+// values of (alpha A W^T + b)(i, :) (descending, lowest column among equal values; NaN
+// counts as -inf) go to lists[i*Pr ..], and count[i] gets how many of them it takes for
+// their softmax probabilities to reach tau (0 if all Pr fall short), so the logits never
+// leave the chip.  As gemm_f16_argmax, a block owns 64 rows and walks all N columns in
+// 64-column tiles with the tile of W (and its biases) in LDS, but with the MFMA operands
+// swapped: lane l's accumulators hold 4 columns of ONE row (row l%16 of the query block), so
+// each lane keeps, per query block, its P best (value, column) sorted in registers and an
+// online softmax normaliser (running max, sum of exponentials).  The 8 partial lists of a
+// row (4 lanes x 2 waves) are merged by one thread per row through LDS.
+template <int KS, int P>
+__global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float tau, uint32_t *lists,
+                                                     uint32_t *count) {
+  constexpr int K = 32 * KS, CPR = K / 8, PITCH = CPR + 1;
+  __shared__ f16x8 Ws[kGT * PITCH];
+  __shared__ float bs[kGT];
+  __shared__ float mv[kGT][8][P];
+  __shared__ int mi[kGT][8][P];
+  __shared__ float pm[kGT][8], ps[kGT][8];
+  const int t = threadIdx.x, w = t / 64, l = t % 64;
+  const int i0 = blockIdx.x * kGT, rb = (w / 2) * 32, cb = (w % 2) * 32;
+  const _Float16 *A = static_cast<const _Float16 *>(g.A);
+  const _Float16 *W = static_cast<const _Float16 *>(g.B);  // column j of B = row j of W
+  const float alpha = g.alpha != 0.f ? g.alpha : 1.f;
+  f16x8 af[2][KS];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi) {
+    const int i = i0 + rb + bi * 16 + l % 16;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+      af[bi][ks] = i < g.M ? *reinterpret_cast<const f16x8 *>(A + int64_t(i) * g.a_rs + ks * 32 + (l / 16) * 8)
+                           : f16x8{};
+  }
+  float tv[2][P], mx[2], se[2];
+  int ti[2][P];
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi) {
+    mx[bi] = -INFINITY;
+    se[bi] = 0.f;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      tv[bi][s] = -INFINITY;
+      ti[bi][s] = 0x7FFFFFFF;
+    }
+  }
+  // Tiles of W are fetched two tiles ahead into two register sets that alternate (the tile
+  // loop is written out twice per iteration, so each set keeps fixed registers).
+  f16x8 wr0[KS], wr1[KS];
+  float br0 = 0.f, br1 = 0.f;
+  auto wload = [&](int j0, f16x8 (&wr)[KS], float &br) {
+#pragma unroll
+    for (int e = 0; e < KS; ++e) {
+      const int c = t + 256 * e, j = j0 + c / CPR;
+      wr[e] = j < g.N ? *reinterpret_cast<const f16x8 *>(W + int64_t(j) * g.b_cs + (c % CPR) * 8) : f16x8{};
+    }
+    if (t < kGT) br = (j0 + t < g.N && g.bias) ? g.bias[j0 + t] : 0.f;
+  };
+  auto tile = [&](int j0, f16x8 (&wr)[KS], float &br) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < KS; ++e) {
+      const int c = t + 256 * e;
+      Ws[(c / CPR) * PITCH + c % CPR] = wr[e];
+    }
+    if (t < kGT) bs[t] = br;
+    __syncthreads();
+    if (j0 + 2 * kGT < g.N) wload(j0 + 2 * kGT, wr, br);
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj) {
+        const f16x8 wv = Ws[(cb + bj * 16 + l % 16) * PITCH + ks * 4 + l / 16];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+          acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wv, af[bi][ks], acc[bi][bj], 0, 0, 0);
+      }
+    // Lane l holds row rb + bi*16 + l%16, columns j0 + cb + bj*16 + 4(l/16) + r.
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      const int jc = cb + bj * 16 + (l / 16) * 4;
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) {
+        float v[4];
+        float m4 = -INFINITY;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          v[r] = j0 + jc + r < g.N ? alpha * acc[bi][bj][r] + bs[jc + r] : -INFINITY;
+          if (v[r] != v[r]) v[r] = -INFINITY;
+          m4 = fmaxf(m4, v[r]);
+        }
+        if (m4 > mx[bi]) {
+          se[bi] = mx[bi] == -INFINITY ? 0.f : se[bi] * __expf(mx[bi] - m4);
+          mx[bi] = m4;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (v[r] == -INFINITY) continue;
+          se[bi] += __expf(v[r] - mx[bi]);
+          // Branch-free insertion from the bottom: slot s takes slot s-1's entry when v
+          // belongs above it, v itself when it belongs exactly there, else keeps its own.
+          // A lane sees its columns in ascending order, so an equal value goes after.
+          if (v[r] > tv[bi][P - 1]) {
+            const int j = j0 + jc + r;
+#pragma unroll
+            for (int s = P - 1; s > 0; --s) {
+              const bool above = v[r] > tv[bi][s - 1];
+              const bool here = !above && v[r] > tv[bi][s];
+              const float nv = above ? tv[bi][s - 1] : (here ? v[r] : tv[bi][s]);
+              const int ni = above ? ti[bi][s - 1] : (here ? j : ti[bi][s]);
+              tv[bi][s] = nv;
+              ti[bi][s] = ni;
+            }
+            if (v[r] > tv[bi][0]) {
+              tv[bi][0] = v[r];
+              ti[bi][0] = j;
+            }
+          }
+        }
+      }
+    }
+  };
+  wload(0, wr0, br0);
+  if (kGT < g.N) wload(kGT, wr1, br1);
+  for (int j0 = 0; j0 < g.N; j0 += 2 * kGT) {
+    tile(j0, wr0, br0);
+    if (j0 + kGT < g.N) tile(j0 + kGT, wr1, br1);
+  }
+  // Partial p = 4 (w % 2) + l/16 of row rb + bi*16 + l%16.
+  const int part = (w % 2) * 4 + l / 16;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi) {
+    const int rr = rb + bi * 16 + l % 16;
+#pragma unroll
+    for (int s = 0; s < P; ++s) {
+      mv[rr][part][s] = tv[bi][s];
+      mi[rr][part][s] = ti[bi][s];
+    }
+    pm[rr][part] = mx[bi];
+    ps[rr][part] = se[bi];
+  }
+  __syncthreads();
+  if (t >= kGT || i0 + t >= g.M) return;
+  const int rr = t, i = i0 + t;
+  float M = -INFINITY;
+  for (int q = 0; q < 8; ++q) M = fmaxf(M, pm[rr][q]);
+  float S = 0.f;
+  for (int q = 0; q < 8; ++q)
+    if (pm[rr][q] != -INFINITY) S += ps[rr][q] * __expf(pm[rr][q] - M);
+  int head[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  float mass = 0.f;
+  uint32_t cnt = 0;
+  for (int r = 0; r < Pr; ++r) {
+    int bq = -1;
+    float bv = -INFINITY;
+    int bidx = 0x7FFFFFFF;
+    for (int q = 0; q < 8; ++q) {
+      if (head[q] >= P) continue;
+      const float v = mv[rr][q][head[q]];
+      const int j = mi[rr][q][head[q]];
+      if (j == 0x7FFFFFFF) continue;
+      if (bq < 0 || v > bv || (v == bv && j < bidx)) {
+        bq = q;
+        bv = v;
+        bidx = j;
+      }
+    }
+    if (bq >= 0) ++head[bq];
+    lists[int64_t(i) * Pr + r] = bq >= 0 ? (uint32_t)bidx : 0xFFFFFFFFu;
+    if (cnt == 0 && bq >= 0 && S > 0.f) {
+      mass += __expf(bv - M) / S;
+      if (mass >= tau) cnt = r + 1;
+    }
+  }
+  count[i] = cnt;
+}
+
 // General operands (any element strides, FP32 or FP16): 32-wide k chunks staged element by
 // element.
 template <typename TA, typename TB>
@@ -313,28 +494,43 @@ int gemm(const GemmArgs &g, hipStream_t st) {
   return RBC_OK;
 }
 
-// Softmax cross-entropy over rows of logits [B][N]: dZ = softmax - onehot as FP16 (the
-// 1/B of the mean is applied to the weight gradients in FP32, so that FP16 does not have
-// to hold values of order 1/(B N)); the mean loss accumulates into *loss.  One wave per
-// row.
-__global__ __launch_bounds__(256) void softmax_ce(const float *Z, const uint32_t *labels, int B, int N,
+// Softmax cross-entropy of rows of logits [B][N] against soft targets: row i's target is
+// uniform over its valid labels labels[i*L .. i*L+L) (a label >= N is no label; with L = 1
+// this is the one-hot cross-entropy).  dZ = softmax - target as FP16 (the 1/B of the mean
+// is applied to the weight gradients in FP32, so that FP16 does not have to hold values of
+// order 1/(B N)); the mean loss sum_l (logsumexp - z_label_l) / L_valid over the B rows
+// accumulates into *loss.  A row without a valid label contributes neither.  One wave per
+// row; the target of wave w is built in LDS (N floats per wave).
+__global__ __launch_bounds__(256) void softmax_ce(const float *Z, const uint32_t *labels, int L, int B, int N,
                                                   _Float16 *dZ, float *loss) {
-  const int row = blockIdx.x * 4 + threadIdx.x / 64, l = threadIdx.x % 64;
+  extern __shared__ float tgt_all[];
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  const int row = blockIdx.x * (blockDim.x / 64) + w;
   if (row >= B) return;
+  float *tgt = tgt_all + size_t(w) * N;
   const float *z = Z + int64_t(row) * N;
+  for (int j = l; j < N; j += 64) tgt[j] = 0.f;
+  uint32_t lab = l < L ? labels[int64_t(row) * L + l] : 0xFFFFFFFFu;
+  const bool valid = lab < (uint32_t)N;
+  const int nv = __popcll(__ballot(valid));
   float mx = -INFINITY;
   for (int j = l; j < N; j += 64) mx = fmaxf(mx, z[j]);
   for (int s = 32; s > 0; s >>= 1) mx = fmaxf(mx, __shfl_xor(mx, s, 64));
   float se = 0.f;
   for (int j = l; j < N; j += 64) se += __expf(z[j] - mx);
   for (int s = 32; s > 0; s >>= 1) se += __shfl_xor(se, s, 64);
-  const uint32_t lab = labels[row];
-  const float inv = 1.f / (float)B;
+  const float share = nv ? 1.f / (float)nv : 0.f;
+  __builtin_amdgcn_wave_barrier();
+  if (valid) atomicAdd(&tgt[lab], share);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   for (int j = l; j < N; j += 64) {
-    float p = __expf(z[j] - mx) / se;
-    dZ[int64_t(row) * N + j] = (_Float16)(p - (j == (int)lab ? 1.f : 0.f));
+    const float p = __expf(z[j] - mx) / se;
+    dZ[int64_t(row) * N + j] = (_Float16)(nv ? p - tgt[j] : 0.f);
   }
-  if (l == 0 && loss) atomicAdd(loss, (logf(se) + mx - z[lab]) * inv);
+  float lv = valid ? (logf(se) + mx - z[lab]) * share : 0.f;
+  for (int s = 32; s > 0; s >>= 1) lv += __shfl_xor(lv, s, 64);
+  if (l == 0 && loss && nv) atomicAdd(loss, lv / (float)B);
 }
 
 // dZ = dH * gelu'(Z), FP16 out.
@@ -396,9 +592,10 @@ __global__ void argmax_decode(const unsigned long long *key, uint32_t B, uint32_
   if (i < B) out[i] = key[i] ? 0xFFFFFFFFu - (uint32_t)key[i] : 0xFFFFFFFFu;
 }
 
-__global__ void gather_u32(const uint32_t *src, const uint32_t *idx, int B, uint32_t *out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < B) out[i] = src[idx[i]];
+// out[i][0..L) = src[idx[i]][0..L).
+__global__ void gather_u32(const uint32_t *src, const uint32_t *idx, int B, int L, uint32_t *out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < B * L) out[e] = src[int64_t(idx[e / L]) * L + e % L];
 }
 
 }  // namespace
@@ -420,6 +617,25 @@ struct rbc_mlp {
   rbc::DevBuf<float> z1, z2, z3, dh2, dh1, loss;
   rbc::DevBuf<uint32_t> lab, idx;
   rbc::DevBuf<unsigned long long> amax;  // inference: per-row argmax keys
+  uint32_t labcap = 0;                   // label entries m->lab holds (rows x labels per row)
+  // Inference activations (FP16 hidden layers only), for up to icap rows.
+  uint32_t icap = 0;
+  rbc::DevBuf<_Float16> ih1, ih2;
+
+  int ensure_infer(uint32_t B) {
+    if (B <= icap) return RBC_OK;
+    RBC_TRY(ih1.alloc(size_t(B) * H1));
+    RBC_TRY(ih2.alloc(size_t(B) * H2));
+    icap = B;
+    return RBC_OK;
+  }
+
+  int ensure_labels(size_t n) {
+    if (n <= labcap) return RBC_OK;
+    RBC_TRY(lab.alloc(n));
+    labcap = (uint32_t)n;
+    return RBC_OK;
+  }
 
   int ensure(uint32_t B) {
     if (B <= cap) return RBC_OK;
@@ -435,7 +651,7 @@ struct rbc_mlp {
     RBC_TRY(z3.alloc(size_t(B) * C));
     RBC_TRY(dh2.alloc(size_t(B) * H2));
     RBC_TRY(dh1.alloc(size_t(B) * H1));
-    RBC_TRY(lab.alloc(B));
+    RBC_TRY(ensure_labels(B));
     RBC_TRY(idx.alloc(B));
     RBC_TRY(amax.alloc(B));
     RBC_TRY(loss.alloc(1));
@@ -475,12 +691,25 @@ int mlp_forward(rbc_mlp *m, uint32_t B, float *logits, hipStream_t st, unsigned 
   return RBC_OK;
 }
 
-// Backward pass for the batch (labels in m->lab); gradients to m->g.
-int mlp_backward(rbc_mlp *m, uint32_t B, hipStream_t st) {
-  RBC_TRY(fill_f32(m->loss.get(), 0.f, 1, st));
-  hipLaunchKernelGGL(softmax_ce, dim3(cdiv(B, 4)), dim3(256), 0, st, m->z3.get(), m->lab.get(), (int)B,
-                     (int)m->C, m->dz3.get(), m->loss.get());
+// Soft-target cross-entropy and its logit gradient (L labels per row in m->lab): waves per
+// block as the LDS for the per-wave targets (N floats each) allows.
+int launch_softmax_ce(rbc_mlp *m, uint32_t B, uint32_t L, hipStream_t st) {
+  const size_t per_wave = size_t(m->C) * sizeof(float);
+  const uint32_t waves = (uint32_t)std::min<size_t>(4, std::max<size_t>(1, 65536 / per_wave));
+  if (waves * per_wave > 65536) {
+    set_error("training supports at most 16384 outputs (n_out=%u)", m->C);
+    return RBC_ERR_LIMIT;
+  }
+  hipLaunchKernelGGL(softmax_ce, dim3(cdiv(B, waves)), dim3(64 * waves), waves * per_wave, st, m->z3.get(),
+                     m->lab.get(), (int)L, (int)B, (int)m->C, m->dz3.get(), m->loss.get());
   RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+// Backward pass for the batch (L labels per row in m->lab); gradients to m->g.
+int mlp_backward(rbc_mlp *m, uint32_t B, uint32_t L, hipStream_t st) {
+  RBC_TRY(fill_f32(m->loss.get(), 0.f, 1, st));
+  RBC_TRY(launch_softmax_ce(m, B, L, st));
   const _Float16 *dz[3] = {m->dz1.get(), m->dz2.get(), m->dz3.get()};
   const _Float16 *in[3] = {m->x16.get(), m->h1.get(), m->h2.get()};
   const uint32_t outd[3] = {m->H1, m->H2, m->C}, ind[3] = {m->D, m->H1, m->H2};
@@ -532,6 +761,67 @@ int mlp_propose(rbc_mlp *m, const float *q, size_t ldq, uint32_t rows, uint32_t 
     RBC_TRY(fill_u32(reinterpret_cast<uint32_t *>(m->amax.get()), 0u, 2 * size_t(B), st));
     RBC_TRY(mlp_forward(m, B, nullptr, st, m->amax.get()));
     hipLaunchKernelGGL(argmax_decode, dim3(cdiv(B, 256)), dim3(256), 0, st, m->amax.get(), B, out + r0);
+    RBC_HIP_TRY(hipGetLastError());
+  }
+  return RBC_OK;
+}
+
+// The P best lists of each row by logit and how many of them reach softmax mass tau
+// (0 = not within P), into lists[rows][P] and count[rows].  Stream-ordered.  Inference
+// keeps only what it needs: layer 1 reads the FP32 queries directly, the hidden layers
+// store only their FP16 activations, and the last layer's logits stay on the chip (the
+// top-P and the softmax mass are fused into its GEMM).
+int mlp_propose_lists(rbc_mlp *m, const float *q, size_t ldq, uint32_t rows, uint32_t P, float tau,
+                      uint32_t *lists, uint32_t *count, hipStream_t st) {
+  if (P < 1 || P > 16) {
+    set_error("a list proposal holds 1 to 16 lists (P=%u)", P);
+    return RBC_ERR_ARG;
+  }
+  if (m->H2 % 32 != 0 || m->H2 > 256 || m->H1 % 8 != 0) {
+    set_error("list proposals need hidden widths with H1 %% 8 == 0 and H2 %% 32 == 0, H2 <= 256 (%u, %u)",
+              m->H1, m->H2);
+    return RBC_ERR_LIMIT;
+  }
+  // One pass over up to 2^20 rows at a time (FP16 activations only: 1 KB a row at 256 + 256).
+  const uint32_t chunk = 1u << 20;
+  RBC_TRY(m->ensure_infer(std::min(rows, chunk)));
+  for (uint32_t r0 = 0; r0 < rows; r0 += chunk) {
+    const uint32_t B = std::min(chunk, rows - r0);
+    GemmArgs g{};
+    g.A = q + size_t(r0) * ldq; g.a_rs = (int64_t)ldq; g.a_cs = 1;
+    g.B = m->W16(0); g.b_rs = 1; g.b_cs = m->D;
+    g.H = m->ih1.get(); g.ldh = m->H1; g.bias = m->Bv(0);
+    g.M = B; g.N = m->H1; g.K = m->D; g.gelu = 1;
+    RBC_TRY((gemm<float, _Float16>(g, st)));
+    g = GemmArgs{};
+    g.A = m->ih1.get(); g.a_rs = m->H1; g.a_cs = 1;
+    g.B = m->W16(1); g.b_rs = 1; g.b_cs = m->H1;
+    g.H = m->ih2.get(); g.ldh = m->H2; g.bias = m->Bv(1);
+    g.M = B; g.N = m->H2; g.K = m->H1; g.gelu = 1;
+    RBC_TRY((gemm<_Float16, _Float16>(g, st)));
+    g = GemmArgs{};
+    g.A = m->ih2.get(); g.a_rs = m->H2; g.a_cs = 1;
+    g.B = m->W16(2); g.b_rs = 1; g.b_cs = m->H2;
+    g.bias = m->Bv(2);
+    g.M = B; g.N = m->C; g.K = m->H2;
+    uint32_t *lo = lists + size_t(r0) * P, *co = count + r0;
+    const dim3 grid(cdiv(B, kGT)), blk(256);
+    const int KS = (int)m->H2 / 32;
+#define RBC_TOPP(ks, pp) hipLaunchKernelGGL((gemm_f16_topp<ks, pp>), grid, blk, 0, st, g, (int)P, tau, lo, co)
+#define RBC_TOPP_P(ks) \
+  if (P <= 4) RBC_TOPP(ks, 4); else if (P <= 8) RBC_TOPP(ks, 8); else RBC_TOPP(ks, 16)
+    switch (KS) {
+      case 1: RBC_TOPP_P(1); break;
+      case 2: RBC_TOPP_P(2); break;
+      case 3: RBC_TOPP_P(3); break;
+      case 4: RBC_TOPP_P(4); break;
+      case 5: RBC_TOPP_P(5); break;
+      case 6: RBC_TOPP_P(6); break;
+      case 7: RBC_TOPP_P(7); break;
+      default: RBC_TOPP_P(8); break;
+    }
+#undef RBC_TOPP_P
+#undef RBC_TOPP
     RBC_HIP_TRY(hipGetLastError());
   }
   return RBC_OK;
@@ -697,16 +987,19 @@ int rbc_mlp_set_params(rbc_mlp *m, const float *params) {
   return RBC_OK;
 }
 
-int rbc_mlp_loss_grad(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels,
-                      float *loss, float *grads) {
+int rbc_mlp_loss_grad_multi(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels,
+                            uint32_t labels_per_row, float *loss, float *grads) {
   RBC_ARG_CHECK(m && x && labels && rows, "bad argument");
   RBC_ARG_CHECK(ldx >= m->D, "ldx < input width");
+  RBC_ARG_CHECK(labels_per_row >= 1 && labels_per_row <= 64, "labels per row %u outside [1, 64]",
+                labels_per_row);
   hipStream_t st = nullptr;
   RBC_TRY(m->ensure(rows));
+  RBC_TRY(m->ensure_labels(size_t(rows) * labels_per_row));
   DevBuf<float> xd;
   RBC_TRY(xd.alloc(size_t(rows) * ldx));
   RBC_HIP_TRY(hipMemcpy(xd.get(), x, size_t(rows) * ldx * 4, hipMemcpyHostToDevice));
-  RBC_HIP_TRY(hipMemcpy(m->lab.get(), labels, size_t(rows) * 4, hipMemcpyHostToDevice));
+  RBC_HIP_TRY(hipMemcpy(m->lab.get(), labels, size_t(rows) * labels_per_row * 4, hipMemcpyHostToDevice));
   std::vector<uint32_t> hi(rows);
   for (uint32_t i = 0; i < rows; ++i) hi[i] = i;
   RBC_HIP_TRY(hipMemcpy(m->idx.get(), hi.data(), size_t(rows) * 4, hipMemcpyHostToDevice));
@@ -714,11 +1007,16 @@ int rbc_mlp_loss_grad(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, c
   hipLaunchKernelGGL(gather_half, dim3((uint32_t)cdivz(ne, 256)), dim3(256), 0, st, xd.get(), (int64_t)ldx,
                      m->idx.get(), (int)rows, (int)m->D, m->x16.get());
   RBC_TRY(mlp_forward(m, rows, nullptr, st));
-  RBC_TRY(mlp_backward(m, rows, st));
+  RBC_TRY(mlp_backward(m, rows, labels_per_row, st));
   RBC_HIP_TRY(hipDeviceSynchronize());
   if (loss) RBC_HIP_TRY(hipMemcpy(loss, m->loss.get(), 4, hipMemcpyDeviceToHost));
   if (grads) RBC_HIP_TRY(hipMemcpy(grads, m->g.get(), m->np * 4, hipMemcpyDeviceToHost));
   return RBC_OK;
+}
+
+int rbc_mlp_loss_grad(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels,
+                      float *loss, float *grads) {
+  return rbc_mlp_loss_grad_multi(m, x, rows, ldx, labels, 1, loss, grads);
 }
 
 int rbc_mlp_logits(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, float *logits) {
@@ -753,18 +1051,38 @@ int rbc_mlp_propose(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, uin
   return RBC_OK;
 }
 
-int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels, int mem,
-                  uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *epoch_loss) {
+int rbc_mlp_propose_lists(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, uint32_t P, float mass,
+                          uint32_t *lists, uint32_t *counts) {
+  RBC_ARG_CHECK(m && x && lists && counts && rows, "bad argument");
+  RBC_ARG_CHECK(ldx >= m->D, "ldx < input width");
+  DevBuf<float> xd;
+  DevBuf<uint32_t> ld, cd;
+  RBC_TRY(xd.alloc(size_t(rows) * ldx));
+  RBC_TRY(ld.alloc(size_t(rows) * P));
+  RBC_TRY(cd.alloc(rows));
+  RBC_HIP_TRY(hipMemcpy(xd.get(), x, size_t(rows) * ldx * 4, hipMemcpyHostToDevice));
+  RBC_TRY(mlp_propose_lists(m, xd.get(), ldx, rows, P, mass, ld.get(), cd.get(), nullptr));
+  RBC_HIP_TRY(hipMemcpy(lists, ld.get(), size_t(rows) * P * 4, hipMemcpyDeviceToHost));
+  RBC_HIP_TRY(hipMemcpy(counts, cd.get(), size_t(rows) * 4, hipMemcpyDeviceToHost));
+  return RBC_OK;
+}
+
+int rbc_mlp_train_multi(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels,
+                        uint32_t labels_per_row, int mem, uint32_t epochs, uint32_t batch, float lr, uint64_t seed,
+                        float *epoch_loss) {
   RBC_ARG_CHECK(m && x && labels && rows && batch, "bad argument");
   RBC_ARG_CHECK(ldx >= m->D, "ldx < input width");
+  const uint32_t L = labels_per_row;
+  RBC_ARG_CHECK(L >= 1 && L <= 64, "labels per row %u outside [1, 64]", L);
   hipStream_t st = nullptr;
   RBC_TRY(m->ensure(batch));
+  RBC_TRY(m->ensure_labels(size_t(batch) * L));
   DevMat X;
   RBC_TRY(X.stage(x, rows, m->D, ldx, mem, st));
   DevBuf<uint32_t> labd, perm;
-  RBC_TRY(labd.alloc(rows));
+  RBC_TRY(labd.alloc(size_t(rows) * L));
   RBC_TRY(perm.alloc(rows));
-  RBC_TRY(copy_in(labd.get(), labels, size_t(rows) * 4, mem, st));
+  RBC_TRY(copy_in(labd.get(), labels, size_t(rows) * L * 4, mem, st));
   std::vector<uint32_t> hp(rows);
   for (uint32_t i = 0; i < rows; ++i) hp[i] = i;
   std::mt19937_64 rng(seed);
@@ -781,11 +1099,11 @@ int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const
       int64_t ne = int64_t(B) * m->D;
       hipLaunchKernelGGL(gather_half, dim3((uint32_t)cdivz(ne, 256)), dim3(256), 0, st, X.p, (int64_t)X.ld,
                          perm.get() + r0, (int)B, (int)m->D, m->x16.get());
-      hipLaunchKernelGGL(gather_u32, dim3(cdiv(B, 256)), dim3(256), 0, st, labd.get(), perm.get() + r0, (int)B,
-                         m->lab.get());
+      hipLaunchKernelGGL(gather_u32, dim3(cdiv(B * L, 256)), dim3(256), 0, st, labd.get(), perm.get() + r0,
+                         (int)B, (int)L, m->lab.get());
       RBC_HIP_TRY(hipGetLastError());
       RBC_TRY(mlp_forward(m, B, nullptr, st));
-      RBC_TRY(mlp_backward(m, B, st));
+      RBC_TRY(mlp_backward(m, B, L, st));
       ++m->step;
       float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
       hipLaunchKernelGGL(adam_step, dim3((uint32_t)cdivz(m->np, 256)), dim3(256), 0, st, m->w.get(), m->w16.get(),
@@ -801,6 +1119,11 @@ int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const
   }
   RBC_HIP_TRY(hipDeviceSynchronize());
   return RBC_OK;
+}
+
+int rbc_mlp_train(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ldx, const uint32_t *labels, int mem,
+                  uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *epoch_loss) {
+  return rbc_mlp_train_multi(m, x, rows, ldx, labels, 1, mem, epochs, batch, lr, seed, epoch_loss);
 }
 
 }  // extern "C"

@@ -97,6 +97,7 @@ _SIGS = {
     "rbc_exact_save": (_int, [_vp, ctypes.c_char_p]),
     "rbc_exact_load": (_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "rbc_exact_set_proposal": (_int, [_vp, _vp]),
+    "rbc_exact_set_proposal_mass": (_int, [_vp, ctypes.c_float]),
     "rbc_mlp_create": (_int, [_u32, _u32, _u32, _u32, _u64, ctypes.POINTER(_vp)]),
     "rbc_mlp_destroy": (_int, [_vp]),
     "rbc_mlp_num_params": (_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
@@ -105,6 +106,10 @@ _SIGS = {
     "rbc_mlp_save": (_int, [_vp, ctypes.c_char_p]),
     "rbc_mlp_load": (_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "rbc_mlp_loss_grad": (_int, [_vp, _vp, _u32, _u32, _vp, ctypes.POINTER(ctypes.c_float), _vp]),
+    "rbc_mlp_loss_grad_multi": (_int, [_vp, _vp, _u32, _u32, _vp, _u32, ctypes.POINTER(ctypes.c_float), _vp]),
+    "rbc_mlp_propose_lists": (_int, [_vp, _vp, _u32, _u32, _u32, ctypes.c_float, _vp, _vp]),
+    "rbc_mlp_train_multi": (_int, [_vp, _vp, _u32, _u32, _vp, _u32, _int, _u32, _u32, ctypes.c_float, _u64,
+                                   _vp]),
     "rbc_mlp_logits": (_int, [_vp, _vp, _u32, _u32, _vp]),
     "rbc_mlp_propose": (_int, [_vp, _vp, _u32, _u32, _vp]),
     "rbc_mlp_train": (_int, [_vp, _vp, _u32, _u32, _vp, _int, _u32, _u32, ctypes.c_float, _u64, _vp]),
@@ -340,8 +345,11 @@ class ExactRBC:
                                       ctypes.byref(st)))
         return st.as_dict()
 
-    def set_proposal(self, mlp):
+    def set_proposal(self, mlp, mass=None):
+        """rbc_exact_set_proposal (and, when given, rbc_exact_set_proposal_mass)."""
         _check(lib().rbc_exact_set_proposal(self.h, mlp.h if mlp is not None else None))
+        if mass is not None:
+            _check(lib().rbc_exact_set_proposal_mass(self.h, mass))
         self._proposal = mlp
 
     def info(self):
@@ -411,12 +419,14 @@ class ProposalMLP:
         _check(lib().rbc_mlp_set_params(self.h, pp))
 
     def loss_grad(self, x, labels):
+        """Loss and gradient; labels [rows] (one label) or [rows, L] (soft targets)."""
         x, xp = _host(x, np.float32)
         lab, lp = _host(labels, np.uint32)
+        L = 1 if lab.ndim == 1 else lab.shape[1]
         loss = ctypes.c_float()
         g = np.empty(self.num_params, np.float32)
-        _check(lib().rbc_mlp_loss_grad(self.h, xp, x.shape[0], x.shape[1], lp, ctypes.byref(loss),
-                                       g.ctypes.data))
+        _check(lib().rbc_mlp_loss_grad_multi(self.h, xp, x.shape[0], x.shape[1], lp, L, ctypes.byref(loss),
+                                             g.ctypes.data))
         return loss.value, g
 
     def logits(self, x):
@@ -432,12 +442,23 @@ class ProposalMLP:
         _check(lib().rbc_mlp_propose(self.h, xp, x.shape[0], x.shape[1], out.ctypes.data))
         return out
 
+    def propose_lists(self, x, P, mass):
+        """rbc_mlp_propose_lists: (lists [rows, P], counts [rows])."""
+        x, xp = _host(x, np.float32)
+        lists = np.empty((x.shape[0], P), np.uint32)
+        counts = np.empty(x.shape[0], np.uint32)
+        _check(lib().rbc_mlp_propose_lists(self.h, xp, x.shape[0], x.shape[1], P, mass, lists.ctypes.data,
+                                           counts.ctypes.data))
+        return lists, counts
+
     def train(self, x, labels, epochs=5, batch=1024, lr=1e-3, seed=0):
+        """Adam training; labels [rows] (one label) or [rows, L] (soft targets over L)."""
         x, xp = _host(x, np.float32)
         lab, lp = _host(labels, np.uint32)
+        L = 1 if lab.ndim == 1 else lab.shape[1]
         losses = np.zeros(epochs, np.float32)
-        _check(lib().rbc_mlp_train(self.h, xp, x.shape[0], x.shape[1], lp, MEM_HOST, epochs, batch,
-                                   lr, seed, losses.ctypes.data))
+        _check(lib().rbc_mlp_train_multi(self.h, xp, x.shape[0], x.shape[1], lp, L, MEM_HOST, epochs, batch,
+                                         lr, seed, losses.ctypes.data))
         return losses
 
     def close(self):
