@@ -22,8 +22,9 @@ peak, re-rank gathers and query/result bytes at the HBM peak).  ``measured_peaks
 the device's own BF16 MFMA and FP32 FMA rates (``rbc_measure_peaks``) and the
 bandwidth of a 4 GiB device copy, next to the published figures used for ``frac``.
 
-``cpu_baseline`` is the oracle's FP32 brute force (the reference's bruteK arithmetic) on
-a bounded query sample, on the host's cores.
+``cpu_baseline`` is the CPU exact RBC (the same index's lists, triangle and Voronoi-plane
+pruning, AVX2 + OpenMP on the host's cores) on a bounded query sample; ``cpu_other`` has
+the reference's single-thread bruteCPU (k = 1) and an all-core FP32 brute-force k-NN.
 
 ``--workload batched256`` times the batched form instead: 256 independent databases
 answered by one hipGraph-replayed persistent launch.
@@ -296,7 +297,10 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
-    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--cpu-sample", type=int, default=2048,
+                    help="queries timed for the CPU exact RBC baseline")
+    ap.add_argument("--cpu-sample-brute", type=int, default=256)
+    ap.add_argument("--cpu-sample-1thread", type=int, default=8)
     ap.add_argument("--check-sample", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
@@ -490,14 +494,43 @@ def main():
         import oracle  # CPU baseline leg (and its checker)
 
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        threads = min(threads, len(os.sched_getaffinity(0)))
+        # CPU exact RBC over the GPU index's own lists (same reps, radii and Voronoi cells).
+        L = idx.lists()
+        t0 = time.time()
+        crbc = oracle.CpuRBC(x, L["rep_ids"], L["perm"], L["offsets"], L["radii"], True, threads)
+        cpu_build = time.time() - t0
         ns = min(args.cpu_sample, m)
         t0 = time.time()
-        oracle.brute_k_f32(x, q[:ns], k, 0, threads)
+        ci, _, cpairs = crbc.knn(q[:ns], k, threads)
         cs = time.time() - t0
+        gi_s = ids[:ns].cpu().numpy().astype(np.uint32)
         cpu = {"value": round(ns / cs, 2), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"first {ns} of {m} queries, FP32 brute force over all {n} points "
-                         f"(bruteK arithmetic, brute.cu:65-88), {cs:.1f}s"}
-        log(f"cpu baseline {ns/cs:.2f} q/s on {threads} threads ({cs:.1f}s)")
+               "sample": f"first {ns} of {m} queries, CPU exact RBC (triangle + Voronoi-plane "
+                         f"pruning, AVX2 + OpenMP) over the same index's lists, {cs:.1f}s; "
+                         f"{cpairs / ns / n:.4f} of the points scanned per query; layout "
+                         f"{cpu_build:.1f}s (not timed)",
+               "ids_equal_gpu_sample": float((ci == gi_s).all(1).mean())}
+        log(f"cpu exact RBC {ns/cs:.2f} q/s on {threads} threads ({cs:.1f}s), "
+            f"{cpairs/ns/n:.4f} of points per query")
+        del crbc
+        # The reference's bruteCPU (brute.cu:91-112): FP32 1-NN, one thread.
+        n1 = max(1, min(args.cpu_sample_1thread, m))
+        t0 = time.time()
+        oracle.brute_cpu(x, q[:n1], 0, 1)
+        c1 = time.time() - t0
+        # FP32 brute-force k-NN (kernels.cu distance arithmetic), all cores.
+        nb = min(args.cpu_sample_brute, m)
+        t0 = time.time()
+        oracle.brute_k_f32(x, q[:nb], k, 0, threads)
+        cb = time.time() - t0
+        extra["cpu_other"] = {
+            "bruteCPU_1nn_1thread": {"value": round(n1 / c1, 3), "unit": "queries/s", "cores": 1,
+                                     "sample": f"first {n1} queries, k=1, {c1:.1f}s"},
+            "brute_fp32_knn_allcores": {"value": round(nb / cb, 2), "unit": "queries/s",
+                                        "cores": threads, "sample": f"first {nb} queries, {cb:.1f}s"},
+        }
+        log(f"bruteCPU 1 thread {n1/c1:.3f} q/s; FP32 brute k-NN {nb/cb:.2f} q/s on {threads} threads")
         # Checker on a sample: FP64 oracle vs the GPU result.
         nc = min(args.check_sample, m)
         oi, od = oracle.brute_k_f64(x, q[:nc], k, 0, threads)

@@ -62,6 +62,13 @@ def lib():
                                               _int]
         L.orc_exact_owners.argtypes = [_f32p, _u32, _u32, _u32, _int, _u32p, _u32, _u32p, _int]
         L.orc_exact_radii.argtypes = [_f32p, _u32, _u32, _u32, _int, _u32p, _u32, _u32p, _f32p]
+        L.orc_rbc_soa_size.argtypes = [_u32p, _u32, _u32]
+        L.orc_rbc_soa_size.restype = ctypes.c_size_t
+        L.orc_rbc_soa.argtypes = [_f32p, _u32, _u32, _u32p, _u32p, _u32, _f32p]
+        L.orc_rep_dist2.argtypes = [_f32p, _u32, _u32, _u32, _f64p, _int]
+        L.orc_rbc_exact_knn.argtypes = [_f32p, _f32p, _u32, _u32, _u32, _f32p, _u32, _u32, _u32p,
+                                        _u32p, _f32p, ctypes.c_void_p, _u32, _u32p, _f32p, _u32, _int,
+                                        ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -193,6 +200,45 @@ def exact_radii(x, rep_ids, owner, metric=0):
     lib().orc_exact_radii(x, x.shape[0], x.shape[1], x.shape[1], metric, rep_ids,
                           rep_ids.shape[0], _c(owner, np.uint32), rad)
     return rad
+
+
+class CpuRBC:
+    """CPU exact RBC search over the lists of an exact index (rep_ids, perm, offsets, radii):
+    the points are laid out list-major in 8-point blocks once, then ``knn`` answers queries
+    with triangle-inequality pruning, AVX2 + OpenMP (the CPU baseline)."""
+
+    def __init__(self, x, rep_ids, perm, offsets, radii, planes=True, threads=0):
+        x = _c(x, np.float32)
+        self.d = x.shape[1]
+        self.reps = _c(x[_c(rep_ids, np.int64)], np.float32)
+        self.perm = _c(perm, np.uint32)
+        self.off = _c(offsets, np.uint32)
+        self.radii = _c(radii, np.float32)
+        R = self.reps.shape[0]
+        self.soa = np.empty(lib().orc_rbc_soa_size(self.off, R, self.d), np.float32)
+        lib().orc_rbc_soa(x, x.shape[1], self.d, self.perm, self.off, R, self.soa)
+        self.rr = None
+        if planes:
+            self.rr = np.empty((R, R), np.float64)
+            lib().orc_rep_dist2(self.reps, self.d, R, self.d, self.rr, threads)
+
+    def knn(self, q, k, threads=0):
+        """(ids, dists, pairs evaluated) of the k nearest points of each query."""
+        q = _c(q, np.float32)
+        m = q.shape[0]
+        ids = np.empty((m, k), np.uint32)
+        dists = np.empty((m, k), np.float32)
+        pairs = ctypes.c_double()
+        lib().orc_rbc_exact_knn(self.soa, q, m, q.shape[1], self.d, self.reps, self.d,
+                                self.reps.shape[0], self.perm, self.off, self.radii,
+                                None if self.rr is None else self.rr.ctypes.data, k, ids, dists,
+                                k, threads, ctypes.byref(pairs))
+        return ids, dists, pairs.value
+
+
+def rbc_exact_knn(x, q, rep_ids, perm, offsets, radii, k, threads=0, planes=True):
+    """CPU exact RBC k-NN over the lists of an exact index: (ids, dists, pairs evaluated)."""
+    return CpuRBC(x, rep_ids, perm, offsets, radii, planes, threads).knn(q, k, threads)
 
 
 def knn_f64_numpy(x, q, k, metric=0):

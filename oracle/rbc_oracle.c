@@ -7,6 +7,7 @@
 #include "rbc_oracle.h"
 
 #include <float.h>
+#include <immintrin.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -386,4 +387,160 @@ void orc_exact_radii(const float *x, uint32_t n, uint32_t ldx, uint32_t d, int m
     radius[r] = f;
   }
   free(rad);
+}
+
+/* Floats of the list-major, 8-point-blocked copy of the points (orc_rbc_soa). */
+size_t orc_rbc_soa_size(const uint32_t *off, uint32_t R, uint32_t d) {
+  size_t blocks = 0;
+  for (uint32_t r = 0; r < R; ++r) blocks += (off[r + 1] - off[r] + 7) / 8;
+  return blocks * 8 * d;
+}
+
+/* The CPU search's layout of the points: list by list (perm order), blocks of 8 points
+ * stored coordinate-major ([d][8]), the last block of a list zero-padded. */
+void orc_rbc_soa(const float *x, uint32_t ldx, uint32_t d, const uint32_t *perm,
+                 const uint32_t *off, uint32_t R, float *soa) {
+  size_t b = 0;
+  for (uint32_t r = 0; r < R; ++r) {
+    for (uint32_t p0 = off[r]; p0 < off[r + 1]; p0 += 8, ++b) {
+      float *blk = soa + b * 8 * d;
+      for (uint32_t l = 0; l < 8; ++l) {
+        const uint32_t p = p0 + l;
+        for (uint32_t c = 0; c < d; ++c)
+          blk[c * 8 + l] = p < off[r + 1] ? x[(size_t)perm[p] * ldx + c] : 0.0f;
+      }
+    }
+  }
+}
+
+/* Sequential-FMA squared distances (orc_dist_f32_fma's arithmetic, lane by lane) of
+ * nb <= 4 consecutive 8-point blocks to q: four independent FMA chains per step. */
+static void dist_blocks(const float *blk, uint32_t nb, const float *q, uint32_t d, float *out) {
+  __m256 a0 = _mm256_setzero_ps(), a1 = a0, a2 = a0, a3 = a0;
+  const size_t s = (size_t)8 * d;
+  for (uint32_t c = 0; c < d; ++c) {
+    const __m256 qc = _mm256_set1_ps(q[c]);
+    __m256 t = _mm256_sub_ps(_mm256_loadu_ps(blk + c * 8), qc);
+    a0 = _mm256_fmadd_ps(t, t, a0);
+    if (nb > 1) {
+      t = _mm256_sub_ps(_mm256_loadu_ps(blk + s + c * 8), qc);
+      a1 = _mm256_fmadd_ps(t, t, a1);
+    }
+    if (nb > 2) {
+      t = _mm256_sub_ps(_mm256_loadu_ps(blk + 2 * s + c * 8), qc);
+      a2 = _mm256_fmadd_ps(t, t, a2);
+    }
+    if (nb > 3) {
+      t = _mm256_sub_ps(_mm256_loadu_ps(blk + 3 * s + c * 8), qc);
+      a3 = _mm256_fmadd_ps(t, t, a3);
+    }
+  }
+  _mm256_storeu_ps(out, a0);
+  _mm256_storeu_ps(out + 8, a1);
+  _mm256_storeu_ps(out + 16, a2);
+  _mm256_storeu_ps(out + 24, a3);
+}
+
+typedef struct {
+  float v;
+  uint32_t r;
+} orc_key;
+
+static int key_cmp(const void *a, const void *b) {
+  const orc_key *x = (const orc_key *)a, *y = (const orc_key *)b;
+  if (x->v != y->v) return x->v < y->v ? -1 : 1;
+  return x->r < y->r ? -1 : (x->r > y->r);
+}
+
+/* Squared FP64 distances between the R representatives (rr[R][R]), for the plane bound. */
+void orc_rep_dist2(const float *reps, uint32_t ldr, uint32_t R, uint32_t d, double *rr, int threads) {
+  set_threads(threads);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (long i = 0; i < (long)R; ++i)
+    for (uint32_t j = 0; j < R; ++j)
+      rr[(size_t)i * R + j] = orc_dist_f64(reps + (size_t)i * ldr, reps + (size_t)j * ldr, d, 0);
+}
+
+/* CPU exact RBC k-NN: the reference's RBC search structure (rbc.cu:19-123, representatives
+ * and their lists) restated on the host as an exact search with the triangle-inequality
+ * pruning of the exact RBC (computeRadii, rbc.cu:250-256; deprecated.cu:354-464) and the
+ * Voronoi-plane bound of an exact (nearest-rep) assignment, vectorized over points (AVX2,
+ * 8-point blocks of orc_rbc_soa's layout) and parallel over queries (OpenMP).  Each query
+ * computes its distances to the R representatives, visits the lists in ascending
+ * representative distance and skips list j when every point of it is provably farther than
+ * its current k-th result.  With e = (d' + 4) 2^-24 (1.01), d' = d rounded up to a multiple
+ * of 4 (the relative error of an FP32 sequential-FMA distance, and of the FP32 owner test
+ * that made the lists), the bounds on the true distance of q to a point of list j are
+ *   triangle: sqrt(Dqr_j (1 - e)) - psi_j,
+ *   plane:    (Dqr_j (1 - e) - Dqr_i (1 + e) - 2 e psi_j^2) / (2 |r_j - r_i|)
+ * for the query's P nearest reps r_i (P = 4; a point x owned by r_j has |x - r_j|^2 <=
+ * |x - r_i|^2 + 2 e psi_j^2 for every rep), and the list is skipped when the largest bound
+ * L > 0 has L^2 (1 - e) above the k-th FP32 distance.  Inside a list it keeps the k best
+ * (FP32 sequential-FMA distances, lowest original index among equals), so the results
+ * equal orc_brute_k_f32's.  L2 only.  perm maps permuted rows to original ids, off[R+1]
+ * are the list offsets, reps[R][ldr], radii[R] the list radii (max true distance of a
+ * list's points to its rep, rounded up), rr the FP64 rep-to-rep squared distances
+ * (orc_rep_dist2; NULL: triangle bound only), soa the points in orc_rbc_soa's layout.
+ * *pairs receives the number of (query, point) distances evaluated. */
+void orc_rbc_exact_knn(const float *soa, const float *q, uint32_t m, uint32_t ldq, uint32_t d,
+                       const float *reps, uint32_t ldr, uint32_t R, const uint32_t *perm,
+                       const uint32_t *off, const float *radii, const double *rr, uint32_t k,
+                       uint32_t *ids, float *dists, uint32_t ldo, int threads, double *pairs) {
+  set_threads(threads);
+  const double eps = ((double)((d + 3u) & ~3u) + 4.0) * 0x1p-24 * 1.01;
+  const uint32_t P = R < 4 ? R : 4;
+  size_t *boff = (size_t *)malloc(sizeof(size_t) * (R + 1));
+  boff[0] = 0;
+  for (uint32_t r = 0; r < R; ++r) boff[r + 1] = boff[r] + (off[r + 1] - off[r] + 7) / 8;
+  double total = 0.0;
+#pragma omp parallel reduction(+ : total)
+  {
+    orc_key *key = (orc_key *)malloc(sizeof(orc_key) * R);
+    float dd[32];
+#pragma omp for schedule(dynamic, 1)
+    for (long i = 0; i < (long)m; ++i) {
+      const float *qi = q + (size_t)i * ldq;
+      float *dv = dists + (size_t)i * ldo;
+      uint32_t *iv = ids + (size_t)i * ldo;
+      for (uint32_t j = 0; j < k; ++j) {
+        dv[j] = FLT_MAX;
+        iv[j] = ORC_DUMMY;
+      }
+      for (uint32_t r = 0; r < R; ++r) {
+        key[r].v = orc_dist_f32_fma(reps + (size_t)r * ldr, qi, d, 0);
+        key[r].r = r;
+      }
+      qsort(key, R, sizeof(orc_key), key_cmp);
+      for (uint32_t a = 0; a < R; ++a) {
+        const uint32_t r = key[a].r;
+        if (dv[k - 1] < FLT_MAX) {
+          const double psi = (double)radii[r], Dlo = (double)key[a].v * (1.0 - eps);
+          double L = sqrt(Dlo) - psi;
+          for (uint32_t pi = 0; rr && pi < P; ++pi) {
+            const uint32_t ri = key[pi].r;
+            const double rij = rr[(size_t)ri * R + r];
+            if (ri == r || !(rij > 0.0)) continue;
+            const double num = Dlo - (double)key[pi].v * (1.0 + eps) - 2.0 * eps * psi * psi;
+            const double pl = num / (2.0 * sqrt(rij) * (1.0 + 1e-12));
+            if (pl > L) L = pl;
+          }
+          L *= 1.0 - 1e-12;
+          if (L > 0.0 && L * L * (1.0 - eps) * (1.0 - 1e-12) > (double)dv[k - 1]) continue;
+        }
+        const uint32_t len = off[r + 1] - off[r];
+        const uint32_t nblk = (len + 7) / 8;
+        for (uint32_t b = 0; b < nblk; b += 4) {
+          const uint32_t nb = nblk - b < 4 ? nblk - b : 4;
+          dist_blocks(soa + (boff[r] + b) * 8 * d, nb, qi, d, dd);
+          const uint32_t p0 = b * 8, cnt = len - p0 < 32 ? len - p0 : 32;
+          for (uint32_t l = 0; l < cnt; ++l)
+            if (dd[l] <= dv[k - 1]) topk_insert_f(dv, iv, k, dd[l], perm[off[r] + p0 + l]);
+        }
+        total += (double)len;
+      }
+    }
+    free(key);
+  }
+  free(boff);
+  if (pairs) *pairs = total;
 }

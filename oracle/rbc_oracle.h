@@ -15,6 +15,7 @@
 #ifndef RBC_ORACLE_H
 #define RBC_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -95,6 +96,19 @@ void orc_exact_owners(const float *x, uint32_t n, uint32_t ldx, uint32_t d, int 
 void orc_exact_radii(const float *x, uint32_t n, uint32_t ldx, uint32_t d, int metric,
                      const uint32_t *rep_ids, uint32_t R, const uint32_t *owner,
                      float *radius);
+
+/* CPU exact RBC k-NN (triangle and Voronoi-plane pruning, AVX2 over points, OpenMP over
+ * queries, L2): the results of orc_brute_k_f32 from the lists of an exact index (perm,
+ * off[R+1], reps, radii, FP64 rep-to-rep distances rr from orc_rep_dist2) with the points
+ * in orc_rbc_soa's layout (orc_rbc_soa_size floats). */
+size_t orc_rbc_soa_size(const uint32_t *off, uint32_t R, uint32_t d);
+void orc_rbc_soa(const float *x, uint32_t ldx, uint32_t d, const uint32_t *perm,
+                 const uint32_t *off, uint32_t R, float *soa);
+void orc_rep_dist2(const float *reps, uint32_t ldr, uint32_t R, uint32_t d, double *rr, int threads);
+void orc_rbc_exact_knn(const float *soa, const float *q, uint32_t m, uint32_t ldq, uint32_t d,
+                       const float *reps, uint32_t ldr, uint32_t R, const uint32_t *perm,
+                       const uint32_t *off, const float *radii, const double *rr, uint32_t k,
+                       uint32_t *ids, float *dists, uint32_t ldo, int threads, double *pairs);
 
 #ifdef __cplusplus
 }

@@ -275,3 +275,32 @@ def test_golden_fixture_reproduces():
     ids, ds = oracle.brute_k_f64(g["x"], g["q"], 32)
     np.testing.assert_array_equal(ids, g["ids"])
     np.testing.assert_array_equal(ds, g["dists"])
+
+
+@pytest.mark.parametrize("data", ["gmm", "lattice"])
+def test_cpu_exact_rbc_equals_brute_force(data):
+    # The CPU exact RBC (the CPU baseline's search) returns orc_brute_k_f32's results exactly,
+    # ties included (the lattice has many equal distances), while skipping lists; lists
+    # have ragged lengths (partial 8-point blocks).
+    rng = np.random.default_rng(11)
+    if data == "gmm":
+        x = datagen.gmm(4000, 12, 3)
+        q = datagen.gmm(300, 12, 4)
+    else:
+        x = rng.integers(0, 4, size=(4000, 6)).astype(np.float32)
+        q = rng.integers(0, 4, size=(300, 6)).astype(np.float32)
+    R, k = 50, 7
+    reps = np.sort(rng.choice(x.shape[0], R, replace=False)).astype(np.uint32)
+    owner = oracle.exact_owners(x, reps)
+    radii = oracle.exact_radii(x, reps, owner)
+    perm = np.argsort(owner, kind="stable").astype(np.uint32)
+    off = np.concatenate([[0], np.cumsum(np.bincount(owner, minlength=R))]).astype(np.uint32)
+    bi, bd = oracle.brute_k_f32(x, q, k)
+    seen = []
+    for planes in (False, True):
+        ids, ds, pairs = oracle.rbc_exact_knn(x, q, reps, perm, off, radii, k, planes=planes)
+        np.testing.assert_array_equal(ids, bi)
+        np.testing.assert_array_equal(ds, bd)
+        seen.append(pairs)
+    # Both bounds skip lists, and the plane bound skips more.
+    assert seen[1] <= seen[0] < 0.9 * x.shape[0] * q.shape[0]
