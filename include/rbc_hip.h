@@ -142,6 +142,48 @@ int rbc_kquery_plan(const float *q, uint32_t m, uint32_t ldq, const rbcStruct *r
 /* destroyRBC (rbc.cu:363-368). */
 int rbc_destroy(rbcStruct *rbcs);
 
+/* The reference's build and query internals (rbc.h:15-26), for callers that assemble a
+ * one-shot search step by step.  Device matrices are device pointers, as in the
+ * reference; their leading dimensions must be multiples of 4 (the reference's PAD(c) =
+ * multiple of 16 always is).  The distance is the process-wide metric (rbc_set_metric).
+ *
+ * distSubMat (rbc.cu:356-360, over dist1Wrap, kernelWrap.cu:13-34): D[i][j] = distance of
+ * rep row start+i of r to point row j of x for i < length, j < nx (FP32, FMA accumulation
+ * in coordinate order), D with leading dimension ldd >= nx. */
+int rbc_dist_sub(const float *r, uint32_t ldr, uint32_t start, uint32_t length, const float *x,
+                 uint32_t nx, uint32_t ldx, uint32_t d, float *D, uint32_t ldd, void *stream);
+/* computeReps (rbc.cu:233-247, over nnWrap): nearest of the R reps for each of the m
+ * queries (lowest index among equal distances) and its distance.  q and reps are host or
+ * device memory per mem_in, rep_ids and dist_to_reps per mem_out (the reference: device
+ * in, host out). */
+int rbc_compute_reps(const float *q, uint32_t m, uint32_t ldq, const float *reps, uint32_t R,
+                     uint32_t ldr, uint32_t d, uint32_t *rep_ids, float *dist_to_reps, int mem_in,
+                     int mem_out, void *stream);
+/* initCompPlan (rbc.cu:375-431): the device compPlan of a host computation matrix cm
+ * (num_reps x num_reps bytes, row stride ldcm; nonzero = planned), the per-rep query
+ * counts group_count_q and list lengths group_count_x.  Allocates the four device arrays;
+ * rbc_plan_free (freeCompPlan, rbc.cu:435-440) releases them. */
+int rbc_plan_init(compPlan *out, const char *cm, uint32_t ldcm, const uint32_t *group_count_q,
+                  const uint32_t *group_count_x, uint32_t num_reps);
+int rbc_plan_free(compPlan *plan);
+/* computeNNs / computeKNNs (rbc.cu:317-351, over planNNKernel / planKNNKernel,
+ * kernels.cu:15-161): each query in the device query map qmap (comp_length slots, blocks
+ * of 16 slots taking the group of their first slot, DUMMY_IDX = empty slot) scans the
+ * lists its group plans: entries [0, groupCountX) of rows qGroupToXGroup of the device
+ * xMap (xmap_rows x ldxmap, point ids < nx) over the device database x.  The K best
+ * (K = 1: computeNNs; KMAX: computeKNNs) of the union of those lists as a multiset, per
+ * query row 0..m-1 (ld ldo; a query in no slot gets (MAX_REAL, DUMMY_IDX)), go to ids /
+ * dists in host or device memory per mem_out.  The plan and maps are checked (ranges,
+ * each query at most once) before any point is read. */
+int rbc_plan_query(const float *x, uint32_t nx, uint32_t ldx, const uint32_t *xmap,
+                   uint32_t xmap_rows, uint32_t ldxmap, const float *q, uint32_t m, uint32_t ldq,
+                   uint32_t d, const uint32_t *qmap, uint32_t comp_length, const compPlan *plan,
+                   uint32_t K, uint32_t *ids, float *dists, uint32_t ldo, int mem_out, void *stream);
+/* setupReps (rbc.cu:212-229) with the reps given: copies rows rep_ids[0..R) of the host
+ * matrix x into a new zero-padded device matrix (PAD(R) x PAD(d), ld PAD(d)). */
+int rbc_setup_reps(const float *x, uint32_t n, uint32_t d, uint32_t ldx, const uint32_t *rep_ids,
+                   uint32_t R, matrix *out);
+
 /* ---------------------------------------------------------------- exact RBC ----- */
 
 /* Opaque handle of an exact-search index: representatives, the database permuted

@@ -122,18 +122,6 @@ void readDataText(const char *file, matrix x) {
   fclose(fp);
 }
 
-// Host distance between row i of a and row j of b, summed in coordinate order with a
-// rounding after every product and every addition (utils.cu:183-191).
-real distVec(matrix a, matrix b, unint i, unint j) {
-  real ans = 0;
-  for (unint c = 0; c < a.c; ++c) {
-    volatile real t = a.mat[IDX(i, c, a.ld)] - b.mat[IDX(j, c, b.ld)];
-    volatile real term = useL1 ? fabsf(t) : t * t;
-    ans += term;
-  }
-  return ans;
-}
-
 void meanStd(const std::vector<double> &v, double *mean, double *sd) {
   double s = 0;
   for (double a : v) s += a;

@@ -168,20 +168,18 @@ __global__ void row_totals(const uint32_t *off, uint32_t rows, uint32_t nseg, ui
 // for any plan (fullIntersection, rbc.cu:305-312, included).
 constexpr uint32_t kPlanSlots = 8;
 
-static int oneshot_query_plan(const rbcStruct *s, const float *Qp, size_t ldQ, uint32_t m, uint32_t K,
-                       const uint32_t *qorder, const std::vector<uint32_t> &hqoff, const uint8_t *plan,
-                       uint32_t ldp, const std::vector<uint32_t> &hxo, const std::vector<uint32_t> &hxl,
-                       int metric, float *od, uint32_t *oi, hipStream_t st) {
-  const uint32_t d = s->dx.c, R = s->dr.r;
-  // Planned lists of each group that has queries.
-  std::vector<std::vector<uint32_t>> lists(R);
+// One planned list of a query group: its xMap entries [x_begin, x_begin + x_count).
+struct PlanList {
+  uint32_t x_begin, x_count;
+};
+
+// Queries src.qids[hqoff[g] .. hqoff[g+1]) (group g) scan lists[g] in order; m rows of K
+// results (ascending (distance, id)) land in od/oi on the device.
+static int oneshot_query_plan(const TileSrc &src, uint32_t m, uint32_t K, const std::vector<uint32_t> &hqoff,
+                              const std::vector<std::vector<PlanList>> &lists, int metric, float *od, uint32_t *oi,
+                              hipStream_t st) {
   size_t most = 0;
-  for (uint32_t i = 0; i < R; ++i) {
-    if (hqoff[i + 1] == hqoff[i]) continue;
-    for (uint32_t j = 0; j < R; ++j)
-      if (plan[size_t(i) * ldp + j]) lists[i].push_back(j);
-    most = std::max(most, lists[i].size());
-  }
+  for (const auto &l : lists) most = std::max(most, l.size());
   const uint32_t per = kPlanSlots - 1;
   const size_t slot = size_t(m) * K;
   DevBuf<float> pd;
@@ -190,19 +188,18 @@ static int oneshot_query_plan(const rbcStruct *s, const float *Qp, size_t ldQ, u
   RBC_TRY(pi.alloc(kPlanSlots * slot));
   RBC_TRY(fill_f32(pd.get(), FLT_MAX, slot, st));
   RBC_TRY(fill_u32(pi.get(), kDummy, slot, st));
-  TileSrc src{s->dx.mat, s->dx.ld, s->dxMap.mat, nullptr, Qp, ldQ, qorder, d, (d + 3) & ~3u};
   DevBuf<ScanItem> items;
   for (size_t first = 0; first < most; first += per) {
     std::vector<ScanItem> iv;
-    for (uint32_t i = 0; i < R; ++i) {
-      const uint32_t cnt = hqoff[i + 1] - hqoff[i];
-      for (size_t k = first; k < std::min(first + per, lists[i].size()); ++k) {
-        const uint32_t j = lists[i][k];
+    for (size_t g = 0; g < lists.size(); ++g) {
+      const uint32_t cnt = hqoff[g + 1] - hqoff[g];
+      for (size_t k = first; k < std::min(first + per, lists[g].size()); ++k) {
+        if (lists[g][k].x_count == 0) continue;
         for (uint32_t t = 0; t < cnt; t += kTQ) {
           ScanItem it;
-          it.x_begin = hxo[j];
-          it.x_count = hxl[j];
-          it.q_begin = hqoff[i] + t;
+          it.x_begin = lists[g][k].x_begin;
+          it.x_count = lists[g][k].x_count;
+          it.q_begin = hqoff[g] + t;
           it.q_count = std::min<uint32_t>(kTQ, cnt - t);
           it.aux = 1 + (uint32_t)(k - first);
           iv.push_back(it);
@@ -211,16 +208,18 @@ static int oneshot_query_plan(const rbcStruct *s, const float *Qp, size_t ldQ, u
     }
     RBC_TRY(fill_f32(pd.get() + slot, FLT_MAX, per * slot, st));
     RBC_TRY(fill_u32(pi.get() + slot, kDummy, per * slot, st));
-    RBC_TRY(upload_items(iv, items, st));
-    if (K == 1) {
-      EpiArgMin epi;
-      epi.out_d = pd.get();
-      epi.out_i = pi.get();
-      epi.slot_stride = m;
-      RBC_TRY(launch_tiles_m(metric, src, items.get(), (uint32_t)iv.size(), epi, st));
-    } else {
-      EpiTopK epi{K, pd.get(), pi.get(), K, m};
-      RBC_TRY(launch_tiles_m(metric, src, items.get(), (uint32_t)iv.size(), epi, st));
+    if (!iv.empty()) {
+      RBC_TRY(upload_items(iv, items, st));
+      if (K == 1) {
+        EpiArgMin epi;
+        epi.out_d = pd.get();
+        epi.out_i = pi.get();
+        epi.slot_stride = m;
+        RBC_TRY(launch_tiles_m(metric, src, items.get(), (uint32_t)iv.size(), epi, st));
+      } else {
+        EpiTopK epi{K, pd.get(), pi.get(), K, m};
+        RBC_TRY(launch_tiles_m(metric, src, items.get(), (uint32_t)iv.size(), epi, st));
+      }
     }
     // In place: each query's slots are read by one wave (one thread for K = 1) before it
     // writes slot 0.
@@ -268,8 +267,15 @@ int oneshot_query_common(const float *q, uint32_t m, uint32_t ldq, const rbcStru
     std::vector<uint32_t> hqoff(R + 1);
     RBC_HIP_TRY(hipMemcpyAsync(hqoff.data(), qoff.get(), (R + 1) * 4, hipMemcpyDeviceToHost, st));
     RBC_HIP_TRY(hipStreamSynchronize(st));
-    RBC_TRY(oneshot_query_plan(s, Q.p, Q.ld, m, K, qorder.get(), hqoff, plan, ldp, hxo, hxl, metric, od.get(),
-                               oi.get(), st));
+    // Planned lists of each group that has queries, ascending j.
+    std::vector<std::vector<PlanList>> lists(R);
+    for (uint32_t i = 0; i < R; ++i) {
+      if (hqoff[i + 1] == hqoff[i]) continue;
+      for (uint32_t j = 0; j < R; ++j)
+        if (plan[size_t(i) * ldp + j]) lists[i].push_back(PlanList{hxo[j], hxl[j]});
+    }
+    TileSrc src{s->dx.mat, s->dx.ld, s->dxMap.mat, nullptr, Q.p, Q.ld, qorder.get(), d, (d + 3) & ~3u};
+    RBC_TRY(oneshot_query_plan(src, m, K, hqoff, lists, metric, od.get(), oi.get(), st));
     RBC_TRY(rows_out(ids, size_t(ldo) * 4, oi.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
     if (dists) RBC_TRY(rows_out(dists, size_t(ldo) * 4, od.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
     RBC_HIP_TRY(hipStreamSynchronize(st));
@@ -450,4 +456,205 @@ int rbc_destroy(rbcStruct *s) {
   return RBC_OK;
 }
 
+// ------------------------------------------------ the reference's internals (rbc.h:15-26)
+
+int rbc_dist_sub(const float *r, uint32_t ldr, uint32_t start, uint32_t length, const float *x, uint32_t nx,
+                 uint32_t ldx, uint32_t d, float *D, uint32_t ldd, void *stream) {
+  RBC_ARG_CHECK(d >= 1 && d <= RBC_MAX_DIM, "dimension %u outside [1, %d]", d, RBC_MAX_DIM);
+  RBC_ARG_CHECK(ldr >= d && ldx >= d && ldr % 4 == 0 && ldx % 4 == 0,
+                "leading dimensions (%u, %u) must be >= d = %u and multiples of 4", ldr, ldx, d);
+  RBC_ARG_CHECK(ldd >= nx, "distance matrix leading dimension %u < %u points", ldd, nx);
+  if (!length || !nx) return RBC_OK;
+  RBC_ARG_CHECK(r && x && D, "NULL pointer");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  std::vector<ScanItem> iv;
+  const uint32_t seg = 1u << 16;
+  for (uint32_t x0 = 0; x0 < nx; x0 += seg)
+    for (uint32_t q0 = 0; q0 < length; q0 += kTQ)
+      iv.push_back(ScanItem{x0, std::min(seg, nx - x0), q0, std::min((uint32_t)kTQ, length - q0), x0});
+  DevBuf<ScanItem> items;
+  RBC_TRY(upload_items(iv, items, st));
+  TileSrc src{x, ldx, nullptr, nullptr, r + size_t(start) * ldr, ldr, nullptr, d, (d + 3) & ~3u};
+  EpiDense epi{D, ldd};
+  RBC_TRY(launch_tiles_m(metric_setting(), src, items.get(), (uint32_t)iv.size(), epi, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  return RBC_OK;
+}
+
+int rbc_compute_reps(const float *q, uint32_t m, uint32_t ldq, const float *reps, uint32_t R, uint32_t ldr,
+                     uint32_t d, uint32_t *rep_ids, float *dist_to_reps, int mem_in, int mem_out, void *stream) {
+  RBC_ARG_CHECK(d >= 1 && d <= RBC_MAX_DIM, "dimension %u outside [1, %d]", d, RBC_MAX_DIM);
+  RBC_ARG_CHECK(R >= 1, "no representatives");
+  RBC_ARG_CHECK(rep_ids && dist_to_reps, "NULL output");
+  if (m == 0) return RBC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DevMat Q, Rm;
+  RBC_TRY(Q.stage(q, m, d, ldq, mem_in, st));
+  RBC_TRY(Rm.stage(reps, R, d, ldr, mem_in, st));
+  DevBuf<uint32_t> ni;
+  DevBuf<float> nd;
+  RBC_TRY(ni.alloc(m));
+  RBC_TRY(nd.alloc(m));
+  RBC_TRY(nn_device(Rm.p, Rm.ld, R, nullptr, Q.p, Q.ld, m, d, metric_setting(), nd.get(), ni.get(), st));
+  RBC_TRY(rows_out(rep_ids, 4, ni.get(), 4, 4, m, mem_out, st));
+  RBC_TRY(rows_out(dist_to_reps, 4, nd.get(), 4, 4, m, mem_out, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  return RBC_OK;
+}
+
+int rbc_plan_init(compPlan *out, const char *cm, uint32_t ldcm, const uint32_t *group_count_q,
+                  const uint32_t *group_count_x, uint32_t num_reps) {
+  RBC_ARG_CHECK(out && group_count_q && group_count_x && (cm || num_reps == 0), "NULL argument");
+  RBC_ARG_CHECK(ldcm >= num_reps, "computation matrix leading dimension %u < %u reps", ldcm, num_reps);
+  const uint32_t R = num_reps;
+  std::vector<uint32_t> ng(R, 0);
+  uint32_t ld = 0;
+  for (uint32_t i = 0; i < R; ++i) {
+    for (uint32_t j = 0; j < R; ++j) ng[i] += cm[size_t(i) * ldcm + j] != 0;
+    ld = std::max(ld, ng[i]);
+  }
+  std::vector<uint32_t> q2g;
+  for (uint32_t i = 0; i < R; ++i) q2g.insert(q2g.end(), pad16(group_count_q[i]), i);
+  std::vector<uint32_t> g2x(size_t(R) * ld, 0), gcx(size_t(R) * ld, 0);
+  for (uint32_t i = 0; i < R; ++i)
+    for (uint32_t j = 0, k = 0; j < R; ++j)
+      if (cm[size_t(i) * ldcm + j]) {
+        g2x[size_t(i) * ld + k] = j;
+        gcx[size_t(i) * ld + k++] = group_count_x[j];
+      }
+  DevBuf<uint32_t> dng, dgcx, dq2g, dg2x;
+  RBC_TRY(dng.alloc(std::max<size_t>(R, 1)));
+  RBC_TRY(dgcx.alloc(std::max<size_t>(gcx.size(), 1)));
+  RBC_TRY(dq2g.alloc(std::max<size_t>(q2g.size(), 1)));
+  RBC_TRY(dg2x.alloc(std::max<size_t>(g2x.size(), 1)));
+  RBC_TRY(upload_host(dng.get(), ng.data(), ng.size() * 4, nullptr));
+  RBC_TRY(upload_host(dgcx.get(), gcx.data(), gcx.size() * 4, nullptr));
+  RBC_TRY(upload_host(dq2g.get(), q2g.data(), q2g.size() * 4, nullptr));
+  RBC_TRY(upload_host(dg2x.get(), g2x.data(), g2x.size() * 4, nullptr));
+  out->numGroups = dng.p;
+  out->groupCountX = dgcx.p;
+  out->qToQGroup = dq2g.p;
+  out->qGroupToXGroup = dg2x.p;
+  out->ld = ld;
+  dng.p = dgcx.p = dq2g.p = dg2x.p = nullptr;
+  return RBC_OK;
+}
+
+int rbc_plan_free(compPlan *p) {
+  RBC_ARG_CHECK(p, "NULL compPlan");
+  for (unint **a : {&p->numGroups, &p->groupCountX, &p->qToQGroup, &p->qGroupToXGroup}) {
+    if (*a) (void)hipFree(*a);
+    *a = nullptr;
+  }
+  return RBC_OK;
+}
+
+int rbc_plan_query(const float *x, uint32_t nx, uint32_t ldx, const uint32_t *xmap, uint32_t xmap_rows,
+                   uint32_t ldxmap, const float *q, uint32_t m, uint32_t ldq, uint32_t d, const uint32_t *qmap,
+                   uint32_t comp_length, const compPlan *plan, uint32_t K, uint32_t *ids, float *dists, uint32_t ldo,
+                   int mem_out, void *stream) {
+  RBC_ARG_CHECK(plan, "NULL compPlan");
+  RBC_ARG_CHECK(d >= 1 && d <= RBC_MAX_DIM, "dimension %u outside [1, %d]", d, RBC_MAX_DIM);
+  RBC_ARG_CHECK(K >= 1 && K <= KMAX, "K=%u outside [1, %d]", K, KMAX);
+  RBC_ARG_CHECK(ldo >= K, "output leading dimension %u < K=%u", ldo, K);
+  RBC_ARG_CHECK(ids && (dists || K == 1), "NULL output");
+  RBC_ARG_CHECK(ldx >= d && ldx % 4 == 0, "database leading dimension %u must be >= d and a multiple of 4", ldx);
+  if (m == 0) return RBC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  // The plan's query slots run in blocks of 16 that all take the group of the block's
+  // first slot (planNNKernel, kernels.cu:32-33); a partial last block is not run
+  // (planNNWrap, kernelWrap.cu:121-126).
+  const uint32_t slots = comp_length / BLOCK_SIZE * BLOCK_SIZE;
+  std::vector<uint32_t> hq(slots), hg(slots);
+  if (slots) {
+    RBC_ARG_CHECK(qmap && plan->qToQGroup, "NULL query map");
+    RBC_HIP_TRY(hipMemcpy(hq.data(), qmap, size_t(slots) * 4, hipMemcpyDeviceToHost));
+    RBC_HIP_TRY(hipMemcpy(hg.data(), plan->qToQGroup, size_t(slots) * 4, hipMemcpyDeviceToHost));
+  }
+  uint32_t G = 0;
+  for (uint32_t b = 0; b < slots; b += BLOCK_SIZE) G = std::max(G, hg[b] + 1);
+  const uint32_t ld = plan->ld;
+  std::vector<uint32_t> hng(G), hg2x(size_t(G) * ld), hgcx(size_t(G) * ld);
+  if (G) {
+    RBC_HIP_TRY(hipMemcpy(hng.data(), plan->numGroups, size_t(G) * 4, hipMemcpyDeviceToHost));
+    if (ld) {
+      RBC_HIP_TRY(hipMemcpy(hg2x.data(), plan->qGroupToXGroup, hg2x.size() * 4, hipMemcpyDeviceToHost));
+      RBC_HIP_TRY(hipMemcpy(hgcx.data(), plan->groupCountX, hgcx.size() * 4, hipMemcpyDeviceToHost));
+    }
+  }
+  // Queries of each group (slot order), each query at most once.
+  std::vector<std::vector<uint32_t>> members(G);
+  std::vector<char> seen(m, 0);
+  for (uint32_t b = 0; b < slots; b += BLOCK_SIZE)
+    for (uint32_t s = b; s < b + BLOCK_SIZE; ++s) {
+      const uint32_t qi = hq[s];
+      if (qi == DUMMY_IDX) continue;
+      RBC_ARG_CHECK(qi < m, "query map entry %u at slot %u >= %u queries", qi, s, m);
+      RBC_ARG_CHECK(!seen[qi], "query %u appears twice in the query map", qi);
+      seen[qi] = 1;
+      members[hg[b]].push_back(qi);
+    }
+  // Planned lists of each group, checked against the xMap's extent and contents.
+  std::vector<uint32_t> hxmap;
+  if (G && ld) {
+    RBC_ARG_CHECK(xmap, "NULL xMap");
+    hxmap.resize(size_t(xmap_rows) * ldxmap);
+    if (!hxmap.empty())
+      RBC_HIP_TRY(hipMemcpy(hxmap.data(), xmap, hxmap.size() * 4, hipMemcpyDeviceToHost));
+  }
+  std::vector<std::vector<PlanList>> lists(G);
+  for (uint32_t g = 0; g < G; ++g) {
+    if (members[g].empty()) continue;
+    RBC_ARG_CHECK(hng[g] <= ld, "group %u plans %u lists, more than the plan's ld %u", g, hng[g], ld);
+    for (uint32_t k = 0; k < hng[g]; ++k) {
+      const uint32_t xg = hg2x[size_t(g) * ld + k], cnt = hgcx[size_t(g) * ld + k];
+      RBC_ARG_CHECK(xg < xmap_rows, "group %u plans list %u of %u", g, xg, xmap_rows);
+      RBC_ARG_CHECK(cnt <= ldxmap, "list %u count %u exceeds the xMap row length %u", xg, cnt, ldxmap);
+      for (uint32_t p = 0; p < cnt; ++p)
+        RBC_ARG_CHECK(hxmap[size_t(xg) * ldxmap + p] < nx, "xMap[%u][%u] = %u >= %u points", xg, p,
+                      hxmap[size_t(xg) * ldxmap + p], nx);
+      lists[g].push_back(PlanList{xg * ldxmap, cnt});
+    }
+  }
+  std::vector<uint32_t> order, hqoff(G + 1, 0);
+  for (uint32_t g = 0; g < G; ++g) {
+    order.insert(order.end(), members[g].begin(), members[g].end());
+    hqoff[g + 1] = (uint32_t)order.size();
+  }
+  DevMat Q;
+  RBC_TRY(Q.stage(q, m, d, ldq, RBC_MEM_DEVICE, st));
+  DevBuf<uint32_t> qorder, oi;
+  DevBuf<float> od;
+  RBC_TRY(qorder.alloc(std::max<size_t>(order.size(), 1)));
+  if (!order.empty()) RBC_TRY(upload_host(qorder.get(), order.data(), order.size() * 4, st));
+  RBC_TRY(od.alloc(size_t(m) * K));
+  RBC_TRY(oi.alloc(size_t(m) * K));
+  TileSrc src{x, ldx, xmap, nullptr, Q.p, Q.ld, qorder.get(), d, (d + 3) & ~3u};
+  RBC_TRY(oneshot_query_plan(src, m, K, hqoff, lists, metric_setting(), od.get(), oi.get(), st));
+  RBC_TRY(rows_out(ids, size_t(ldo) * 4, oi.get(), size_t(K) * 4, size_t(K) * 4, m, mem_out, st));
+  if (dists) RBC_TRY(rows_out(dists, size_t(ldo) * 4, od.get(), size_t(K) * 4, size_t(K) * 4, m, mem_out, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  return RBC_OK;
+}
+
+int rbc_setup_reps(const float *x, uint32_t n, uint32_t d, uint32_t ldx, const uint32_t *rep_ids, uint32_t R,
+                   matrix *out) {
+  RBC_ARG_CHECK(out && x && rep_ids, "NULL argument");
+  RBC_ARG_CHECK(ldx >= d && d >= 1, "leading dimension %u < dimension %u", ldx, d);
+  RBC_ARG_CHECK(R >= 1 && R <= n, "num_reps=%u outside [1, n=%u]", R, n);
+  const uint32_t pd = pad16(d), pR = pad16(R);
+  std::vector<float> h(size_t(pR) * pd, 0.f);
+  for (uint32_t i = 0; i < R; ++i) {
+    RBC_ARG_CHECK(rep_ids[i] < n, "rep id %u >= n=%u", rep_ids[i], n);
+    std::copy(x + size_t(rep_ids[i]) * ldx, x + size_t(rep_ids[i]) * ldx + d, h.begin() + size_t(i) * pd);
+  }
+  DevBuf<float> dr;
+  RBC_TRY(dr.alloc(h.size()));
+  RBC_TRY(upload_host(dr.get(), h.data(), h.size() * 4, nullptr));
+  *out = matrix{dr.p, R, d, pR, pd, pd};
+  dr.p = nullptr;
+  return RBC_OK;
+}
+
 }  // extern "C"
+

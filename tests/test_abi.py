@@ -57,3 +57,17 @@ def test_argument_errors_do_not_touch_the_gpu():
     # Null outputs of the peak probe.
     assert L.rbc_measure_peaks(None, None, None) == 1
     assert b"null" in L.rbc_last_error()
+
+
+def test_internals_reject_bad_arguments_before_the_gpu():
+    L = rbc.lib()
+    # rbc_dist_sub: d = 0; rbc_compute_reps: no reps; rbc_plan_init: NULL plan;
+    # rbc_plan_query: K above KMAX; rbc_setup_reps: NULL rows.
+    assert L.rbc_dist_sub(None, 4, 0, 16, None, 16, 4, 0, None, 16, None) == 1
+    assert L.rbc_compute_reps(None, 1, 4, None, 0, 4, 4, None, None, 0, 0, None) == 1
+    assert L.rbc_plan_init(None, None, 0, None, None, 0) == 1
+    assert L.rbc_plan_free(None) == 1
+    assert L.rbc_plan_query(None, 16, 16, None, 1, 16, None, 1, 16, 4, None, 16, None, 1, None, None, 1, 0,
+                            None) == 1
+    assert b"compPlan" in L.rbc_last_error()
+    assert L.rbc_setup_reps(None, 10, 4, 4, None, 2, None) == 1
