@@ -159,6 +159,47 @@ def bench_batched(args, rbc, datagen, torch, peaks):
         f"{first['tile_queries']} queries per tile)")
     log(f"graph replay: {1e3 * wall / args.steps:.3f} ms/step ({st['ms_graph']:.3f} ms on the GPU), "
         f"{qps:,.0f} q/s, replayed {bool(st['graph_replayed'])}, same results as the first call: {same}")
+    # Roofline of the persistent kernel: the FLOPs it evaluates (2 d per (query, point)
+    # and per (query, rep) distance, counted on the device) over the graph's hipEvent time
+    # (which also holds the re-rank kernel), against the FP32 peak: the kernel runs its
+    # distances on the FP32 vector ALUs.
+    gflop = st["flops"] / 1e9
+    ach = gflop / st["ms_graph"]  # TFLOP/s
+    roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(ach / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "batched_exact_kernel<0> + batched_select (one graph replay)",
+            "flops_per_launch": st["flops"], "pairs_per_launch": st["pairs_scan"],
+            "pairs_frac_of_brute": round(st["pairs_scan"] / float((ns.astype(np.float64) * ms).sum()), 4),
+            "peak_note": "FP32 vector FMA rate (the kernel's distances are FP32 VALU)",
+            "frac_of_measured_fp32": round(ach / peaks["fp32_fma_tflops"], 4)}
+    log(f"batched kernel: {st['pairs_scan']:.4e} pairs, {gflop:.1f} GFLOP in {st['ms_graph']:.2f} ms = "
+        f"{ach:.1f} TFLOP/s ({ach / FP32_PEAK_TFLOPS:.1%} of the FP32 peak)")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle  # CPU baseline leg
+
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                      len(os.sched_getaffinity(0)))
+        # A sample proportional to each instance's batch: its first ceil(m_i / frac) queries,
+        # through the CPU exact RBC over the instance's own lists.
+        frac = args.batched_cpu_frac
+        t_cpu, nq, agree = 0.0, 0, 0
+        for i in range(len(ns)):
+            s = int(np.ceil(int(ms[i]) / frac))
+            L = b.lists(i)
+            c = oracle.CpuRBC(xs[i], L["rep_ids"], L["perm"], L["offsets"], L["radii"], True, threads)
+            t0 = time.time()
+            ci, _, _ = c.knn(qs[i][:s], k, threads)
+            t_cpu += time.time() - t0
+            nq += s
+            agree += int((ci == ids[i][:s].cpu().numpy().astype(np.uint32)).all(1).sum())
+        cpu = {"value": round(nq / t_cpu, 2), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"first ceil(m_i/{frac}) queries of every instance ({nq} queries), CPU exact "
+                         f"RBC (triangle + Voronoi-plane pruning, AVX2 + OpenMP) over each instance's "
+                         f"lists, {t_cpu:.1f}s of search",
+               "ids_equal_gpu_sample": round(agree / nq, 6)}
+        log(f"cpu exact RBC {nq / t_cpu:.1f} q/s on {threads} threads over {nq} queries ({t_cpu:.1f}s)")
     line = {
         "metric": "exact k-NN queries/sec, 256 databases in one batched launch (k=16)",
         "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
@@ -168,8 +209,10 @@ def bench_batched(args, rbc, datagen, torch, peaks):
         "config": {"workload": BATCHED_WORKLOAD, "instances": len(ns), "n_total": int(ns.sum()),
                    "m_total": m_total, "k": k, "tile_queries": int(st["tile_queries"]),
                    "build_s": round(build_s, 2)},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": roof, "cpu_baseline": cpu,
         "graph_ms": round(st["ms_graph"], 3), "graph_replayed": bool(st["graph_replayed"]),
+        "autotune_ms": {"16": round(first["tune_ms"][0], 3), "32": round(first["tune_ms"][1], 3),
+                        "64": round(first["tune_ms"][2], 3)},
         "uncertified_queries": int(st["uncertified"]), "replays_match_first_call": same,
         "measured_peaks": peaks,
     }
@@ -297,10 +340,12 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 comparison step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
-    ap.add_argument("--cpu-sample", type=int, default=2048,
+    ap.add_argument("--cpu-sample", type=int, default=8192,
                     help="queries timed for the CPU exact RBC baseline")
     ap.add_argument("--cpu-sample-brute", type=int, default=256)
     ap.add_argument("--cpu-sample-1thread", type=int, default=8)
+    ap.add_argument("--batched-cpu-frac", type=int, default=32,
+                    help="batched CPU baseline: every instance's first ceil(m_i / this) queries")
     ap.add_argument("--check-sample", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")

@@ -328,6 +328,8 @@ typedef struct {
   uint32_t pad;
   double tune_ms[3];        /* autotuner timings for 16, 32, 64 queries per tile (first call) */
   double guard_fp64;        /* candidates ordered in FP64 by the near-tie guard */
+  double pairs_scan;        /* (query, point) distances the persistent kernel evaluated */
+  double flops;             /* 2 d per pair scanned plus 2 d per (query, rep) distance */
 } rbc_batched_stats;
 
 /* count indexes; x[i] has n[i] rows of d[i] floats with leading dimension ldx[i]; index
@@ -342,6 +344,9 @@ int rbc_batched_kquery(rbc_batched *h, const float *const *q, const uint32_t *m,
                        const uint32_t *ldq, uint32_t k, uint32_t *const *nn_ids,
                        float *const *nn_dists, const uint32_t *ldo, int mem, void *stream,
                        rbc_batched_stats *stats);
+/* The exact index of instance i (owned by h: valid until rbc_batched_destroy), for
+ * rbc_exact_get_info / rbc_exact_get_lists. */
+int rbc_batched_get_index(const rbc_batched *h, uint32_t i, const rbc_exact **out);
 int rbc_batched_destroy(rbc_batched *h);
 
 #ifdef __cplusplus

@@ -65,6 +65,7 @@ struct BatchArgs {
   float *cand_d;      // [mtot][Kp]
   uint32_t *cert;     // [mtot]
   uint32_t *qinst;    // [mtot] index of each global query
+  unsigned long long *stats;  // [4] (query, point) pairs scanned, [5] their FLOPs + the q->rep FLOPs
 };
 
 // Dense distances of the tile's queries to a block of reps into the scratch rows.
@@ -121,6 +122,7 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
       TileSrc rsrc{I.reps, I.ld, nullptr, nullptr, I.q, I.ldq, nullptr, I.d, I.dpad};
       EpiScratch es{scr, a.Rmax, 0};
       tile_steps<METRIC>(rsrc, 0, I.R, es, ctx, smem);
+      if (t == 0) atomicAdd(&a.stats[5], 2ull * I.d * T.qn * I.R);
     }
     __syncthreads();
     // Nearest rep per query (lowest index on ties).
@@ -197,7 +199,12 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
       }
       __syncthreads();
       if (!misc[1]) continue;
-      tile_steps<METRIC>(qsrc, I.list_off[j], I.list_off[j + 1] - I.list_off[j], epi, ctx, smem);
+      const uint32_t len = I.list_off[j + 1] - I.list_off[j];
+      tile_steps<METRIC>(qsrc, I.list_off[j], len, epi, ctx, smem);
+      if (t == 0) {
+        atomicAdd(&a.stats[4], (unsigned long long)T.qn * len);
+        atomicAdd(&a.stats[5], 2ull * I.d * T.qn * len);
+      }
       // Refresh each query's bound from its current k-th value.
       EpiTopK::L L = epi.lay(ctx);
       if (t < kTQ) {
@@ -278,9 +285,11 @@ __global__ __launch_bounds__(256) void batched_select(BSelArgs a) {
   }
 }
 
+constexpr int kBatchStats = 8;
+
 __global__ void reset_counter(uint32_t *c, unsigned long long *stats) {
   c[0] = 0;
-  stats[0] = stats[1] = stats[2] = stats[3] = 0;
+  for (int i = 0; i < kBatchStats; ++i) stats[i] = 0;
 }
 
 }  // namespace
@@ -353,6 +362,7 @@ int batch_launch(rbc_batched *b, uint32_t k, uint32_t Kp, hipStream_t st) {
   a.cand_d = b->cand_d.get();
   a.cert = b->cert.get();
   a.qinst = b->qinst.get();
+  a.stats = b->stats.get();
   const size_t lds = batch_lds_bytes(b->dpad_max, b->Rpad);
   if (b->metric == RBC_L1)
     hipLaunchKernelGGL(batched_exact_kernel<1>, dim3(b->grid), dim3(kNT), lds, st, a, b->dpad_max);
@@ -434,7 +444,7 @@ int rbc_batched_build(uint32_t count, const float *const *x, const uint32_t *n, 
   b->grid = (uint32_t)prop.multiProcessorCount;  // one resident workgroup per CU
   int s;
   if ((s = b->scratch.alloc(size_t(b->grid) * kTQ * b->Rmax)) || (s = b->counter.alloc(1)) ||
-      (s = b->stats.alloc(4))) {
+      (s = b->stats.alloc(kBatchStats))) {
     delete b;
     return s;
   }
@@ -588,7 +598,7 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
     (void)hipGraphExecDestroy(b->plan.exec);
     b->plan.exec = nullptr;
   }
-  unsigned long long hs[4];
+  unsigned long long hs[kBatchStats];
   RBC_HIP_TRY(hipMemcpy(hs, b->stats.get(), sizeof(hs), hipMemcpyDeviceToHost));
   if (hs[2]) {
     set_error("%llu queries have more than 64 candidates within the FP32 error bound of each other", hs[2]);
@@ -643,7 +653,16 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
     stats->tune_ms[1] = tune_ms[1];
     stats->tune_ms[2] = tune_ms[2];
     stats->guard_fp64 = (double)hs[0];
+    stats->pairs_scan = (double)hs[4];
+    stats->flops = (double)hs[5];
   }
+  return RBC_OK;
+}
+
+int rbc_batched_get_index(const rbc_batched *b, uint32_t i, const rbc_exact **out) {
+  RBC_ARG_CHECK(b && out, "NULL argument");
+  RBC_ARG_CHECK(i < b->idx.size(), "index %u of %zu", i, b->idx.size());
+  *out = b->idx[i];
   return RBC_OK;
 }
 

@@ -115,6 +115,7 @@ _SIGS = {
     "rbc_mlp_train": (_int, [_vp, _vp, _u32, _u32, _vp, _int, _u32, _u32, ctypes.c_float, _u64, _vp]),
     "rbc_batched_build": (_int, [_u32, _vp, _vp, _vp, _vp, _vp, _int, _u64, _int, ctypes.POINTER(_vp)]),
     "rbc_batched_kquery": (_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _int, _vp, _vp]),
+    "rbc_batched_get_index": (_int, [_vp, _u32, ctypes.POINTER(_vp)]),
     "rbc_batched_destroy": (_int, [_vp]),
 }
 
@@ -478,7 +479,8 @@ class ProposalMLP:
 class BatchedStats(ctypes.Structure):
     _fields_ = [("ms_graph", ctypes.c_double), ("tile_queries", _u32), ("graph_replayed", _u32),
                 ("uncertified", _u32), ("pad", _u32), ("tune_ms", ctypes.c_double * 3),
-                ("guard_fp64", ctypes.c_double)]
+                ("guard_fp64", ctypes.c_double), ("pairs_scan", ctypes.c_double),
+                ("flops", ctypes.c_double)]
 
 
 def _ptr_array(ptrs, ctype=_vp):
@@ -516,6 +518,22 @@ class BatchedRBC:
         stats = {n: getattr(st, n) for n, _ in st._fields_ if n != "tune_ms"}
         stats["tune_ms"] = list(st.tune_ms)
         return ids, ds, stats
+
+    def lists(self, i):
+        """Instance i's exact index lists (as ExactRBC.lists)."""
+        h = _vp()
+        _check(lib().rbc_batched_get_index(self.h, i, ctypes.byref(h)))
+        inf = ExactInfo()
+        _check(lib().rbc_exact_get_info(h, ctypes.byref(inf)))
+        R, n = inf.num_reps, inf.n
+        reps = np.empty(R, np.uint32)
+        off = np.empty(R + 1, np.uint32)
+        perm = np.empty(n, np.uint32)
+        owner = np.empty(n, np.uint32)
+        radii = np.empty(R, np.float32)
+        _check(lib().rbc_exact_get_lists(h, reps.ctypes.data, off.ctypes.data, perm.ctypes.data,
+                                         owner.ctypes.data, radii.ctypes.data))
+        return {"rep_ids": reps, "offsets": off, "perm": perm, "owner": owner, "radii": radii}
 
     def kquery_device(self, qs, k, ids, ds):
         """qs, ids, ds: lists of device tensors; returns the stats dict."""
