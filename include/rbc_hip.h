@@ -224,6 +224,15 @@ int rbc_exact_kquery(rbc_exact *h, const float *q, uint32_t m, uint32_t ldq, uin
                      int precision, uint32_t *nn_ids, float *nn_dists, uint32_t ldo, int mem,
                      void *stream, rbc_query_stats *stats);
 
+/* The query-to-representative distances of the exact query's first step (the role of
+ * computeReps' distance pass, rbc.cu:233-247): D[i * ldd + j] = distance value (squared for
+ * L2) of query i to rep j, as the query computes them for `precision` -- FP32 on the vector
+ * ALUs, or split-BF16 MFMA for RBC_PREC_BF16 (L2) -- with |D - exact| <= *rel_err * exact
+ * + abs_err[i] (rel_err 0 for BF16; abs_err 0 for FP32; either may be NULL). */
+int rbc_exact_rep_dists(rbc_exact *h, const float *q, uint32_t m, uint32_t ldq, int precision,
+                        float *D, uint32_t ldd, float *abs_err, double *rel_err, int mem,
+                        void *stream);
+
 typedef struct {
   uint32_t n, d, ld, num_reps, metric;
   uint32_t max_list, min_list;

@@ -26,6 +26,12 @@ struct rbc_exact {
   rbc::DevBuf<uint16_t> reps_b;   // R x ld BF16 copy
   rbc::DevBuf<float> reps_bnorm;  // ||bf16(r)||^2
   rbc::DevBuf<float> reps_qerr;   // ||r - bf16(r)||, rounded up
+  // L2, BF16 filter: the reps split as r = hi + lo + delta (hi = reps_b, lo = bf16(r - hi),
+  // R x ldb) for the split-BF16 MFMA query-rep distances, ||r||^2 (FP64 sum rounded to
+  // nearest) and the largest ||r|| (rounded up).
+  rbc::DevBuf<uint16_t> reps_lo;
+  rbc::DevBuf<float> reps_n2;
+  double rep_nmax = 0.0;
   rbc::DevBuf<uint32_t> rep_ids;  // R
   rbc::DevBuf<float> radius;      // R, metric distance, rounded up
   rbc::DevBuf<float> rr;          // R x R FP32 rep-rep distances (metric value)
@@ -62,6 +68,8 @@ inline double fma_rel_err(uint32_t dpad) { return (double(dpad) + 4.0) * 0x1p-24
 
 // Fills h->rr_den from h->rr (exact_build.hip).
 int voronoi_denominators(rbc_exact *h, hipStream_t st);
+// Fills h->reps_lo, h->reps_n2 and h->rep_nmax from h->reps and h->reps_b (exact_build.hip).
+int rep_splits(rbc_exact *h, hipStream_t st);
 // Relative error bound of the tree-ordered re-rank over P (power of two) coordinates.
 inline double tree_rel_err(uint32_t P) {
   int lg = 0;

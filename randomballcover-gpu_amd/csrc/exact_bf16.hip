@@ -36,13 +36,16 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
   return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
-// One wave per query: BF16 copy (zero padded to ldb), |q~|^2 and |q - q~| (FP64).
+// One wave per query: BF16 copy (zero padded to ldb), |q~|^2 and |q - q~| (FP64); with
+// qn2, also ||q||^2 (FP64 sum rounded to nearest) and the absolute error bound of the
+// split-BF16 query-rep distances of this query (qr_error).
 __global__ __launch_bounds__(256) void prep_queries(const float *q, size_t ldq, uint32_t m, uint32_t d,
-                                                    uint32_t ldb, uint16_t *qb, float *qn, float *qe) {
+                                                    uint32_t ldb, uint16_t *qb, float *qn, float *qe, float *qn2,
+                                                    float *qerr, double c_split, double rep_nmax) {
   const uint32_t r = blockIdx.x * 4 + threadIdx.x / kWave;
   if (r >= m) return;
   const int l = lane_id();
-  double nb = 0.0, e2 = 0.0;
+  double nb = 0.0, e2 = 0.0, n2 = 0.0;
   for (uint32_t c = l; c < ldb; c += kWave) {
     float v = c < d ? q[size_t(r) * ldq + c] : 0.f;
     uint16_t h = f32_to_bf16_rne(v);
@@ -51,16 +54,133 @@ __global__ __launch_bounds__(256) void prep_queries(const float *q, size_t ldq, 
     nb += hv * hv;
     double e = (double)v - hv;
     e2 += e * e;
+    n2 += (double)v * (double)v;
   }
   for (int s = kWave / 2; s > 0; s >>= 1) {
     nb += __shfl_xor(nb, s, kWave);
     e2 += __shfl_xor(e2, s, kWave);
+    n2 += __shfl_xor(n2, s, kWave);
   }
   if (l == 0) {
     qn[r] = (float)nb;
     float ef = (float)sqrt(e2);
     if ((double)ef < sqrt(e2)) ef = nextafterf(ef, FLT_MAX);
     qe[r] = ef;
+    if (qn2) {
+      qn2[r] = (float)n2;
+      // |D~ - D| for every rep r (qr_mfma): the dot product's error 2 c |q| |r|, the
+      // rounding of the two stored norms (u each) and of the epilogue's add and FMA
+      // (u (|q|^2 + |r|^2) and u (|q|^2 + |r|^2 + 2 |q.r~|)), with |r| <= rep_nmax.
+      const double u = 0x1p-24, nq = sqrt(n2), nr = rep_nmax;
+      const double E = (2.0 * c_split * nq * nr + 4.0 * u * (n2 + nr * nr) + 2.0 * u * (1.0 + c_split) * nq * nr) *
+                           1.01 + 1e-30;
+      float Ef = (float)E;
+      if ((double)Ef < E) Ef = nextafterf(Ef, FLT_MAX);
+      qerr[r] = Ef;
+    }
+  }
+}
+
+// ---- query -> rep distances on MFMA (L2) --------------------------------------------
+// D~[q][r] = |q|^2 + |r|^2 - 2 q.r with q.r from split BF16 operands: q = qh + ql + dq,
+// r = rh + rl + dr (hi = bf16(v), lo = bf16(v - hi), |d| <= 2^-16 |v| per coordinate), and
+// q.r ~ qh.rh + qh.rl + ql.rh on three v_mfma_f32_16x16x32_bf16 per tile and k-step.  The
+// dropped terms are at most 3.1 2^-16 sum |q_i r_i|, the FP32 accumulation of the 3 K
+// exact products at most gamma_{3K+1} (u' = 2^-23) of their absolute sum (<= 1.02 sum
+// |q_i r_i|): together c_split |q| |r| (qr_split_err).  A = 16 reps, B = 16 queries, so
+// lane l holds reps 4 (l/16) .. +3 of query l % 16 and writes them as one 16-byte run of
+// D's row.  A wave computes 64 reps x 64 queries, a block of four waves 128 x 128; the
+// 16 rep tiles of a query tile run on one XCD (block b on XCD b % 8), so the query rows
+// are fetched from HBM once.  The reps' hi/lo rows (R x ldb, zero padded) stay in L2.
+constexpr int kQM = 128;  // queries and reps per block
+
+template <int KS, bool VQ>
+__global__ __launch_bounds__(256) void qr_mfma(const float *Q, size_t ldq, uint32_t m, uint32_t d,
+                                               const uint16_t *rhi, const uint16_t *rlo, uint32_t ldb,
+                                               const float *rn2, uint32_t R, const float *qn2, float *D,
+                                               uint32_t nqt, uint32_t nrt) {
+  const uint32_t b = blockIdx.x, xcd = b % kXCD, s = b / kXCD;
+  const uint32_t rt = s % nrt, qt = (s / nrt) * kXCD + xcd;
+  if (qt >= nqt) return;
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const uint32_t q0 = qt * kQM + (w / 2) * 64, r0 = rt * kQM + (w % 2) * 64;
+  const uint32_t koff = (l / 16) * 8;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const uint32_t c0 = ks * 32 + koff;
+    bf16x8 Ah[4], Al[4], Bh[4], Bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t r = r0 + i * 16 + (l % 16);
+      if (r < R) {
+        Ah[i] = *reinterpret_cast<const bf16x8 *>(rhi + size_t(r) * ldb + c0);
+        Al[i] = *reinterpret_cast<const bf16x8 *>(rlo + size_t(r) * ldb + c0);
+      } else {
+        Ah[i] = bf16x8{};
+        Al[i] = bf16x8{};
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t q = q0 + j * 16 + (l % 16);
+      float v[8];
+      if (q < m && VQ && c0 + 8 <= d) {
+        const float4 *src = reinterpret_cast<const float4 *>(Q + size_t(q) * ldq + c0);
+        const float4 a = src[0], bb = src[1];
+        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = bb.x, v[5] = bb.y, v[6] = bb.z, v[7] = bb.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (q < m && c0 + e < d) ? Q[size_t(q) * ldq + c0 + e] : 0.f;
+      }
+      typedef __attribute__((__vector_size__(8 * sizeof(uint16_t)))) uint16_t u16x8;
+      u16x8 hv, lv;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const uint16_t h = f32_to_bf16_rne(v[e]);
+        hv[e] = h;
+        lv[e] = f32_to_bf16_rne(v[e] - __uint_as_float(uint32_t(h) << 16));
+      }
+      Bh[j] = __builtin_bit_cast(bf16x8, hv);
+      Bl[j] = __builtin_bit_cast(bf16x8, lv);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[i], Bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[i], Bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[i], Bh[j], acc[i][j], 0, 0, 0);
+      }
+  }
+  // Epilogue: lane l holds reps r0 + 16 i + 4 (l/16) + e of query q0 + 16 j + l % 16.
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t q = q0 + j * 16 + (l % 16);
+    if (q >= m) continue;
+    const float nq = qn2[q];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t rb = r0 + i * 16 + (l / 16) * 4;
+      float o[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float nr = rb + e < R ? rn2[rb + e] : 0.f;
+        o[e] = fmaxf(__builtin_fmaf(-2.f, acc[i][j][e], nq + nr), 0.f);
+      }
+      float *dst = D + size_t(q) * R + rb;
+      if (rb + 3 < R && (R % 4) == 0) {
+        *reinterpret_cast<float4 *>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (rb + e < R) dst[e] = o[e];
+      }
+    }
   }
 }
 
@@ -682,8 +802,41 @@ void launch_probe(const Bf16ProbeArgs &a, uint32_t nitems, hipStream_t st) {
 }  // namespace
 
 int bf16_prepare_queries(rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint16_t *qb, float *qn,
-                         float *qe, hipStream_t st) {
-  hipLaunchKernelGGL(prep_queries, dim3(cdiv(m, 4)), dim3(256), 0, st, q, ldq, m, h->d, h->ldb, qb, qn, qe);
+                         float *qe, hipStream_t st, float *qn2, float *qerr) {
+  hipLaunchKernelGGL(prep_queries, dim3(cdiv(m, 4)), dim3(256), 0, st, q, ldq, m, h->d, h->ldb, qb, qn, qe, qn2, qerr,
+                     qr_split_err(h), h->rep_nmax);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+double qr_split_err(const rbc_exact *h) {
+  const double up = 0x1p-23, N = 3.0 * h->ldb + 1.0;
+  return 3.1 * 0x1p-16 + 1.02 * N * up / (1.0 - N * up);
+}
+
+template <int KS>
+static void launch_qr(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, const float *qn2, float *D, bool vq,
+                      uint32_t grid, uint32_t nqt, uint32_t nrt, hipStream_t st) {
+  if (vq)
+    hipLaunchKernelGGL((qr_mfma<KS, true>), dim3(grid), dim3(256), 0, st, q, ldq, m, h->d, h->reps_b.get(),
+                       h->reps_lo.get(), h->ldb, h->reps_n2.get(), h->R, qn2, D, nqt, nrt);
+  else
+    hipLaunchKernelGGL((qr_mfma<KS, false>), dim3(grid), dim3(256), 0, st, q, ldq, m, h->d, h->reps_b.get(),
+                       h->reps_lo.get(), h->ldb, h->reps_n2.get(), h->R, qn2, D, nqt, nrt);
+}
+
+int bf16_qr_dist(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, const float *qn2, float *D,
+                 hipStream_t st) {
+  RBC_ARG_CHECK(h->metric == RBC_L2 && h->ldb <= 128 && h->reps_lo.get(), "split-BF16 q->rep distances need L2, d <= 128");
+  const uint32_t nqt = cdiv(m, kQM), nrt = cdiv(h->R, kQM);
+  const uint32_t grid = cdiv(nqt, kXCD) * kXCD * nrt;
+  const bool vq = ldq % 4 == 0 && reinterpret_cast<uintptr_t>(q) % 16 == 0;
+  switch (h->ldb / 32) {
+    case 1: launch_qr<1>(h, q, ldq, m, qn2, D, vq, grid, nqt, nrt, st); break;
+    case 2: launch_qr<2>(h, q, ldq, m, qn2, D, vq, grid, nqt, nrt, st); break;
+    case 3: launch_qr<3>(h, q, ldq, m, qn2, D, vq, grid, nqt, nrt, st); break;
+    default: launch_qr<4>(h, q, ldq, m, qn2, D, vq, grid, nqt, nrt, st); break;
+  }
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }

@@ -160,6 +160,22 @@ __global__ void reps_bf16(const float *reps, uint32_t R, uint32_t d, uint32_t ld
   rqe[j] = qf;
 }
 
+// Second BF16 terms of the reps (lo = bf16(r - hi), zero padded to ldb) and ||r||^2 (FP64
+// sum, rounded to nearest).
+__global__ void reps_split(const float *reps, const uint16_t *rb, uint32_t R, uint32_t d, uint32_t ld,
+                           uint32_t ldb, uint16_t *rlo, float *rn2) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= R) return;
+  double n2 = 0.0;
+  for (uint32_t c = 0; c < ldb; ++c) {
+    const float v = c < d ? reps[size_t(j) * ld + c] : 0.f;
+    const float hi = bf16_to_f32(rb[size_t(j) * ldb + c]);
+    rlo[size_t(j) * ldb + c] = f32_to_bf16_rne(v - hi);  // v - hi is exact in FP32
+    n2 += (double)v * (double)v;
+  }
+  rn2[j] = (float)n2;
+}
+
 uint64_t splitmix64(uint64_t &s) {
   uint64_t z = (s += 0x9E3779B97F4A7C15ull);
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -168,6 +184,23 @@ uint64_t splitmix64(uint64_t &s) {
 }
 
 }  // namespace
+
+int rep_splits(rbc_exact *h, hipStream_t st) {
+  const uint32_t R = h->R;
+  RBC_TRY(h->reps_lo.alloc(size_t(R) * h->ldb));
+  RBC_TRY(h->reps_n2.alloc(R));
+  hipLaunchKernelGGL(reps_split, dim3(cdiv(R, 64)), dim3(64), 0, st, h->reps.get(), h->reps_b.get(), R, h->d,
+                     h->ld, h->ldb, h->reps_lo.get(), h->reps_n2.get());
+  RBC_HIP_TRY(hipGetLastError());
+  std::vector<float> n2(R);
+  RBC_HIP_TRY(hipMemcpyAsync(n2.data(), h->reps_n2.get(), size_t(R) * 4, hipMemcpyDeviceToHost, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  double mx = 0.0;
+  for (float v : n2) mx = std::max(mx, (double)v);
+  // The stored ||r||^2 is within 2^-24 (relative) of the FP64 sum.
+  h->rep_nmax = std::sqrt(mx * (1.0 + 0x1p-23)) * (1.0 + 1e-12);
+  return RBC_OK;
+}
 
 int voronoi_denominators(rbc_exact *h, hipStream_t st) {
   const size_t n = size_t(h->R) * h->R;
@@ -305,6 +338,7 @@ int rbc_exact_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32
     RBC_HIP_TRY(hipStreamSynchronize(st));
   }
   RBC_TRY(voronoi_denominators(h, st));
+  RBC_TRY(rep_splits(h, st));
   h->h_list_off.resize(R + 1);
   RBC_HIP_TRY(hipMemcpy(h->h_list_off.data(), h->list_off.get(), (R + 1) * 4, hipMemcpyDeviceToHost));
   RBC_HIP_TRY(hipDeviceSynchronize());

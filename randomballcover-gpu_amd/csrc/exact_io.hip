@@ -244,6 +244,7 @@ int rbc_exact_load(const char *path, rbc_exact **out) {
   RBC_ARG_CHECK(fread(&extra, 1, 1, fp) == 0, "index file has data after its checksum");
   RBC_TRY(check_structure(h->n, h->R, perm, inv, owner, off, len, rep_ids, radius, qerr, nmax));
   RBC_TRY(voronoi_denominators(h.get(), nullptr));
+  RBC_TRY(rep_splits(h.get(), nullptr));
   h->h_list_off = std::move(off);
   *out = h.release();
   return RBC_OK;

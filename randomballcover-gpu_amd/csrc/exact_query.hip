@@ -1,7 +1,9 @@
 // Exact k-NN query over an rbc_exact index.
 //
 // Pipeline (one stream; the proposal network's inference runs beside step 1 on a second):
-//  1. q->reps   : FP32 distances of every query to every representative.
+//  1. q->reps   : distances of every query to every representative: FP32 on the vector
+//                 ALUs (relative error e), or, for the BF16 filter, split-BF16 MFMA tiles
+//                 with a per-query absolute error bound (exact_bf16.hip, qr_mfma).
 //  2. row stats : nearest rep of each query and an upper bound on its k-th smallest rep
 //                 distance (reps are database points, so that bounds the k-th NN distance);
 //                 BF16: also the reps of its smallest lane minima (probe lists, plane reps).
@@ -174,6 +176,7 @@ struct BoundArgs {
   const float *rep_kth;          // upper bound on the k-th smallest FP32 rep distance (FLT_MAX if R < k)
   const uint32_t *near, *key1, *list_len;
   double e_probe, e_rep;         // relative error of the probe values and of rep_kth
+  const float *qerr;             // per query: absolute error of the rep distances (NULL: e_rep)
   int metric;
   float *tau;
   double *gam;
@@ -202,7 +205,9 @@ __global__ void probe_bounds(BoundArgs b) {
     }
   }
   double gp = kth >= FLT_MAX ? INFINITY : (double)kth / (1.0 - b.e_probe);
-  double gr = b.rep_kth[q] >= FLT_MAX ? INFINITY : (double)b.rep_kth[q] / (1.0 - b.e_rep);
+  double gr = INFINITY;
+  if (b.rep_kth[q] < FLT_MAX)
+    gr = b.qerr ? (double)b.rep_kth[q] + (double)b.qerr[q] : (double)b.rep_kth[q] / (1.0 - b.e_rep);
   double ghi = fmin(gp, gr) * (1.0 + 1e-12);
   double t = ghi * (1.0 + b.e_rep) * (1.0 + 1e-12);
   float tf = t >= (double)FLT_MAX ? INFINITY : (float)t;
@@ -351,7 +356,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
                                                   const uint32_t *near,
                                                   const uint32_t *key1, const double *gam,
                                                   const uint32_t *cert, const float *radius,
-                                                  const float *rr_den, double e, int metric,
+                                                  const float *rr_den, double e, const float *qerr, int metric,
                                                   const uint32_t *planeR, uint32_t P,
                                                   uint32_t Wq, uint32_t *bits, uint32_t *list_cnt) {
   static_assert(kPlanCQ == 64, "four waves of 16 queries make two 32-query words");
@@ -367,6 +372,10 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
     const uint32_t j0 = near[q];
     const uint32_t j1 = key1 ? key1[q] : R;
     const uint32_t cb = cert[q];
+    const double Eq = qerr ? (double)qerr[q] : 0.0;  // wave-uniform, loaded with the others
+    const double Eud = Eq * (1.0 + 0x1p-21);
+    float Eu = (float)Eud;
+    if ((double)Eu < Eud) Eu = nextafterf(Eu, INFINITY);
     bool keep = false;
     if (j < R) {
       if (j == j0) {
@@ -381,10 +390,12 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
           // bound sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as
           // Dqj (1-e) <= (G' + psi)^2, and each plane bound num / den <= G' as
           // num <= G' den with den rounded up; both keep at least what it keeps.
+          // With split-BF16 rep distances, D~ is within qerr[q] of the true value
+          // (absolute), and Dlo and the upper values B_i below carry that term too.
           const double Gp = G * (1.0 + 2e-12);
           const double Gd = Gp * (1.0 + 1e-13);
           const double rhs = Gp + psi;
-          const double Dlo = Dqj * (1.0 - e);
+          const double Dlo = Dqj * (1.0 - e) - Eq;
           keep = Dlo <= rhs * rhs * (1.0 + 1e-13);
           // The planes only matter for lists the triangle bound keeps; their rows of
           // rr_den are not read for the others.
@@ -411,8 +422,9 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
             }
             // The plane test num > G' den in FP32, with every rounding pushed towards
             // keeping: A <= Dlo - slack (rounded down), and fl(Gf den + B_i) >= G' den +
-            // Dqi (1+e), since Gf >= G' (1 + 8u) and B_i = fl(Dqi fl(1 + e + 8u)) >=
-            // Dqi (1+e)(1 + 6u) (u = 2^-24).  A > that implies the exact test.
+            // Dqi (1+e) + Eq, since Gf >= G' (1 + 8u) and B_i = fl(Dqi fl(1 + e + 8u) + Eu)
+            // >= (Dqi (1+e) + Eq)(1 + 6u) with Eu >= Eq (1 + 8u) (u = 2^-24).  A > that
+            // implies the exact test.
             const double Ad = Dlo - slack;
             float A = (float)Ad;
             if ((double)A > Ad) A = nextafterf(A, -INFINITY);
@@ -422,7 +434,8 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
             const float cb1 = (float)(1.0 + e + 0x1p-21);
             bool far = false;
 #pragma unroll
-            for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) far |= A > __builtin_fmaf(Gf, den[pi], Dqi[pi] * cb1);
+            for (uint32_t pi = 0; pi <= kPlaneReps; ++pi)
+              far |= A > __builtin_fmaf(Gf, den[pi], __builtin_fmaf(Dqi[pi], cb1, Eu));
             if (far) keep = false;
           }
         } else {
@@ -617,6 +630,39 @@ struct Timer {
   }
 };
 
+// Step 1 (q -> reps) into D (m x R).  BF16 filter: also the queries' BF16 copies (qb,
+// qn, qe) and, for L2, the split-BF16 MFMA distances with their per-query absolute error
+// bounds (qn2, qerr); otherwise FP32 VALU distances (relative error fma_rel_err(dpad)).
+static int rep_distances(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, int precision, float *Dqr,
+                         uint16_t *qb, float *qn, float *qe, float *qn2, float *qerr, hipStream_t st) {
+  const uint32_t R = h->R, d = h->d;
+  const bool qr_split = precision == RBC_PREC_BF16 && h->metric == RBC_L2;
+  if (precision == RBC_PREC_BF16) {
+    RBC_TRY(bf16_prepare_queries(h, qd, ldq, m, qb, qn, qe, st, qn2, qerr));
+    if (qr_split) RBC_TRY(bf16_qr_dist(h, qd, ldq, m, qn2, Dqr, st));
+  }
+  if (!qr_split) {
+    const dim3 g(cdiv(m, kQRT), cdiv(R, kQRT));
+    // 16-byte query loads need 16-byte aligned rows (the reps are padded to 16 floats).
+    const bool vq = ldq % 4 == 0 && reinterpret_cast<uintptr_t>(qd) % 16 == 0;
+    const float *rp = h->reps.get();
+    const size_t ldr = h->ld;
+    if (h->metric == RBC_L2) {
+      if (vq)
+        hipLaunchKernelGGL((qr_dist<0, true>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
+      else
+        hipLaunchKernelGGL((qr_dist<0, false>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
+    } else {
+      if (vq)
+        hipLaunchKernelGGL((qr_dist<1, true>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
+      else
+        hipLaunchKernelGGL((qr_dist<1, false>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
+    }
+    RBC_HIP_TRY(hipGetLastError());
+  }
+  return RBC_OK;
+}
+
 int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, uint32_t k, int precision,
                        float *od, uint32_t *oi, uint32_t ldo, hipStream_t st, rbc_query_stats *stats) {
   const uint32_t R = h->R, d = h->d;
@@ -647,28 +693,28 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     RBC_TRY(mlp_propose_lists(h->proposal, qd, ldq, m, Pprop, h->prop_mass, plists, pcnt, h->side));
     RBC_HIP_TRY(hipEventRecord(h->ev_out, h->side));
   }
-  // 1. q -> reps
+  // 1. q -> reps.  BF16 filter, L2: split-BF16 MFMA distances with a per-query absolute
+  // error bound (qerr); otherwise FP32 VALU distances (relative error e).
   RBC_TRY(ws.get(kQ + 0, size_t(m) * R * 4, &p));
   float *Dqr = static_cast<float *>(p);
-  {
-    const dim3 g(cdiv(m, kQRT), cdiv(R, kQRT));
-    // 16-byte query loads need 16-byte aligned rows (the reps are padded to 16 floats).
-    const bool vq = ldq % 4 == 0 && reinterpret_cast<uintptr_t>(qd) % 16 == 0;
-    const float *rp = h->reps.get();
-    const size_t ldr = h->ld;
-    if (h->metric == RBC_L2) {
-      if (vq)
-        hipLaunchKernelGGL((qr_dist<0, true>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
-      else
-        hipLaunchKernelGGL((qr_dist<0, false>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
-    } else {
-      if (vq)
-        hipLaunchKernelGGL((qr_dist<1, true>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
-      else
-        hipLaunchKernelGGL((qr_dist<1, false>), g, dim3(256), 0, st, qd, ldq, m, rp, ldr, R, d, Dqr);
-    }
-    RBC_HIP_TRY(hipGetLastError());
+  uint16_t *qb = nullptr;
+  float *qn = nullptr, *qe = nullptr, *qerr = nullptr;
+  if (precision == RBC_PREC_BF16) {
+    RBC_TRY(ws.get(kQ + 20, size_t(m) * h->ldb * 2, &p));
+    qb = static_cast<uint16_t *>(p);
+    RBC_TRY(ws.get(kQ + 21, size_t(m) * 4, &p));
+    qn = static_cast<float *>(p);
+    RBC_TRY(ws.get(kQ + 22, size_t(m) * 4, &p));
+    qe = static_cast<float *>(p);
   }
+  float *qn2 = nullptr;
+  if (precision == RBC_PREC_BF16 && h->metric == RBC_L2) {
+    RBC_TRY(ws.get(kQ + 38, size_t(m) * 4, &p));
+    qn2 = static_cast<float *>(p);
+    RBC_TRY(ws.get(kQ + 39, size_t(m) * 4, &p));
+    qerr = static_cast<float *>(p);
+  }
+  RBC_TRY(rep_distances(h, qd, ldq, m, precision, Dqr, qb, qn, qe, qn2, qerr, st));
   const bool bf16 = precision == RBC_PREC_BF16;
   // The BF16 probe keeps at most 32 candidates, so it serves k <= 32.  It scans the Pp
   // lists of the smallest lane minima of each query's rep distances in one pass.
@@ -700,17 +746,6 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   // BF16 MFMA tiles keeping the K' best approximate distances per (query, list); the
   // query's K' best over its lists are re-ranked in FP32.  Only the bound is used from it:
   // the scan visits every surviving list, probed or not.
-  uint16_t *qb = nullptr;
-  float *qn = nullptr, *qe = nullptr;
-  if (bf16) {
-    RBC_TRY(ws.get(kQ + 20, size_t(m) * h->ldb * 2, &p));
-    qb = static_cast<uint16_t *>(p);
-    RBC_TRY(ws.get(kQ + 21, size_t(m) * 4, &p));
-    qn = static_cast<float *>(p);
-    RBC_TRY(ws.get(kQ + 22, size_t(m) * 4, &p));
-    qe = static_cast<float *>(p);
-    RBC_TRY(bf16_prepare_queries(h, qd, ldq, m, qb, qn, qe, st));
-  }
   DevBuf<ScanItem> pitems;
   uint32_t npitems = 0;
   // Runs one probe slot for the queries whose key (list) is < R.
@@ -805,6 +840,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     ba.list_len = h->list_len.get();
     ba.e_probe = bprobe ? tree_rel_err(tree_width(d)) : e;
     ba.e_rep = e;
+    ba.qerr = qerr;
     ba.metric = h->metric;
     ba.no_cert = bprobe;
     ba.tau = tau;
@@ -825,7 +861,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
   hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanCQ), cdiv(R, kWave)), dim3(256), 0, st, Dqr, m, R, near,
-                     key1, gam, cert, h->radius.get(), h->rr_den.get(), e, h->metric, planeR, P2, Wq,
+                     key1, gam, cert, h->radius.get(), h->rr_den.get(), e, qerr, h->metric, planeR, P2, Wq,
                      bits, lcnt);
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, kSlotPlanScan, st));
@@ -990,6 +1026,42 @@ extern "C" int rbc_exact_set_proposal(rbc_exact *h, rbc_mlp *mlp) {
                   din, nout, h->d, h->R);
   }
   h->proposal = mlp;
+  return RBC_OK;
+}
+
+extern "C" int rbc_exact_rep_dists(rbc_exact *h, const float *q, uint32_t m, uint32_t ldq, int precision, float *D,
+                                   uint32_t ldd, float *abs_err, double *rel_err, int mem, void *stream) {
+  RBC_ARG_CHECK(h, "NULL handle");
+  RBC_ARG_CHECK(precision == RBC_PREC_FP32 || precision == RBC_PREC_BF16, "unknown precision %d", precision);
+  RBC_ARG_CHECK(precision == RBC_PREC_FP32 || h->metric == RBC_L2, "BF16 filter needs the L2 metric");
+  RBC_ARG_CHECK(D && ldd >= h->R, "NULL output or leading dimension %u < %u reps", ldd, h->R);
+  const bool split = precision == RBC_PREC_BF16;
+  if (rel_err) *rel_err = split ? 0.0 : fma_rel_err(h->dpad);
+  if (m == 0) return RBC_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DevMat Q;
+  RBC_TRY(Q.stage(q, m, h->d, ldq, mem, st));
+  DevBuf<float> Dd, qn, qe, qn2, qerr;
+  DevBuf<uint16_t> qb;
+  RBC_TRY(Dd.alloc(size_t(m) * h->R));
+  if (split) {
+    RBC_TRY(qb.alloc(size_t(m) * h->ldb));
+    RBC_TRY(qn.alloc(m));
+    RBC_TRY(qe.alloc(m));
+    RBC_TRY(qn2.alloc(m));
+    RBC_TRY(qerr.alloc(m));
+  }
+  RBC_TRY(rep_distances(h, Q.p, Q.ld, m, precision, Dd.get(), qb.get(), qn.get(), qe.get(), qn2.get(), qerr.get(), st));
+  RBC_TRY(rows_out(D, size_t(ldd) * 4, Dd.get(), size_t(h->R) * 4, size_t(h->R) * 4, m, mem, st));
+  if (abs_err) {
+    if (split)
+      RBC_TRY(rows_out(abs_err, 4, qerr.get(), 4, 4, m, mem, st));
+    else if (mem == RBC_MEM_HOST)
+      std::fill(abs_err, abs_err + m, 0.f);
+    else
+      RBC_TRY(fill_f32(abs_err, 0.f, m, st));
+  }
+  RBC_HIP_TRY(hipStreamSynchronize(st));
   return RBC_OK;
 }
 

@@ -92,6 +92,8 @@ _SIGS = {
     "rbc_exact_kquery": (_int, [_vp, _vp, _u32, _u32, _u32, _int, _vp, _vp, _u32, _int, _vp,
                                 ctypes.POINTER(QueryStats)]),
     "rbc_exact_get_info": (_int, [_vp, ctypes.POINTER(ExactInfo)]),
+    "rbc_exact_rep_dists": (_int, [_vp, _vp, _u32, _u32, _int, _vp, _u32, _vp, ctypes.POINTER(ctypes.c_double),
+                                   _int, _vp]),
     "rbc_exact_get_lists": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rbc_exact_destroy": (_int, [_vp]),
     "rbc_exact_save": (_int, [_vp, ctypes.c_char_p]),
@@ -352,6 +354,18 @@ class ExactRBC:
         if mass is not None:
             _check(lib().rbc_exact_set_proposal_mass(self.h, mass))
         self._proposal = mlp
+
+    def rep_dists(self, q, precision=PREC_FP32):
+        """rbc_exact_rep_dists on host arrays: (D [m, num_reps], abs_err [m], rel_err) with
+        |D - exact| <= rel_err * exact + abs_err[i]."""
+        q, qp = _host(q, np.float32)
+        m = q.shape[0]
+        D = np.empty((m, self.num_reps), np.float32)
+        ae = np.empty(m, np.float32)
+        rel = ctypes.c_double()
+        _check(lib().rbc_exact_rep_dists(self.h, qp, m, q.shape[1], precision, D.ctypes.data, self.num_reps,
+                                         ae.ctypes.data, ctypes.byref(rel), MEM_HOST, None))
+        return D, ae, rel.value
 
     def info(self):
         inf = ExactInfo()

@@ -187,3 +187,41 @@ def test_exact_strided_host_buffers_match_device_path():
     np.testing.assert_array_equal(ids[:, :k], gi.cpu().numpy().view(np.uint32))
     np.testing.assert_array_equal(ds[:, :k], gd.cpu().numpy())
     check_against_fp64(x, q[:500], ids[:500, :k], ds[:500, :k], k)
+
+
+def _rep_d64(x, rep_ids, q, metric):
+    r = x[rep_ids].astype(np.float64)
+    q = q.astype(np.float64)
+    if metric == 1:
+        return np.abs(q[:, None, :] - r[None, :, :]).sum(2)
+    return ((q[:, None, :] - r[None, :, :]) ** 2).sum(2)
+
+
+@pytest.mark.parametrize("gen,n,d,R,metric,prec", [
+    ("lowrank", 20000, 128, 300, 0, "bf16"),
+    ("gmm", 20000, 96, 257, 0, "bf16"),
+    ("uniform", 8000, 33, 100, 0, "bf16"),
+    ("gmm", 20000, 64, 173, 0, "fp32"),
+    ("uniform", 8000, 33, 100, 1, "fp32"),
+])
+def test_rep_distances_within_their_stated_bound(gen, n, d, R, metric, prec):
+    # The query-rep distances the exact query prunes with (split-BF16 MFMA for the BF16
+    # filter, FP32 VALU otherwise) against FP64: within rel_err * D + abs_err[q] for every
+    # pair, and the bound not loose by more than the operands' scale warrants.
+    x = getattr(datagen, gen)(n, d, 0)
+    q = getattr(datagen, gen)(500, d, 1)
+    q[:3] = x[:3]  # queries on top of points (and reps): D near 0, where cancellation bites
+    idx = rbc.ExactRBC(x, R, metric=metric, seed=5)
+    p = rbc.PREC_BF16 if prec == "bf16" else rbc.PREC_FP32
+    D, ae, rel = idx.rep_dists(q, p)
+    D64 = _rep_d64(x, idx.lists()["rep_ids"], q, metric)
+    err = np.abs(D.astype(np.float64) - D64)
+    bound = rel * D64 + ae.astype(np.float64)[:, None]
+    assert (err <= bound).all(), f"max excess {(err - bound).max()}"
+    if prec == "bf16":
+        assert rel == 0.0 and (ae > 0).all()
+        scale = (q.astype(np.float64) ** 2).sum(1) + (x[idx.lists()["rep_ids"]].astype(np.float64) ** 2).sum(1).max()
+        assert (ae <= 1e-3 * scale).all()
+        assert err.max() > 0  # a real rounding error exists for the bound to cover
+    else:
+        assert (ae == 0).all() and 0 < rel < 1e-5
