@@ -29,6 +29,7 @@ extern "C" {
 #define RBC_ERR_NOMEM 3    /* device allocation failed (rbc.cu:151-154 exits instead) */
 #define RBC_ERR_NOCONV 4   /* range bisection hit its iteration cap (kernels.cu:368-409 has none) */
 #define RBC_ERR_LIMIT 5    /* a documented size limit was exceeded */
+#define RBC_ERR_CHECK 6    /* a device bounds check failed (builds with -DRBC_CHECK only) */
 
 #define RBC_L2 0 /* squared Euclidean, defs.h:21 */
 #define RBC_L1 1 /* sum of absolute differences, defs.h:20 */
@@ -42,6 +43,15 @@ extern "C" {
 #define RBC_MAX_K 64
 
 const char *rbc_last_error(void);
+/* 1 in a build with device bounds checks (-DRBC_CHECK, librbc_hip_check.so), else 0.  In
+ * such a build a kernel index outside its buffer is clamped to 0 instead of read or
+ * written, and the entry point returns RBC_ERR_CHECK naming the source file and line.
+ * No counterpart in the reference (its debugging aid is cuda-memcheck). */
+int rbc_check_enabled(void);
+/* Reads element `index` of a device buffer of `bound` words (holding 1000 + i) through a
+ * checked index into *value: the check build returns RBC_ERR_CHECK (and *value = 1000)
+ * for index >= bound; other builds reject index >= bound before any launch. */
+int rbc_check_selftest(uint32_t index, uint32_t bound, uint32_t *value);
 /* Library version as 10000*major + 100*minor + patch. */
 int rbc_version(void);
 /* Select the HIP device used by subsequent calls of this thread (driver.cu:37-41). */

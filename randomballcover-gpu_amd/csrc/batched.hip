@@ -274,7 +274,10 @@ __global__ __launch_bounds__(256) void batched_select(BSelArgs a) {
   }
   float tv = FLT_MAX;
   uint32_t ti = kDummy;
-  auto cand = [&](uint32_t e) -> uint32_t { return a.cand_i[size_t(g) * a.Kp + e]; };
+  auto cand = [&](uint32_t e) -> uint32_t {
+    const uint32_t id = a.cand_i[size_t(g) * a.Kp + e];
+    return id == kDummy ? id : RBC_CHECKED(id, I.n);
+  };
   rerank_candidates<METRIC, NPL>(tv, ti, a.Kp, cand, I.xp, I.ld, I.inv, qv, I.d);
   const double P = (double)a.P;
   const double e_tree = ((P <= 64 ? 6.0 : 7.0) + 4.0) * 0x1p-24 * 1.01;
@@ -656,7 +659,7 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
     stats->pairs_scan = (double)hs[4];
     stats->flops = (double)hs[5];
   }
-  return RBC_OK;
+  return device_check();
 }
 
 int rbc_batched_get_index(const rbc_batched *b, uint32_t i, const rbc_exact **out) {

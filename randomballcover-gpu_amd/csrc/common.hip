@@ -3,6 +3,8 @@
 #include <cstring>
 #include <mutex>
 
+#include <vector>
+
 #include "launch.hpp"
 
 namespace rbc {
@@ -29,6 +31,36 @@ void set_error(const char *fmt, ...) {
 }
 
 int metric_setting() { return g_metric; }
+
+namespace {
+struct CheckReader {
+  int (*read)(unsigned int *);
+  const char *unit;
+};
+std::vector<CheckReader> &check_readers() {
+  static std::vector<CheckReader> r;
+  return r;
+}
+}  // namespace
+
+void register_check_reader(int (*read)(unsigned int *), const char *unit) {
+  check_readers().push_back(CheckReader{read, unit});
+}
+
+int device_check() {
+  for (const CheckReader &r : check_readers()) {
+    unsigned int line = 0;
+    if (r.read(&line) != 0) {
+      set_error("device bounds check: reading the check word of %s failed", r.unit);
+      return RBC_ERR_HIP;
+    }
+    if (line) {
+      set_error("device bounds check failed in %s, line %u (of that file or a header it includes)", r.unit, line);
+      return RBC_ERR_CHECK;
+    }
+  }
+  return RBC_OK;
+}
 
 int max_lds_bytes() {
   static int v = 0;
@@ -178,3 +210,35 @@ int rbc_set_metric(int metric) {
 int rbc_get_metric(void) { return rbc::g_metric; }
 
 }  // extern "C"
+
+namespace {
+__global__ void check_probe(const uint32_t *buf, uint32_t index, uint32_t bound, uint32_t *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = buf[RBC_CHECKED(index, bound)];
+}
+}  // namespace
+
+extern "C" int rbc_check_enabled(void) {
+#ifdef RBC_CHECK
+  return 1;
+#else
+  return 0;
+#endif
+}
+
+extern "C" int rbc_check_selftest(uint32_t index, uint32_t bound, uint32_t *value) {
+  RBC_ARG_CHECK(bound >= 1 && bound <= (1u << 20) && value, "bound %u outside [1, 2^20] or NULL value", bound);
+  // Without the checks an index past the buffer would be a real out-of-bounds read.
+  RBC_ARG_CHECK(rbc_check_enabled() || index < bound, "index %u >= bound %u needs a build with RBC_CHECK", index,
+                bound);
+  using namespace rbc;
+  std::vector<uint32_t> h(bound);
+  for (uint32_t i = 0; i < bound; ++i) h[i] = 1000u + i;
+  DevBuf<uint32_t> buf, out;
+  RBC_TRY(buf.alloc(bound));
+  RBC_TRY(out.alloc(1));
+  RBC_TRY(upload_host(buf.get(), h.data(), size_t(bound) * 4, nullptr));
+  hipLaunchKernelGGL(check_probe, dim3(1), dim3(64), 0, nullptr, buf.get(), index, bound, out.get());
+  RBC_HIP_TRY(hipGetLastError());
+  RBC_HIP_TRY(hipMemcpy(value, out.get(), 4, hipMemcpyDeviceToHost));
+  return device_check();
+}

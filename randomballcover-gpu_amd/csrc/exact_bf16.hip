@@ -199,6 +199,7 @@ struct Bf16ScanArgs {
   const uint16_t *xb;    // permuted database, BF16, ldb
   const float *xn;       // |x~|^2 per permuted row
   const uint32_t *perm;  // permuted row -> original id
+  uint32_t n;            // database rows (bounds checks)
   uint32_t ldb;
   float *cand_d;
   uint32_t *cand_i;
@@ -596,7 +597,7 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
                 const uint32_t gs = atomicAdd(&a.cand_n[row], 1u);
                 if (gs < a.cap) {
                   a.cand_d[size_t(row) * a.cap + gs] = a.qn[row] + v;
-                  a.cand_i[size_t(row) * a.cap + gs] = a.perm[x0 + p];
+                  a.cand_i[size_t(row) * a.cap + gs] = a.perm[RBC_CHECKED(x0 + p, a.n)];
                 }
               }
             }
@@ -612,7 +613,7 @@ __global__ __launch_bounds__(256, 2) void bf16_scan_kernel(Bf16ScanArgs a) {
     const uint32_t gs = atomicAdd(&a.cand_n[row], 1u);
     if (gs < a.cap) {
       a.cand_d[size_t(row) * a.cap + gs] = a.qn[row] + cs_v[i];
-      a.cand_i[size_t(row) * a.cap + gs] = a.perm[x0 + cs_p[i]];
+      a.cand_i[size_t(row) * a.cap + gs] = a.perm[RBC_CHECKED(x0 + cs_p[i], a.n)];
     }
   }
   // The next item rewrites qrow, sh, cs_* and s_item.
@@ -889,6 +890,7 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
   a.xb = h->xb.get();
   a.xn = h->xb_norm.get();
   a.perm = h->perm.get();
+  a.n = h->n;
   a.ldb = h->ldb;
   a.cand_d = cand_d;
   a.cand_i = cand_i;

@@ -302,7 +302,7 @@ template <int METRIC, int NPL>
 __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, const float *xp, size_t ldx,
                                                    const uint32_t *inv, uint32_t m, uint32_t d, uint32_t P,
                                                    uint32_t Ke, uint32_t Kp, const float *pv,
-                                                   const uint32_t *pid, uint32_t *pi, float *pd) {
+                                                   const uint32_t *pid, uint32_t *pi, float *pd, uint32_t n) {
   const uint32_t q = blockIdx.x * 4 + threadIdx.x / kWave;
   if (q >= m) return;
   const int l = lane_id();
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
     wave_merge_lower(tv, ti, v, id);
   }
   // Lane l < K' holds the l-th best; the re-rank gathers their rows eight at a time.
-  const uint32_t myid = l < (int)Kp ? ti : kDummy;
+  const uint32_t myid = l < (int)Kp && ti != kDummy ? RBC_CHECKED(ti, n) : kDummy;
   float rv = FLT_MAX;
   uint32_t ri = kDummy;
   rerank_candidates<METRIC, NPL>(rv, ri, Kp, [&](uint32_t) { return myid; }, xp, ldx, inv, qv, d);
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
 // and one atomic per wave reserves the wave's range of the list.
 __global__ __launch_bounds__(256) void plan_scatter(uint32_t Wq, const uint32_t *bits,
                                                     const uint32_t *list_qoff, uint32_t *cursor,
-                                                    uint32_t *qlist) {
+                                                    uint32_t *qlist, uint32_t cap) {
   const uint32_t j = blockIdx.y;
   const uint32_t w0 = (blockIdx.x * 4 + threadIdx.x / kWave) * kWave;
   if (w0 >= Wq) return;
@@ -484,7 +484,8 @@ __global__ __launch_bounds__(256) void plan_scatter(uint32_t Wq, const uint32_t 
   if (l == 0) base = list_qoff[j] + atomicAdd(&cursor[j], total);
   uint32_t pos = __shfl(base, 0, kWave) + incl - c;
   while (word) {
-    qlist[pos++] = t * 32 + (__ffs(word) - 1);
+    qlist[RBC_CHECKED(pos, cap)] = t * 32 + (__ffs(word) - 1);
+    ++pos;
     word &= word - 1;
   }
 }
@@ -514,9 +515,11 @@ __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
   const uint32_t nc = a.cand_n ? min(a.cand_n[q], a.cap) : 0;
   const uint32_t total = np + nc;
   auto cand = [&](uint32_t e) -> uint32_t {
-    if (e < np0) return a.probe_i[size_t(q) * a.Kp + e];
-    if (e < np) return a.probe_i1[size_t(q) * a.Kp + (e - np0)];
-    return a.cand_i[size_t(q) * a.cap + (e - np)];
+    uint32_t id;
+    if (e < np0) id = a.probe_i[size_t(q) * a.Kp + e];
+    else if (e < np) id = a.probe_i1[size_t(q) * a.Kp + (e - np0)];
+    else id = a.cand_i[size_t(q) * a.cap + (e - np)];
+    return id == kDummy ? id : RBC_CHECKED(id, a.n);
   };
   rerank_candidates<METRIC, NPL>(tv, ti, total, cand, a.xp, a.ldx, a.inv, qv, a.d);
   near_tie_guard<METRIC>(tv, ti, total, a.xp, a.ldx, a.inv, qrow, a.d, a.k, a.e_tree, a.stats);
@@ -536,13 +539,13 @@ int launch_probe_merge(const float *Q, size_t ldq, const rbc_exact *h, uint32_t 
   dim3 g(cdiv(m, 4)), b(256);
   if (T <= 64)
     hipLaunchKernelGGL((probe_merge<0, 1>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, P, Ke, Kp, pv, pid, pi, pd);
+                       h->d, P, Ke, Kp, pv, pid, pi, pd, h->n);
   else if (T == 128)
     hipLaunchKernelGGL((probe_merge<0, 2>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, P, Ke, Kp, pv, pid, pi, pd);
+                       h->d, P, Ke, Kp, pv, pid, pi, pd, h->n);
   else
     hipLaunchKernelGGL((probe_merge<0, 4>), g, b, 0, st, Q, ldq, h->xp.get(), (size_t)h->ld, h->inv.get(), m,
-                       h->d, P, Ke, Kp, pv, pid, pi, pd);
+                       h->d, P, Ke, Kp, pv, pid, pi, pd, h->n);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
@@ -663,7 +666,7 @@ static int rep_distances(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, 
   return RBC_OK;
 }
 
-int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, uint32_t k, int precision,
+static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, uint32_t k, int precision,
                        float *od, uint32_t *oi, uint32_t ldo, hipStream_t st, rbc_query_stats *stats) {
   const uint32_t R = h->R, d = h->d;
   Workspace &ws = h->ws;
@@ -872,7 +875,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   RBC_TRY(ws.get(kQ + 14, size_t(std::max(total_pairs, 1u)) * 4, &p));
   uint32_t *qlist = static_cast<uint32_t *>(p);
   hipLaunchKernelGGL(plan_scatter, dim3(cdiv(Wq, 4 * kWave), R), dim3(256), 0, st, Wq, bits, lcnt, cursor,
-                     qlist);
+                     qlist, total_pairs);
   RBC_HIP_TRY(hipGetLastError());
   DevBuf<ScanItem> sitems;
   uint32_t nsitems = 0;
@@ -934,6 +937,7 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
   sa.xp = h->xp.get();
   sa.ldx = h->ld;
   sa.inv = h->inv.get();
+  sa.n = h->n;
   sa.m = m;
   sa.d = d;
   sa.k = k;
@@ -1004,6 +1008,15 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
     stats->ms_total = tm.ms(0, 5);
   }
   return RBC_OK;
+}
+
+int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, uint32_t k, int precision,
+                       float *od, uint32_t *oi, uint32_t ldo, hipStream_t st, rbc_query_stats *stats) {
+  RBC_TRY(exact_query_run(h, qd, ldq, m, k, precision, od, oi, ldo, st, stats));
+#ifdef RBC_CHECK
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+#endif
+  return device_check();
 }
 
 }  // namespace rbc

@@ -12,6 +12,7 @@ struct SelectArgs {
   size_t ldx;
   const uint32_t *inv;
   uint32_t m, d, k, Kp;
+  uint32_t n;                 // database rows (bounds checks)
   const uint32_t *probe_i;   // [m][Kp] first-probe candidates (original ids)
   const uint32_t *probe_i1;  // [m][Kp] second-probe candidates, or NULL
   const uint32_t *key1;      // second-probe list per query (R = none)

@@ -279,7 +279,7 @@ int oneshot_query_common(const float *q, uint32_t m, uint32_t ldq, const rbcStru
     RBC_TRY(rows_out(ids, size_t(ldo) * 4, oi.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
     if (dists) RBC_TRY(rows_out(dists, size_t(ldo) * 4, od.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
     RBC_HIP_TRY(hipStreamSynchronize(st));
-    return RBC_OK;
+    return device_check();
   }
   RBC_TRY(xoff.alloc(R));
   RBC_TRY(xlen.alloc(R));
@@ -307,7 +307,7 @@ int oneshot_query_common(const float *q, uint32_t m, uint32_t ldq, const rbcStru
   RBC_TRY(rows_out(ids, size_t(ldo) * 4, oi.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
   if (dists) RBC_TRY(rows_out(dists, size_t(ldo) * 4, od.get(), size_t(K) * 4, size_t(K) * 4, m, mem, st));
   RBC_HIP_TRY(hipStreamSynchronize(st));
-  return RBC_OK;
+  return device_check();
 }
 
 }  // namespace rbc

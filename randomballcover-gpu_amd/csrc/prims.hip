@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kSortNT) void sort_scatter(const uint32_t *key, siz
     for (uint32_t ww = 0; ww < 4; ++ww) {
       if (w == ww && valid) {
         const uint32_t base = run[k];
-        order[base + rank] = (uint32_t)i;
+        order[RBC_CHECKED(base + rank, (uint32_t)n)] = (uint32_t)i;
         // Every lane of the wave has read run[k] before its leader's write (in-order LDS).
         if (leader) run[k] = base + (uint32_t)__popcll(same);
       }

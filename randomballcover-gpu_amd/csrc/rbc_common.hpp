@@ -15,6 +15,9 @@
 namespace rbc {
 
 void set_error(const char *fmt, ...);
+// RBC_ERR_CHECK (with the failing unit and line as the message) if a device bounds check
+// failed since the last call; RBC_OK otherwise and always in builds without RBC_CHECK.
+int device_check();
 
 #define RBC_HIP_TRY(expr)                                                                     \
   do {                                                                                        \

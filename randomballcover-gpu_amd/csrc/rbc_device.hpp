@@ -14,6 +14,33 @@ namespace rbc {
 constexpr uint32_t kDummy = 0xFFFFFFFFu;
 constexpr int kWave = 64;
 
+// Device bounds checks (a build with -DRBC_CHECK).  RBC_CHECKED(i, n) is i when i < n;
+// otherwise it records its source line in this translation unit's check word and yields
+// 0, an index that is always in bounds, so the kernel finishes normally; the entry point
+// then returns RBC_ERR_CHECK naming the translation unit and line (device_check).
+// Without RBC_CHECK it is just i.
+#ifdef RBC_CHECK
+void register_check_reader(int (*read)(unsigned int *line), const char *unit);
+namespace {
+__device__ unsigned int tu_check_word;
+__device__ __noinline__ void check_fail(unsigned int line) { atomicCAS(&tu_check_word, 0u, line); }
+// Reads and clears this translation unit's check word (current device).
+int read_tu_check(unsigned int *line) {
+  unsigned int v = 0, z = 0;
+  if (hipMemcpyFromSymbol(&v, HIP_SYMBOL(tu_check_word), sizeof(v)) != hipSuccess) return -1;
+  if (v && hipMemcpyToSymbol(HIP_SYMBOL(tu_check_word), &z, sizeof(z)) != hipSuccess) return -1;
+  *line = v;
+  return 0;
+}
+struct TuCheckRegistration {
+  TuCheckRegistration() { register_check_reader(read_tu_check, __BASE_FILE__); }
+} tu_check_registration;
+}  // namespace
+#define RBC_CHECKED(i, n) ((i) < (n) ? (i) : (::rbc::check_fail(__LINE__), decltype(i)(0)))
+#else
+#define RBC_CHECKED(i, n) (i)
+#endif
+
 // (a, ai) < (b, bi) in the order (distance, index).
 __device__ __forceinline__ bool pair_less(float a, uint32_t ai, float b, uint32_t bi) {
   return a < b || (a == b && ai < bi);

@@ -15,7 +15,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librbc_hip.so")
+# RBC_LIB selects another build of the library (librbc_hip_check.so: device bounds checks).
+LIB_PATH = os.environ.get("RBC_LIB") or os.path.join(HERE, "librbc_hip.so")
 
 KMAX = 32
 DUMMY_IDX = 0xFFFFFFFF
@@ -66,6 +67,8 @@ _lib = None
 
 _SIGS = {
     "rbc_last_error": (ctypes.c_char_p, []),
+    "rbc_check_enabled": (_int, []),
+    "rbc_check_selftest": (_int, [_u32, _u32, ctypes.POINTER(_u32)]),
     "rbc_version": (_int, []),
     "rbc_set_device": (_int, [_int]),
     "rbc_set_metric": (_int, [_int]),
