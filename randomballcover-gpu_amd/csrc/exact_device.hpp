@@ -11,12 +11,14 @@ namespace rbc {
 // a float payload.
 __device__ void wave_sort_runs(uint32_t &run, double &key, uint32_t &id, float &pay) {
   const int l = lane_id();
+#pragma unroll
   for (int k = 2; k <= kWave; k <<= 1)
+#pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
-      uint32_t pr = __shfl_xor(run, j, kWave);
-      double pk = __shfl_xor(key, j, kWave);
-      uint32_t pi = __shfl_xor(id, j, kWave);
-      float pp = __shfl_xor(pay, j, kWave);
+      uint32_t pr = xor_lane(run, j);
+      double pk = xor_lane(key, j);
+      uint32_t pi = xor_lane(id, j);
+      float pp = xor_lane(pay, j);
       bool p_less = pr < run || (pr == run && (pk < key || (pk == key && pi < id)));
       bool m_less = run < pr || (run == pr && (key < pk || (key == pk && id < pi)));
       bool keep_min = ((l & j) == 0) == ((l & k) == 0);

@@ -48,11 +48,58 @@ __device__ __forceinline__ bool pair_less(float a, uint32_t ai, float b, uint32_
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Cross-lane exchange without the LDS crossbar: the value of lane l ^ J for J = 1..32, on
+// DPP (quad permutes for 1 and 2, row shifts for 4, a row rotate for 8) and on gfx950's
+// v_permlane16_swap / v_permlane32_swap for 16 and 32 (each swap leaves the partner row's
+// value in one of its two results).  These replace __shfl_xor (ds_bpermute_b32) in the
+// bitonic sort/merge stages of the top-k and in the butterfly reductions.
+template <int J>
+__device__ __forceinline__ uint32_t xor_lane_u32(uint32_t v) {
+  static_assert(J == 1 || J == 2 || J == 4 || J == 8 || J == 16 || J == 32, "lane distance");
+  if constexpr (J == 1) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+  } else if constexpr (J == 2) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+  } else if constexpr (J == 4) {
+    const uint32_t up = __builtin_amdgcn_update_dpp(0u, v, 0x104, 0xF, 0xF, false);  // row_shl:4, lane l+4
+    const uint32_t dn = __builtin_amdgcn_update_dpp(0u, v, 0x114, 0xF, 0xF, false);  // row_shr:4, lane l-4
+    return (lane_id() & 4) ? dn : up;
+  } else if constexpr (J == 8) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  } else if constexpr (J == 16) {
+    const auto s = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane_id() & 16) ? s[0] : s[1];
+  } else {
+    const auto s = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (lane_id() & 32) ? s[0] : s[1];
+  }
+}
+
+// Run-time distance (a constant once the caller's loop is unrolled).
+__device__ __forceinline__ uint32_t xor_lane(uint32_t v, int j) {
+  switch (j) {
+    case 1: return xor_lane_u32<1>(v);
+    case 2: return xor_lane_u32<2>(v);
+    case 4: return xor_lane_u32<4>(v);
+    case 8: return xor_lane_u32<8>(v);
+    case 16: return xor_lane_u32<16>(v);
+    default: return xor_lane_u32<32>(v);
+  }
+}
+__device__ __forceinline__ float xor_lane(float v, int j) {
+  return __uint_as_float(xor_lane(__float_as_uint(v), j));
+}
+__device__ __forceinline__ double xor_lane(double v, int j) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = xor_lane((uint32_t)u, j), hi = xor_lane((uint32_t)(u >> 32), j);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // One compare-exchange stage of a bitonic network across lanes at distance j.
 // keep_min selects whether this lane keeps the smaller of (own, partner).
 __device__ __forceinline__ void cx_stage(float &v, uint32_t &id, int j, bool keep_min) {
-  float pv = __shfl_xor(v, j, kWave);
-  uint32_t pi = __shfl_xor(id, j, kWave);
+  float pv = xor_lane(v, j);
+  uint32_t pi = xor_lane(id, j);
   bool take = keep_min ? pair_less(pv, pi, v, id) : pair_less(v, id, pv, pi);
   if (take) {
     v = pv;
@@ -136,7 +183,7 @@ __device__ __forceinline__ float tree_term(float x, float q) {
 // result equals the stride-halving tree over lanes 0..63.
 __device__ __forceinline__ float wave_tree_sum(float v) {
 #pragma unroll
-  for (int s = kWave / 2; s > 0; s >>= 1) v = __fadd_rn(v, __shfl_xor(v, s, kWave));
+  for (int s = kWave / 2; s > 0; s >>= 1) v = __fadd_rn(v, xor_lane(v, s));
   return v;
 }
 
