@@ -172,6 +172,10 @@ def bench_batched(args, rbc, datagen, torch, peaks):
             "pairs_frac_of_brute": round(st["pairs_scan"] / float((ns.astype(np.float64) * ms).sum()), 4),
             "peak_note": "FP32 vector FMA rate (the kernel's distances are FP32 VALU)",
             "frac_of_measured_fp32": round(ach / peaks["fp32_fma_tflops"], 4)}
+    log(f"batched kernel: {st['steps']:.4e} 64-point steps over all tiles, longest tile "
+        f"{st['longest_tile_ms']:.2f} ms")
+    roof["steps"] = st["steps"]
+    roof["longest_tile_ms"] = round(st["longest_tile_ms"], 3)
     log(f"batched kernel: {st['pairs_scan']:.4e} pairs, {gflop:.1f} GFLOP in {st['ms_graph']:.2f} ms = "
         f"{ach:.1f} TFLOP/s ({ach / FP32_PEAK_TFLOPS:.1%} of the FP32 peak)")
     cpu = None

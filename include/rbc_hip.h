@@ -349,6 +349,8 @@ typedef struct {
   double guard_fp64;        /* candidates ordered in FP64 by the near-tie guard */
   double pairs_scan;        /* (query, point) distances the persistent kernel evaluated */
   double flops;             /* 2 d per pair scanned plus 2 d per (query, rep) distance */
+  double steps;             /* 64-point list steps the persistent kernel ran (all tiles) */
+  double longest_tile_ms;   /* longest single tile (device wall clock), the launch's floor */
 } rbc_batched_stats;
 
 /* count indexes; x[i] has n[i] rows of d[i] floats with leading dimension ldx[i]; index

@@ -39,6 +39,9 @@ struct BInst {
   const uint32_t *inv;
   const float *radius;
   const float *rr;
+  const float *xn2;  // ||x||^2 per permuted row (FP64 sum rounded to nearest)
+  float xmax;        // max ||x|| over the index (rounded up)
+  double csplit;     // split-BF16 dot-product constant for this index's width
   uint32_t n, d, dpad, ld, R, P;  // P: tree width (64, 128)
   const float *q;                 // this index's queries (ldq), m of them
   uint32_t ldq, m, qbase;         // qbase: first global query number
@@ -65,7 +68,8 @@ struct BatchArgs {
   float *cand_d;      // [mtot][Kp]
   uint32_t *cert;     // [mtot]
   uint32_t *qinst;    // [mtot] index of each global query
-  unsigned long long *stats;  // [4] (query, point) pairs scanned, [5] their FLOPs + the q->rep FLOPs
+  unsigned long long *stats;  // [4] (query, point) pairs scanned, [5] their FLOPs + the q->rep FLOPs,
+                              // [6] 64-point list steps, [7] longest tile (wall-clock ticks)
 };
 
 // Dense distances of the tile's queries to a block of reps into the scratch rows.
@@ -88,7 +92,170 @@ struct EpiScratch {
 };
 
 __host__ __device__ inline size_t batch_lds_bytes(uint32_t dpad, uint32_t Rpad) {
-  return tile_lds_base_bytes(dpad) + EpiTopK::lds_bytes() + size_t(Rpad) * 8 + kTQ * 16 + 64;
+  return tile_lds_base_bytes(dpad) + EpiTopK::lds_bytes() + size_t(Rpad) * 8 + kTQ * 16 + 64 + kTQ * 8;
+}
+
+// ---- split-BF16 MFMA list scan (L2, d <= 128) ----------------------------------------
+// The tile's 64 queries are MFMA A operands (wave w: queries 16w.., 16 rows), kept in
+// registers for the whole tile as hi/lo BF16 pairs; a list is scanned 64 points per step,
+// four 16-point B tiles per wave, each split into hi/lo from its FP32 row on the fly.
+// q.x ~ qh.xh + qh.xl + ql.xh (three v_mfma_f32_16x16x32_bf16 per k-step) and
+// D~ = ||q||^2 + ||x||^2 - 2 q.x, within E_q of the true value for every point of the
+// index (the bound of the query-rep kernel with the index's largest point norm).  Lane l
+// holds queries 16w + 4(l/16) + i of point 16t + l%16; candidates below a query's running
+// K'-th value are staged and folded exactly as EpiTopK does (on D~ instead of FP32 values).
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef __attribute__((__vector_size__(8 * sizeof(uint16_t)))) uint16_t u16x8;
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// hi = bf16(v), lo = bf16(v - hi) of 8 coordinates.
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo) {
+  u16x8 h, o;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const uint16_t b = bf16_rne(v[e]);
+    h[e] = b;
+    o[e] = bf16_rne(v[e] - __uint_as_float(uint32_t(b) << 16));
+  }
+  hi = __builtin_bit_cast(bf16x8, h);
+  lo = __builtin_bit_cast(bf16x8, o);
+}
+
+// Coordinates c0..c0+7 of a row of d floats (zero from d on).
+__device__ __forceinline__ void load8(const float *row, uint32_t c0, uint32_t d, bool live, float (&v)[8]) {
+  if (live && c0 + 8 <= d && (reinterpret_cast<uintptr_t>(row + c0) & 15u) == 0) {
+    const float4 a = *reinterpret_cast<const float4 *>(row + c0), b = *reinterpret_cast<const float4 *>(row + c0 + 4);
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = live && c0 + e < d ? row[c0 + e] : 0.f;
+  }
+}
+
+// Stages (v, id) for query qi when it beats the query's running K'-th pair (thd, thi: the
+// pair as of the last fold; EpiTopK::step's first half).
+__device__ __forceinline__ void topk_stage(const EpiTopK::L &L, uint32_t qi, float v, uint32_t id, float thd,
+                                           uint32_t thi) {
+  if (id != kDummy && pair_less(v, id, thd, thi)) {
+    const uint32_t s = atomicAdd(&L.cnt[qi], 1u);
+    L.cd[qi * 64 + s] = v;
+    L.ci[qi * 64 + s] = id;
+  }
+}
+
+// One thread per query folds its staged pairs into its sorted K best (EpiTopK::step's
+// second half); brackets itself with barriers.
+__device__ void topk_fold(const EpiTopK::L &L, uint32_t K) {
+  __syncthreads();
+  if (threadIdx.x < kTQ) {
+    const int qi = threadIdx.x;
+    const uint32_t n = L.cnt[qi];
+    if (n) {
+      float kd = L.thd[qi];
+      uint32_t ki = L.thi[qi];
+      float *td = L.td + qi * 64;
+      uint32_t *ti = L.ti + qi * 64;
+      for (uint32_t c = 0; c < n; ++c) {
+        const float cv = L.cd[qi * 64 + c];
+        const uint32_t cid = L.ci[qi * 64 + c];
+        if (!pair_less(cv, cid, kd, ki)) continue;
+        int pos = (int)K - 1;
+        while (pos > 0 && pair_less(cv, cid, td[pos - 1], ti[pos - 1])) {
+          td[pos] = td[pos - 1];
+          ti[pos] = ti[pos - 1];
+          --pos;
+        }
+        td[pos] = cv;
+        ti[pos] = cid;
+        kd = td[K - 1];
+        ki = ti[K - 1];
+      }
+      L.thd[qi] = kd;
+      L.thi[qi] = ki;
+      L.cnt[qi] = 0;
+    }
+  }
+  __syncthreads();
+}
+
+// Scans permuted rows [off, off + len) of index I for the tile's queries (Qh/Ql: this
+// wave's A operands, qn2s: their ||q||^2, nq: live queries).  The kernel runs one
+// workgroup per CU (its LDS), so each wave has the registers to hold the next step's point
+// rows: they are loaded while the current step's MFMAs, staging and fold run.
+__device__ void mfma_scan_list(const BInst &I, uint32_t off, uint32_t len, const bf16x8 (&Qh)[4],
+                               const bf16x8 (&Ql)[4], const float *qn2s, uint32_t nq, const EpiTopK::L &L,
+                               uint32_t K) {
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const uint32_t koff = (l / 16) * 8;
+  const int KS = (int)((I.d + 31u) / 32u);
+  float nv[4][4][8];  // next step: [tile][k-step][8 coordinates]
+  uint32_t nid[4];
+  float nxn[4];
+  auto fetch = [&](uint32_t p0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint32_t p = p0 + 16 * t + (l % 16);
+      const bool live = p < len;
+      const uint32_t row = off + (live ? p : 0);
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) load8(I.xp + size_t(row) * I.ld, ks * 32 + koff, I.d, live && ks < KS, nv[t][ks]);
+      nid[t] = live ? I.perm[row] : kDummy;
+      nxn[t] = live ? I.xn2[row] : 0.f;
+    }
+  };
+  fetch(0);
+  for (uint32_t p0 = 0; p0 < len; p0 += kTP) {
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      if (ks >= KS) break;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 bh, bl;
+        split8(nv[t][ks], bh, bl);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Qh[ks], bh, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Qh[ks], bl, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ql[ks], bh, acc[t], 0, 0, 0);
+      }
+    }
+    uint32_t id[4];
+    float xn[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      id[t] = nid[t];
+      xn[t] = nxn[t];
+    }
+    // The next step's rows load during this step's staging and fold.
+    if (p0 + kTP < len) fetch(p0 + kTP);
+    // This lane's four queries: their norms and running K'-th pairs (changed only by folds).
+    float qn[4], thd[4];
+    uint32_t thi[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t qi = 16 * w + 4 * (l / 16) + i;
+      qn[i] = qn2s[qi];
+      thd[i] = L.thd[qi];
+      thi[i] = L.thi[qi];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t qi = 16 * w + 4 * (l / 16) + i;
+        if (qi >= nq) continue;
+        const float v = fmaxf(__builtin_fmaf(-2.f, acc[t][i], qn[i] + xn[t]), 0.f);
+        topk_stage(L, qi, v, id[t], thd[i], thi[i]);
+      }
+    }
+    topk_fold(L, K);
+  }
 }
 
 template <int METRIC>
@@ -104,6 +271,8 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
   float *gam = d0s + kTQ;
   float *gtmp = gam + kTQ;  // [kTQ] scratch
   uint32_t *misc = reinterpret_cast<uint32_t *>(gtmp + kTQ);  // [0] tile, [1] need flag, [2] gmax bits
+  float *qn2s = reinterpret_cast<float *>(misc + 16);          // [kTQ] ||q||^2 (MFMA scan)
+  float *eqs = qn2s + kTQ;                                     // [kTQ] absolute error of D~
   float *scr = a.scratch + size_t(blockIdx.x) * kTQ * a.Rmax;
 
   for (;;) {
@@ -113,6 +282,8 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
     const uint32_t tile = misc[0];
     if (tile >= a.ntiles) break;
     const BTile T = a.tiles[tile];
+    const unsigned long long t_begin = wall_clock64();
+    unsigned long long tile_pairs = 0, tile_steps_n = 0;  // thread 0's per-tile counts
     const BInst I = a.inst[T.inst];
     ScanItem it{0, 0, T.q0, T.qn, T.inst};
     // 1. queries -> LDS; distances to all reps -> scratch.
@@ -179,7 +350,35 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
         }
       }
     __syncthreads();
-    // 3. Scan lists in bound order with per-query best K'.
+    // 3. Scan lists in bound order with per-query best K' (FP32 tiles, or split-BF16 MFMA
+    // tiles for L2 with values D~ within eqs[q] of the true distance).
+    const bool mf = METRIC == 0 && I.d <= 128;
+    bf16x8 Qh[4], Ql[4];
+    if (mf) {
+      const int w = t / kWave, l = lane_id();
+      const uint32_t qi = 16 * w + (l % 16);
+      const bool ok = qi < T.qn;
+      const float *qr = I.q + size_t(T.q0 + (ok ? qi : 0)) * I.ldq;
+      double s2 = 0.0;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        float v[8];
+        load8(qr, ks * 32 + (l / 16) * 8, I.d, ok && (uint32_t)ks * 32 < I.d, v);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s2 += (double)v[e] * (double)v[e];
+        split8(v, Qh[ks], Ql[ks]);
+      }
+      s2 += xor_lane(s2, 16);
+      s2 += xor_lane(s2, 32);
+      if (l < 16 && ok) {
+        qn2s[qi] = (float)s2;
+        const double u = 0x1p-24, nq = sqrt(s2), nx = (double)I.xmax, c = I.csplit;
+        const double E = (2.0 * c * nq * nx + 4.0 * u * (s2 + nx * nx) + 2.0 * u * (1.0 + c) * nq * nx) * 1.01 + 1e-30;
+        float Ef = (float)E;
+        if ((double)Ef < E) Ef = nextafterf(Ef, FLT_MAX);
+        eqs[qi] = Ef;
+      }
+    }
     EpiTopK epi{a.Kp, nullptr, nullptr, 0, 0};
     epi.begin(ctx);
     if (t == 0) misc[2] = __float_as_uint(INFINITY);
@@ -200,11 +399,12 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
       __syncthreads();
       if (!misc[1]) continue;
       const uint32_t len = I.list_off[j + 1] - I.list_off[j];
-      tile_steps<METRIC>(qsrc, I.list_off[j], len, epi, ctx, smem);
-      if (t == 0) {
-        atomicAdd(&a.stats[4], (unsigned long long)T.qn * len);
-        atomicAdd(&a.stats[5], 2ull * I.d * T.qn * len);
-      }
+      if (mf)
+        mfma_scan_list(I, I.list_off[j], len, Qh, Ql, qn2s, T.qn, epi.lay(ctx), a.Kp);
+      else
+        tile_steps<METRIC>(qsrc, I.list_off[j], len, epi, ctx, smem);
+      tile_pairs += (unsigned long long)T.qn * len;
+      tile_steps_n += (len + kTP - 1) / kTP;
       // Refresh each query's bound from its current k-th value.
       EpiTopK::L L = epi.lay(ctx);
       if (t < kTQ) {
@@ -212,7 +412,7 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
         if ((uint32_t)t < T.qn) {
           float v = L.td[t * 64 + (a.k - 1)];
           if (v < FLT_MAX) {
-            double ghi = (double)v / (1.0 - a.e) * (1.0 + 1e-12);
+            double ghi = mf ? ((double)v + (double)eqs[t]) * (1.0 + 1e-12) : (double)v / (1.0 - a.e) * (1.0 + 1e-12);
             double gd = METRIC == 0 ? sqrt(ghi) * (1.0 + 1e-12) : ghi;
             g = (float)gd;
             if ((double)g < gd) g = nextafterf(g, INFINITY);
@@ -221,13 +421,20 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
         }
         gtmp[t] = (uint32_t)t < T.qn ? g : 0.f;
       }
-      __syncthreads();
-      if (t == 0) {
-        float mx = 0.f;
-        for (uint32_t qi = 0; qi < T.qn; ++qi) mx = fmaxf(mx, gtmp[qi]);
-        misc[2] = __float_as_uint(mx);
+      // Wave 0 (the threads t < kTQ that wrote them) reduces the bounds to their maximum.
+      if (t < kTQ) {
+        float mx = gtmp[t];
+#pragma unroll
+        for (int s = kWave / 2; s > 0; s >>= 1) mx = fmaxf(mx, xor_lane(mx, s));
+        if (t == 0) misc[2] = __float_as_uint(mx);
       }
       __syncthreads();
+    }
+    if (t == 0) {
+      atomicAdd(&a.stats[4], tile_pairs);
+      atomicAdd(&a.stats[5], 2ull * I.d * tile_pairs);
+      atomicAdd(&a.stats[6], tile_steps_n);
+      atomicMax(&a.stats[7], wall_clock64() - t_begin);
     }
     // 4. Candidates and completeness.
     EpiTopK::L L = epi.lay(ctx);
@@ -239,9 +446,11 @@ __global__ __launch_bounds__(kNT) void batched_exact_kernel(BatchArgs a, uint32_
       a.cand_d[g * a.Kp + jj] = L.td[qi * 64 + jj];
       if (jj == 0) {
         const float vk = L.td[qi * 64 + a.k - 1], vK = L.td[qi * 64 + a.Kp - 1];
-        // Points left out have FP32 value >= vK; they cannot beat the k-th kept when
-        // vK / (1 + e) > vk / (1 - e).
-        bool ok = vK >= FLT_MAX || (double)vK / (1.0 + a.e) > (double)vk / (1.0 - a.e) * (1.0 + 1e-12);
+        // Points left out have a value >= vK; they cannot beat the k-th kept when
+        // vK / (1 + e) > vk / (1 - e) (FP32 values), or vK - E > vk + E (D~).
+        bool ok = vK >= FLT_MAX ||
+                  (mf ? (double)vK - (double)eqs[qi] > ((double)vk + (double)eqs[qi]) * (1.0 + 1e-12)
+                      : (double)vK / (1.0 + a.e) > (double)vk / (1.0 - a.e) * (1.0 + 1e-12));
         a.cert[g] = ok ? 1u : 0u;
       }
     }
@@ -290,6 +499,15 @@ __global__ __launch_bounds__(256) void batched_select(BSelArgs a) {
 
 constexpr int kBatchStats = 8;
 
+// ||x||^2 of every row (FP64 sum rounded to nearest).
+__global__ void row_norms2(const float *xp, uint32_t n, uint32_t d, uint32_t ld, float *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (uint32_t c = 0; c < d; ++c) s += (double)xp[size_t(i) * ld + c] * (double)xp[size_t(i) * ld + c];
+  out[i] = (float)s;
+}
+
 __global__ void reset_counter(uint32_t *c, unsigned long long *stats) {
   c[0] = 0;
   for (int i = 0; i < kBatchStats; ++i) stats[i] = 0;
@@ -321,6 +539,9 @@ struct rbc_batched {
   rbc::DevBuf<unsigned long long> stats;
   uint32_t ntiles = 0, mtot = 0, grid = 0;
   std::map<std::vector<uint32_t>, uint32_t> tuned;  // batch shape -> tile size
+  // Per index: ||x||^2 of its permuted rows and the largest ||x|| (the MFMA scan's bound).
+  std::vector<rbc::DevBuf<float>> xn2;
+  std::vector<float> xmax;
   ~rbc_batched() {
     if (plan.exec) (void)hipGraphExecDestroy(plan.exec);
     for (auto *p : idx) rbc_exact_destroy(p);
@@ -416,6 +637,27 @@ int rbc_batched_build(uint32_t count, const float *const *x, const uint32_t *n, 
       return s;
     }
     b->idx.push_back(h);
+    rbc::DevBuf<float> nb;
+    if ((s = nb.alloc(h->n)) != RBC_OK) {
+      delete b;
+      return s;
+    }
+    hipLaunchKernelGGL(row_norms2, dim3(rbc::cdiv(h->n, 256)), dim3(256), 0, nullptr, h->xp.get(), h->n, h->d, h->ld,
+                       nb.get());
+    std::vector<float> hn(h->n);
+    if (hipMemcpy(hn.data(), nb.get(), size_t(h->n) * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+      delete b;
+      set_error("copying row norms failed");
+      return RBC_ERR_HIP;
+    }
+    double mx = 0.0;
+    for (float v : hn) mx = std::max(mx, (double)v);
+    // The stored norms are within 2^-24 (relative) of the FP64 sums.
+    const double xm = std::sqrt(mx * (1.0 + 0x1p-23)) * (1.0 + 1e-12);
+    float xmf = (float)xm;
+    if ((double)xmf < xm) xmf = nextafterf(xmf, FLT_MAX);
+    b->xn2.push_back(std::move(nb));
+    b->xmax.push_back(xmf);
     b->Rmax = std::max(b->Rmax, h->R);
     b->dpad_max = std::max(b->dpad_max, h->dpad);
   }
@@ -502,6 +744,12 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
     I.inv = h->inv.get();
     I.radius = h->radius.get();
     I.rr = h->rr.get();
+    I.xn2 = b->xn2[i].get();
+    I.xmax = b->xmax[i];
+    {
+      const double up = 0x1p-23, N = 3.0 * 32.0 * cdiv(h->d, 32) + 1.0;
+      I.csplit = 3.1 * 0x1p-16 + 1.02 * N * up / (1.0 - N * up);
+    }
     I.n = h->n;
     I.d = h->d;
     I.dpad = h->dpad;
@@ -658,6 +906,12 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
     stats->guard_fp64 = (double)hs[0];
     stats->pairs_scan = (double)hs[4];
     stats->flops = (double)hs[5];
+    stats->steps = (double)hs[6];
+    int rate_khz = 0;
+    stats->longest_tile_ms =
+        hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, 0) == hipSuccess && rate_khz > 0
+            ? (double)hs[7] / rate_khz
+            : -1.0;
   }
   return device_check();
 }
