@@ -353,8 +353,12 @@ def main():
     ap.add_argument("--check-sample", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration)")
-    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--mlp-params", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-child", action="store_true",
+                    help="run only the measured query twice (for counter collection under rocprofv3)")
+    ap.add_argument("--mlp-params", default=None,
+                    help="with --pmc-child: proposal weights (.npy, as written by --save-mlp)")
+    ap.add_argument("--save-mlp", default=None, metavar="NPY",
+                    help="write the trained proposal's weights here (for a --pmc-child counter run)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -423,6 +427,8 @@ def main():
                              "loss_first": round(float(losses[0]), 4), "loss_last": round(float(losses[-1]), 4),
                              "top8_list_cover_train": round(cover, 4), "params": net.num_params,
                              "mass": 0.9}
+        if args.save_mlp:
+            np.save(args.save_mlp, net.get_params())
         # The same search without the proposal, timed exactly like the headline arm.
         w_plain, s_plain_all = timed_arm()
         s_plain = s_plain_all[-1]
