@@ -341,11 +341,12 @@ typedef struct rbc_batched rbc_batched;
 
 typedef struct {
   double ms_graph;          /* hipEvent time of the graph launch */
-  uint32_t tile_queries;    /* autotuned queries per tile */
+  uint32_t tile_queries;    /* queries per tile (16 per wave for L2; else autotuned 16/32/64) */
   uint32_t graph_replayed;  /* 1 if the call replayed a captured graph */
   uint32_t uncertified;     /* queries re-run through the single-index path */
-  uint32_t pad;
-  double tune_ms[3];        /* autotuner timings for 16, 32, 64 queries per tile (first call) */
+  uint32_t blocks_per_cu;   /* L2: autotuned resident workgroups per CU of the wave kernel */
+  double tune_ms[3];        /* autotuner timings (first call): L2 a quarter, half and all of the
+                               workgroups per CU that fit; otherwise 16-, 32-, 64-query tiles */
   double guard_fp64;        /* candidates ordered in FP64 by the near-tie guard */
   double pairs_scan;        /* (query, point) distances the persistent kernel evaluated */
   double flops;             /* 2 d per pair scanned plus 2 d per (query, rep) distance */

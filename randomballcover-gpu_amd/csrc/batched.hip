@@ -40,6 +40,9 @@ struct BInst {
   const float *radius;
   const float *rr;
   const float *xn2;  // ||x||^2 per permuted row (FP64 sum rounded to nearest)
+  const uint16_t *rhi, *rlo;  // the reps split into BF16 hi/lo rows (ldb)
+  const float *rn2;           // ||r||^2
+  uint32_t ldb;
   float xmax;        // max ||x|| over the index (rounded up)
   double csplit;     // split-BF16 dot-product constant for this index's width
   uint32_t n, d, dpad, ld, R, P;  // P: tree width (64, 128)
@@ -508,6 +511,369 @@ __global__ void row_norms2(const float *xp, uint32_t n, uint32_t d, uint32_t ld,
   out[i] = (float)s;
 }
 
+// ---- wave-autonomous batched search (L2, d <= 128, K' <= 32) --------------------------
+// Each wave is an independent searcher: it takes a tile of up to 16 queries of one index
+// from a global counter and runs the whole exact search for them with no workgroup
+// barrier, so a CU holds as many searching waves as its LDS and registers allow.
+//   1. The 16 queries become MFMA A operands (hi/lo BF16 in registers), with ||q||^2 and
+//      the bound E_q of the split-BF16 distances (largest point norm of the index).
+//   2. D~(q, r) for every rep on split-BF16 MFMA tiles (the reps' hi/lo rows), kept in a
+//      per-wave global scratch block (16 x Rmax, L2 resident).
+//   3. The nearest rep of each query; the lower bound of every list over the wave's
+//      queries (triangle with psi and the Voronoi plane against each query's nearest rep,
+//      with E_q and the FP32 slack e of rr and of the owner test); the lists sorted by it.
+//   4. Lists in that order, skipping a list when no query can still find a point in it and
+//      stopping when the next bound exceeds every query's gamma; each list is scanned 64
+//      points per step as in mfma_scan_list, candidates under a query's running K'-th pair
+//      staged in the wave's LDS and folded by one lane per query.
+//   5. The K' candidates and their certificate (D~_K' - E > D~_k + E), as the other kernel.
+constexpr uint32_t kWQ = 16;   // queries per wave tile (one MFMA row block)
+constexpr uint32_t kWK = 32;   // K' slots per query
+constexpr uint32_t kWS = 64;   // staging slots per query (one 64-point step)
+
+struct WaveLds {
+  float td[kWQ][kWK];
+  uint32_t ti[kWQ][kWK];
+  float cd[kWQ][kWS];
+  uint32_t ci[kWQ][kWS];
+  uint32_t cnt[kWQ];
+  float thd[kWQ];
+  uint32_t thi[kWQ];
+  float gam[kWQ];   // metric-distance bound on each query's k-th NN (INF: none yet)
+  float eq[kWQ];    // |D~ - D| bound
+  float qn2[kWQ];
+  uint32_t j0[kWQ];  // nearest rep
+};
+
+__host__ __device__ inline size_t wave_lds_bytes(uint32_t Rpad) {
+  return (sizeof(WaveLds) + size_t(Rpad) * 8 + 15) & ~size_t(15);
+}
+
+struct WaveArgs {
+  const BInst *inst;
+  const BTile *tiles;
+  uint32_t ntiles;
+  uint32_t *next;
+  float *scratch;  // per wave: kWQ x Rmax query-rep distances
+  uint32_t Rmax, Rpad;
+  uint32_t k, Kp;
+  double e;        // FP32 relative error of rr and of the owner test
+  uint32_t *cand_i;
+  float *cand_d;
+  uint32_t *cert;
+  unsigned long long *stats;
+};
+
+// Lower bound (metric distance, L2) on d(q, x) for the points x of list j: the triangle
+// with psi_j and the Voronoi plane against the query's nearest rep j0 (D~ within E).
+__device__ __forceinline__ double wave_list_bound(double Dqj, double Dq0, double E, double rr_j0, double psi,
+                                                  double e, bool same) {
+  const double lo = fmax(Dqj - E, 0.0);
+  double lb = sqrt(lo) - psi;
+  if (!same && rr_j0 > 0.0) {
+    const double num = lo - (Dq0 + E) - 2.0 * e * psi * psi;
+    const double den = 2.0 * sqrt(rr_j0 * (1.0 + e));
+    if (num > 0.0) lb = fmax(lb, num / den);
+  }
+  return lb * (1.0 - 1e-12);
+}
+
+__global__ __launch_bounds__(256, 2) void batched_wave_kernel(WaveArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char wave_smem[];
+  const int w = threadIdx.x / kWave, l = lane_id();
+  const size_t wl = wave_lds_bytes(a.Rpad);
+  WaveLds &S = *reinterpret_cast<WaveLds *>(wave_smem + w * wl);
+  float *lbs = reinterpret_cast<float *>(wave_smem + w * wl + sizeof(WaveLds));
+  uint32_t *ord = reinterpret_cast<uint32_t *>(lbs + a.Rpad);
+  float *dq = a.scratch + size_t(blockIdx.x * (blockDim.x / kWave) + w) * kWQ * a.Rmax;
+  const uint32_t koff = (l / 16) * 8;
+  for (;;) {
+    uint32_t tile = 0;
+    if (l == 0) tile = atomicAdd(a.next, 1u);
+    tile = __builtin_amdgcn_readfirstlane(tile);
+    if (tile >= a.ntiles) break;
+    const BTile T = a.tiles[tile];
+    const BInst I = a.inst[T.inst];
+    const unsigned long long t_begin = wall_clock64();
+    const uint32_t nq = T.qn, R = I.R;
+    const int KS = (int)((I.d + 31u) / 32u);
+    // 1. Queries.
+    bf16x8 Qh[4], Ql[4];
+    {
+      const uint32_t qi = l % 16;
+      const bool ok = qi < nq;
+      const float *qr = I.q + size_t(T.q0 + (ok ? qi : 0)) * I.ldq;
+      double s2 = 0.0;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        float v[8];
+        load8(qr, ks * 32 + koff, I.d, ok && ks < KS, v);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) s2 += (double)v[e] * (double)v[e];
+        split8(v, Qh[ks], Ql[ks]);
+      }
+      s2 += xor_lane(s2, 16);
+      s2 += xor_lane(s2, 32);
+      if (l < 16) {
+        S.qn2[qi] = (float)s2;
+        const double u = 0x1p-24, nqn = sqrt(s2), nx = (double)I.xmax, c = I.csplit;
+        const double E = (2.0 * c * nqn * nx + 4.0 * u * (s2 + nx * nx) + 2.0 * u * (1.0 + c) * nqn * nx) * 1.01 + 1e-30;
+        float Ef = (float)E;
+        if ((double)Ef < E) Ef = nextafterf(Ef, FLT_MAX);
+        S.eq[qi] = Ef;
+        S.gam[qi] = INFINITY;
+        S.cnt[qi] = 0;
+        S.thd[qi] = FLT_MAX;
+        S.thi[qi] = kDummy;
+      }
+      for (uint32_t i = l; i < kWQ * kWK; i += kWave) {
+        (&S.td[0][0])[i] = FLT_MAX;
+        (&S.ti[0][0])[i] = kDummy;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    float qn4[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) qn4[i] = S.qn2[4 * (l / 16) + i];
+    // 2. Query-rep distances (reps as 16-column B tiles).
+    for (uint32_t r0 = 0; r0 < R; r0 += 64) {
+      f32x4 acc[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        if (ks >= KS) break;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const uint32_t r = r0 + 16 * t + (l % 16);
+          bf16x8 bh = bf16x8{}, bl = bf16x8{};
+          if (r < R) {
+            bh = *reinterpret_cast<const bf16x8 *>(I.rhi + size_t(r) * I.ldb + ks * 32 + koff);
+            bl = *reinterpret_cast<const bf16x8 *>(I.rlo + size_t(r) * I.ldb + ks * 32 + koff);
+          }
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Qh[ks], bh, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Qh[ks], bl, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ql[ks], bh, acc[t], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const uint32_t r = r0 + 16 * t + (l % 16);
+        if (r >= R) continue;
+        const float nr = I.rn2[r];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t q = 4 * (l / 16) + i;
+          dq[size_t(q) * a.Rmax + r] = fmaxf(__builtin_fmaf(-2.f, acc[t][i], qn4[i] + nr), 0.f);
+        }
+      }
+    }
+    __threadfence_block();  // the distance block is read back by other lanes below
+    // 3. Nearest rep per query (four lanes per query, lowest index among equals).
+    {
+      const uint32_t qi = l % 16;
+      float bv = FLT_MAX;
+      uint32_t bj = kDummy;
+      for (uint32_t j = l / 16; j < R; j += 4) {
+        const float v = dq[size_t(qi) * a.Rmax + j];
+        if (v < bv) {
+          bv = v;
+          bj = j;
+        }
+      }
+#pragma unroll
+      for (int s = 16; s <= 32; s <<= 1) {
+        const float ov = xor_lane(bv, s);
+        const uint32_t oj = xor_lane(bj, s);
+        if (pair_less(ov, oj, bv, bj)) {
+          bv = ov;
+          bj = oj;
+        }
+      }
+      if (l < 16) S.j0[qi] = bj;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // Lower bound of every list over the wave's queries (lane: list j0 + l/16, query l%16).
+    for (uint32_t jb = 0; jb < a.Rpad; jb += 4) {
+      const uint32_t j = jb + l / 16, q = l % 16;
+      float b = INFINITY;
+      if (j < R && q < nq) {
+        const uint32_t jq = S.j0[q];
+        const double Dqj = (double)dq[size_t(q) * a.Rmax + j];
+        const double Dq0 = (double)dq[size_t(q) * a.Rmax + jq];
+        const double lb = wave_list_bound(Dqj, Dq0, (double)S.eq[q], (double)I.rr[size_t(jq) * R + j],
+                                          (double)I.radius[j], a.e, j == jq);
+        b = (float)lb;
+        if ((double)b > lb) b = nextafterf(b, -INFINITY);
+      }
+#pragma unroll
+      for (int s = 1; s < 16; s <<= 1) b = fminf(b, xor_lane(b, s));
+      if ((l % 16) == 0 && j < a.Rpad) {
+        lbs[j] = j < R ? b : INFINITY;
+        ord[j] = j;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // Bitonic sort of (bound, list) over Rpad entries by the wave.
+    for (uint32_t kk = 2; kk <= a.Rpad; kk <<= 1)
+      for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+        for (uint32_t i = l; i < a.Rpad; i += kWave) {
+          const uint32_t p = i ^ jj;
+          if (p > i) {
+            const bool up = (i & kk) == 0;
+            const float vi = lbs[i], vp = lbs[p];
+            const uint32_t oi = ord[i], op = ord[p];
+            const bool sw = up ? (vp < vi || (vp == vi && op < oi)) : (vi < vp || (vi == vp && oi < op));
+            if (sw) {
+              lbs[i] = vp;
+              lbs[p] = vi;
+              ord[i] = op;
+              ord[p] = oi;
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    // 4. Lists in bound order.
+    float gmax = INFINITY;
+    unsigned long long pairs = 0, steps = 0;
+    for (uint32_t idx = 0; idx < R; ++idx) {
+      if (lbs[idx] > gmax) break;
+      const uint32_t j = ord[idx];
+      // Does some query still need list j?
+      bool need = false;
+      if (l < (int)nq) {
+        const uint32_t jq = S.j0[l];
+        const double lb = wave_list_bound((double)dq[size_t(l) * a.Rmax + j], (double)dq[size_t(l) * a.Rmax + jq],
+                                          (double)S.eq[l], (double)I.rr[size_t(jq) * R + j], (double)I.radius[j], a.e,
+                                          j == jq);
+        need = lb <= (double)S.gam[l];
+      }
+      if (__ballot(need) == 0) continue;
+      const uint32_t off = I.list_off[j], len = I.list_off[j + 1] - off;
+      pairs += (unsigned long long)nq * len;
+      steps += (len + kTP - 1) / kTP;
+      for (uint32_t p0 = 0; p0 < len; p0 += kTP) {
+        f32x4 acc[4];
+        uint32_t row[4], pid[4];
+        float pxn[4];
+        bool live[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+          const uint32_t pp = p0 + 16 * t + (l % 16);
+          live[t] = pp < len;
+          row[t] = off + (live[t] ? pp : 0);
+          pid[t] = I.perm[row[t]];
+          pxn[t] = I.xn2[row[t]];
+        }
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          if (ks >= KS) break;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            float v[8];
+            load8(I.xp + size_t(row[t]) * I.ld, ks * 32 + koff, I.d, live[t], v);
+            bf16x8 bh, bl;
+            split8(v, bh, bl);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Qh[ks], bh, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Qh[ks], bl, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ql[ks], bh, acc[t], 0, 0, 0);
+          }
+        }
+        float thd[4];
+        uint32_t thi[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          thd[i] = S.thd[4 * (l / 16) + i];
+          thi[i] = S.thi[4 * (l / 16) + i];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          if (!live[t]) continue;
+          const uint32_t id = pid[t];
+          const float xn = pxn[t];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const uint32_t q = 4 * (l / 16) + i;
+            if (q >= nq) continue;
+            const float v = fmaxf(__builtin_fmaf(-2.f, acc[t][i], qn4[i] + xn), 0.f);
+            if (pair_less(v, id, thd[i], thi[i])) {
+              const uint32_t s = atomicAdd(&S.cnt[q], 1u);
+              S.cd[q][s] = v;
+              S.ci[q][s] = id;
+            }
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // One lane per query folds its staged pairs into its sorted K' list.
+        if (l < (int)nq) {
+          const uint32_t n = S.cnt[l];
+          if (n) {
+            float kd = S.thd[l];
+            uint32_t ki = S.thi[l];
+            for (uint32_t c = 0; c < n; ++c) {
+              const float cv = S.cd[l][c];
+              const uint32_t cid = S.ci[l][c];
+              if (!pair_less(cv, cid, kd, ki)) continue;
+              int pos = (int)a.Kp - 1;
+              while (pos > 0 && pair_less(cv, cid, S.td[l][pos - 1], S.ti[l][pos - 1])) {
+                S.td[l][pos] = S.td[l][pos - 1];
+                S.ti[l][pos] = S.ti[l][pos - 1];
+                --pos;
+              }
+              S.td[l][pos] = cv;
+              S.ti[l][pos] = cid;
+              kd = S.td[l][a.Kp - 1];
+              ki = S.ti[l][a.Kp - 1];
+            }
+            S.thd[l] = kd;
+            S.thi[l] = ki;
+            S.cnt[l] = 0;
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      // Refresh the bounds: gamma_q = sqrt(D~_k + E_q) once a query holds k candidates.
+      float g = 0.f;
+      if (l < (int)nq) {
+        const float v = S.td[l][a.k - 1];
+        g = INFINITY;
+        if (v < FLT_MAX) {
+          const double gd = sqrt(((double)v + (double)S.eq[l]) * (1.0 + 1e-12)) * (1.0 + 1e-12);
+          g = (float)gd;
+          if ((double)g < gd) g = nextafterf(g, INFINITY);
+        }
+        S.gam[l] = g;
+      }
+#pragma unroll
+      for (int s = 1; s < kWave; s <<= 1) g = fmaxf(g, xor_lane(g, s));
+      gmax = g;
+      __builtin_amdgcn_wave_barrier();
+    }
+    // 5. Candidates and their certificate.
+    for (uint32_t i = l; i < nq * a.Kp; i += kWave) {
+      const uint32_t q = i / a.Kp, jj = i % a.Kp;
+      const size_t gq = size_t(I.qbase + T.q0 + q);
+      a.cand_i[gq * a.Kp + jj] = S.ti[q][jj];
+      a.cand_d[gq * a.Kp + jj] = S.td[q][jj];
+      if (jj == 0) {
+        const float vk = S.td[q][a.k - 1], vK = S.td[q][a.Kp - 1];
+        const double E = (double)S.eq[q];
+        const bool ok = vK >= FLT_MAX || (double)vK - E > ((double)vk + E) * (1.0 + 1e-12);
+        a.cert[gq] = ok ? 1u : 0u;
+      }
+    }
+    if (l == 0) {
+      atomicAdd(&a.stats[4], pairs);
+      atomicAdd(&a.stats[5], 2ull * I.d * (pairs + (unsigned long long)nq * R));
+      atomicAdd(&a.stats[6], steps);
+      atomicMax(&a.stats[7], wall_clock64() - t_begin);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 __global__ void reset_counter(uint32_t *c, unsigned long long *stats) {
   c[0] = 0;
   for (int i = 0; i < kBatchStats; ++i) stats[i] = 0;
@@ -538,6 +904,7 @@ struct rbc_batched {
   rbc::DevBuf<float> cand_d, scratch, qstage;
   rbc::DevBuf<unsigned long long> stats;
   uint32_t ntiles = 0, mtot = 0, grid = 0;
+  int cus = 0, wave_occ = 0;  // CUs; resident wave-kernel workgroups per CU (0: not used)
   std::map<std::vector<uint32_t>, uint32_t> tuned;  // batch shape -> tile size
   // Per index: ||x||^2 of its permuted rows and the largest ||x|| (the MFMA scan's bound).
   std::vector<rbc::DevBuf<float>> xn2;
@@ -568,8 +935,32 @@ int batch_tiles(rbc_batched *b, const std::vector<uint32_t> &m, uint32_t qt) {
   return RBC_OK;
 }
 
-int batch_launch(rbc_batched *b, uint32_t k, uint32_t Kp, hipStream_t st) {
+// The wave-autonomous kernel serves L2 with K' <= 32.
+bool wave_path(const rbc_batched *b, uint32_t Kp) { return b->metric == RBC_L2 && Kp <= kWK && b->wave_occ > 0; }
+
+// cfg: workgroups per CU for the wave kernel, else unused (the tiles carry the tile size).
+int batch_launch(rbc_batched *b, uint32_t k, uint32_t Kp, hipStream_t st, uint32_t cfg) {
   hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, st, b->counter.get(), b->stats.get());
+  if (wave_path(b, Kp)) {
+    WaveArgs wa;
+    wa.inst = b->dinst.get();
+    wa.tiles = b->dtiles.get();
+    wa.ntiles = b->ntiles;
+    wa.next = b->counter.get();
+    wa.scratch = b->scratch.get();
+    wa.Rmax = b->Rmax;
+    wa.Rpad = b->Rpad;
+    wa.k = k;
+    wa.Kp = Kp;
+    wa.e = fma_rel_err(b->dpad_max);
+    wa.cand_i = b->cand_i.get();
+    wa.cand_d = b->cand_d.get();
+    wa.cert = b->cert.get();
+    wa.stats = b->stats.get();
+    const uint32_t bpc = std::min(std::max(cfg, 1u), (uint32_t)b->wave_occ);
+    hipLaunchKernelGGL(batched_wave_kernel, dim3(b->cus * bpc), dim3(kNT), 4 * wave_lds_bytes(b->Rpad), st, wa);
+    RBC_HIP_TRY(hipGetLastError());
+  } else {
   BatchArgs a;
   a.inst = b->dinst.get();
   a.tiles = b->dtiles.get();
@@ -593,6 +984,7 @@ int batch_launch(rbc_batched *b, uint32_t k, uint32_t Kp, hipStream_t st) {
   else
     hipLaunchKernelGGL(batched_exact_kernel<0>, dim3(b->grid), dim3(kNT), lds, st, a, b->dpad_max);
   RBC_HIP_TRY(hipGetLastError());
+  }
   BSelArgs s{b->dinst.get(), b->qinst.get(), b->mtot, k, Kp, 64, b->cand_i.get(), b->cert.get(), b->stats.get()};
   dim3 g(cdiv(b->mtot, 4));
   for (uint32_t P : {64u, 128u}) {
@@ -687,8 +1079,20 @@ int rbc_batched_build(uint32_t count, const float *const *x, const uint32_t *n, 
   (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batched_exact_kernel<1>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, max_lds_bytes());
   b->grid = (uint32_t)prop.multiProcessorCount;  // one resident workgroup per CU
+  b->cus = prop.multiProcessorCount;
+  if (metric == RBC_L2) {
+    const size_t wlds = 4 * wave_lds_bytes(b->Rpad);
+    if ((int)wlds <= max_lds_bytes()) {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&batched_wave_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)wlds);
+      int occ = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batched_wave_kernel, kNT, wlds) == hipSuccess)
+        b->wave_occ = std::max(occ, 0);
+    }
+  }
+  const size_t scr = std::max(size_t(b->grid) * kTQ * b->Rmax, size_t(b->cus) * b->wave_occ * 4 * kWQ * b->Rmax);
   int s;
-  if ((s = b->scratch.alloc(size_t(b->grid) * kTQ * b->Rmax)) || (s = b->counter.alloc(1)) ||
+  if ((s = b->scratch.alloc(scr)) || (s = b->counter.alloc(1)) ||
       (s = b->stats.alloc(kBatchStats))) {
     delete b;
     return s;
@@ -746,6 +1150,10 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
     I.rr = h->rr.get();
     I.xn2 = b->xn2[i].get();
     I.xmax = b->xmax[i];
+    I.rhi = h->reps_b.get();
+    I.rlo = h->reps_lo.get();
+    I.rn2 = h->reps_n2.get();
+    I.ldb = h->ldb;
     {
       const double up = 0x1p-23, N = 3.0 * 32.0 * cdiv(h->d, 32) + 1.0;
       I.csplit = 3.1 * 0x1p-16 + 1.02 * N * up / (1.0 - N * up);
@@ -782,16 +1190,26 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
   uint32_t qt = 0;
   double tune_ms[3] = {0, 0, 0};
   auto found = b->tuned.find(key);
+  const bool wave = wave_path(b, Kp);
   if (found == b->tuned.end()) {
-    const uint32_t cand_qt[3] = {16, 32, 64};
+    // Wave kernel: 16-query tiles, and the workgroups per CU are tuned (a quarter, half and
+    // all of what fits); otherwise the tile size of the 64-slot tile kernel.
+    uint32_t cand_qt[3] = {16, 32, 64};
+    if (wave) {
+      const uint32_t o = (uint32_t)b->wave_occ;
+      cand_qt[0] = std::max(1u, o / 4);
+      cand_qt[1] = std::max(1u, o / 2);
+      cand_qt[2] = o;
+      RBC_TRY(batch_tiles(b, mv, kWQ));
+    }
     float best = 1e30f;
     hipEvent_t e0, e1;
     RBC_HIP_TRY(hipEventCreate(&e0));
     RBC_HIP_TRY(hipEventCreate(&e1));
     for (int c = 0; c < 3; ++c) {
-      RBC_TRY(batch_tiles(b, mv, cand_qt[c]));
+      if (!wave) RBC_TRY(batch_tiles(b, mv, cand_qt[c]));
       RBC_HIP_TRY(hipEventRecord(e0, st));
-      RBC_TRY(batch_launch(b, k, Kp, st));
+      RBC_TRY(batch_launch(b, k, Kp, st, cand_qt[c]));
       RBC_HIP_TRY(hipEventRecord(e1, st));
       RBC_HIP_TRY(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -818,10 +1236,10 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
       (void)hipGraphExecDestroy(b->plan.exec);
       b->plan.exec = nullptr;
     }
-    RBC_TRY(batch_tiles(b, mv, qt));
+    RBC_TRY(batch_tiles(b, mv, wave ? kWQ : qt));
     hipGraph_t graph;
     RBC_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int s = batch_launch(b, k, Kp, st);
+    int s = batch_launch(b, k, Kp, st, qt);
     hipError_t ce = hipStreamEndCapture(st, &graph);
     if (s != RBC_OK) return s;
     RBC_HIP_TRY(ce);
@@ -897,7 +1315,8 @@ int rbc_batched_kquery(rbc_batched *b, const float *const *q, const uint32_t *m,
   RBC_HIP_TRY(hipStreamSynchronize(st));
   if (stats) {
     stats->ms_graph = ms;
-    stats->tile_queries = qt;
+    stats->tile_queries = wave ? kWQ : qt;
+    stats->blocks_per_cu = wave ? std::min(qt, (uint32_t)b->wave_occ) : 1u;
     stats->graph_replayed = replay ? 1u : 0u;
     stats->uncertified = uncert;
     stats->tune_ms[0] = tune_ms[0];

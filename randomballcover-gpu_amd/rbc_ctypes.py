@@ -495,7 +495,7 @@ class ProposalMLP:
 
 class BatchedStats(ctypes.Structure):
     _fields_ = [("ms_graph", ctypes.c_double), ("tile_queries", _u32), ("graph_replayed", _u32),
-                ("uncertified", _u32), ("pad", _u32), ("tune_ms", ctypes.c_double * 3),
+                ("uncertified", _u32), ("blocks_per_cu", _u32), ("tune_ms", ctypes.c_double * 3),
                 ("guard_fp64", ctypes.c_double), ("pairs_scan", ctypes.c_double),
                 ("flops", ctypes.c_double), ("steps", ctypes.c_double), ("longest_tile_ms", ctypes.c_double)]
 
