@@ -53,7 +53,7 @@ PKG = os.path.join(ROOT, "randomballcover-gpu_amd")
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-SCAN_KERNEL = "bf16_scan_kernel"
+SCAN_KERNELS = ("sk_scan_kernel", "bf16_scan_kernel")  # sketch scan, full-dimension BF16 scan
 
 
 def _load(name, path):
@@ -110,7 +110,7 @@ def query_roofline(st, m, d, k, num_reps, ms_step, mlp_dims):
         "proposal": (2.0 * m * sum(a * b for a, b in zip(mlp_dims[:-1], mlp_dims[1:])) / mfma
                      if mlp_dims else 0.0),
         "probe": 2.0 * d * st["pairs_probe"] / mfma,
-        "scan": 2.0 * d * st["pairs_scan"] / mfma,
+        "scan": st["flops_scan"] * st["pairs_scan"] / mfma,
         "rerank": 4.0 * d * st["pairs_rerank"] / (HBM_PEAK_GBS * 1e9),
         "io": (4.0 * d * m + 8.0 * k * m) / (HBM_PEAK_GBS * 1e9),
     }
@@ -314,9 +314,10 @@ def measure_traffic(args, mlp_params, timeout=240):
             cal = _pmc_pass(counter, ["--copy-calibration", "1024"], os.path.join(out, "cal"), timeout)
             cal_kib = max(float(r["Counter_Value"]) for r in cal)
             factor = (1024.0 * 1024.0) / cal_kib  # 1 GiB = 1,048,576 KiB moved each way
-            rows = [r for r in _pmc_pass(counter, child, out, timeout) if SCAN_KERNEL in r["Kernel_Name"]]
+            rows = [r for r in _pmc_pass(counter, child, out, timeout)
+                    if any(kn in r["Kernel_Name"] for kn in SCAN_KERNELS)]
             if not rows:
-                return None, f"no {SCAN_KERNEL} dispatch in the {counter} pass"
+                return None, f"no scan kernel dispatch in the {counter} pass"
             last = max(rows, key=lambda r: int(r["Dispatch_Id"]))
             res[counter] = {"kib": float(last["Counter_Value"]), "copy_factor": round(factor, 4)}
             log(f"pmc {counter}: {float(last['Counter_Value']):.0f} KiB x {factor:.3f} (copy calibration)")
@@ -468,15 +469,39 @@ def main():
             f"{s_plain['pairs_probe']:.4e} -> {st['pairs_probe']:.4e}, scanned {s_plain['pairs_scan']:.4e} -> "
             f"{st['pairs_scan']:.4e}; identical results: {same}")
 
+    # The same search with the full-dimension BF16 scan instead of the sketch scan, timed
+    # like the headline arm (results must be identical).
+    if prec == rbc.PREC_BF16 and st["sketch"]:
+        ref_ids2, ref_ds2 = ids.clone(), ds.clone()
+        idx.set_sketch(rbc.SKETCH_OFF)
+        w_ns, s_ns_all = timed_arm()
+        idx.set_sketch(rbc.SKETCH_AUTO)
+        s_ns = s_ns_all[-1]
+        extra["without_sketch"] = {
+            "qps": round(m * args.steps / w_ns, 1), "ms_per_step": round(1e3 * w_ns / args.steps, 3),
+            "scan_ms": round(float(np.mean([s_["ms_scan"] for s_ in s_ns_all])), 3),
+            "scan_tflops": round(s_ns["flops_scan"] * s_ns["pairs_scan"] /
+                                 (float(np.mean([s_["ms_scan"] for s_ in s_ns_all])) * 1e-3) / 1e12, 2),
+            "same_results": bool(torch.equal(ref_ids2, ids) and torch.equal(ref_ds2, ds))}
+        extra["sketch_gain"] = round(qps / (m * args.steps / w_ns), 4)
+        extra["sketch"] = idx.sketch_info()
+        log(f"without sketch: {1e3*w_ns/args.steps:.2f} ms/step ({m*args.steps/w_ns:,.0f} q/s), scan "
+            f"{extra['without_sketch']['scan_ms']:.2f} ms; gain x{extra['sketch_gain']:.3f}; identical: "
+            f"{extra['without_sketch']['same_results']}")
+        # Leave the headline results in ids/ds for the checks below.
+        ids.copy_(ref_ids2)
+        ds.copy_(ref_ds2)
+
     # Roofline of the dominant kernel (the list scan): algorithmic FLOPs per launch over
     # its hipEvent-timed duration.
     scan_ms = float(np.mean([s["ms_scan"] for s in stats]))
-    flops = 2.0 * d * st["pairs_scan"]
+    flops = st["flops_scan"] * st["pairs_scan"]
     tflops = flops / (scan_ms * 1e-3) / 1e12
     if prec == rbc.PREC_BF16:
         roof = {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-                "traffic": None, "kernel": SCAN_KERNEL + "<4>",
+                "traffic": None, "kernel": "sk_scan_kernel<16>" if st["sketch"] else "bf16_scan_kernel<4>",
+                "flops_per_pair": st["flops_scan"],
                 "flops_per_launch": flops, "pairs_per_launch": st["pairs_scan"],
                 "alg_bytes_per_launch": st["bytes_scan"],
                 "alg_GBps": round(st["bytes_scan"] / (scan_ms * 1e-3) / 1e9, 1),

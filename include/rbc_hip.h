@@ -224,6 +224,8 @@ typedef struct {
   double guard_fp64;      /* candidates whose order was settled in FP64 */
   double fallbacks;       /* queries re-run because a candidate buffer overflowed */
   double ms_qreps, ms_probe, ms_plan, ms_scan, ms_select, ms_total; /* hipEvent times */
+  double flops_scan;      /* MFMA/FMA FLOPs the scan spends per (query, point) pair */
+  double sketch;          /* 1 if the scan ran on the projected sketches, else 0 */
 } rbc_query_stats;
 
 /* Exact k-NN: ids (original row numbers) and distances of the k nearest points, ascending
@@ -258,6 +260,41 @@ int rbc_exact_get_lists(const rbc_exact *h, uint32_t *rep_ids, uint32_t *list_of
                         uint32_t *perm, uint32_t *owner, float *radii);
 
 int rbc_exact_destroy(rbc_exact *h);
+
+/* Projected lower-bound filter of the BF16 k-query (L2, 32 < d <= 128).  The build keeps
+ * P, the 28 leading principal directions of a 65,536-row sample of the database
+ * (orthonormal rows, FP32), and for every point the BF16 coordinates of P (x - mu) with an
+ * error bound.  Since |P (q - x)| <= |q - x|, the list scan can test the 28-coordinate
+ * projections (one 16x16x32 MFMA per 16 x 16 pairs, norm and threshold carried in four
+ * extra coordinates) instead of the full rows; the exact FP32 re-rank decides the results,
+ * which are the same either way.  RBC_SKETCH_AUTO (the default) uses it when twice the
+ * data's variance outside P is at most 5% of the mean squared point-to-representative
+ * distance (the bound then stays tight); ON and OFF force it.  No reference counterpart. */
+#define RBC_SKETCH_AUTO 0
+#define RBC_SKETCH_OFF 1
+#define RBC_SKETCH_ON 2
+int rbc_exact_set_sketch(rbc_exact *h, int mode);
+
+typedef struct {
+  uint32_t dims;          /* projection rows (0: the index has no sketch) */
+  uint32_t active;        /* 1 if BF16 k-queries scan the sketches */
+  double norm_bound;      /* s_P >= |P|_2 */
+  double resid_var;       /* sample variance outside the projection (trace - kept) */
+  double total_var;       /* sample variance (trace of the covariance) */
+  double mean_owner_d2;   /* sample mean of |x - owner rep|^2 */
+} rbc_sketch_info;
+
+int rbc_exact_sketch_info(const rbc_exact *h, rbc_sketch_info *info);
+
+/* Sketches of m rows (host or device, leading dimension ldx): out[i * 32 + j] = BF16 bits
+ * of y~_j = bf16((P (x_i - mu))_j) for j < dims (the remaining coordinates are the query
+ * form's 1, 1, 0, 0), err[i] >= |y~ - P (x_i - mu)| (P and mu the stored FP32 values), n2[i]
+ * = |y~|^2; any output may be NULL.  The rows of the index itself give its stored sketch
+ * coordinates.  For tests of the bound. */
+int rbc_exact_sketch_rows(rbc_exact *h, const float *x, uint32_t m, uint32_t ldx, int mem,
+                          uint16_t *out, float *err, float *n2);
+/* P (dims x d, row-major) and mu (d) as stored (FP32, host buffers; either may be NULL). */
+int rbc_exact_sketch_basis(const rbc_exact *h, float *P, float *mu);
 
 /* Build once, query many (the reference's readme.txt:62-65; the reference has no file
  * format for an index): writes every array of the index to path, and loads such a file

@@ -38,6 +38,17 @@ struct rbc_exact {
   // L2: R x R denominators of the Voronoi-plane bound, 2 sqrt(rr (1 + e)) rounded up to
   // float (derived from rr at build and at load; e = fma_rel_err(dpad)).
   rbc::DevBuf<float> rr_den;
+  // Projected lower-bound filter of the BF16 query (L2, 32 < d <= 128; exact_sketch.hip):
+  // P (sk_dims x ld, FP32, the leading principal directions of a row sample), mu (ld), the
+  // permuted rows' 32-coordinate BF16 sketches, per list the largest sketch error bound and
+  // the largest |y~|^2, and s_P >= |P|_2.  sk_active: the BF16 query scans sketches.
+  uint32_t sk_dims = 0;
+  int sk_mode = RBC_SKETCH_AUTO;
+  bool sk_active = false;
+  double sk_scale = 1.0, sk_resid = 0.0, sk_total = 0.0, sk_owner_d2 = 0.0;
+  rbc::DevBuf<float> sk_P, sk_mu;
+  rbc::DevBuf<uint16_t> xs;
+  rbc::DevBuf<float> sk_lqe, sk_lnm;
   std::vector<uint32_t> h_list_off;
   rbc::Workspace ws;
   // Candidate-buffer capacity per query and distance filter (FP32, BF16), grown to 1.25x

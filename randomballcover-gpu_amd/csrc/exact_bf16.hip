@@ -21,6 +21,7 @@
 // register-staged LDS ring, each lane keeping its best few packed (value, slot) keys in
 // registers.
 #include "exact_query.hpp"
+#include "scan_sched.hpp"
 
 namespace rbc {
 
@@ -28,8 +29,6 @@ namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
-
-constexpr int kXCD = 8;
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
   uint32_t u = __float_as_uint(f);
@@ -286,111 +285,6 @@ __device__ __forceinline__ void dma_step(const void *xs, const void *ns, uint32_
 template <int N>
 __device__ __forceinline__ void wait_vm() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
-}
-
-// Splits the list-major scan items into kXCD contiguous ranges of about equal work and
-// clears the per-range counters.  The work of an item is its number of steps times (its
-// live waves + 1): the live waves issue the MFMAs, and every item pays the ring loads and
-// barriers of all its steps.  A list's items all have the same steps, so a range boundary
-// inside a list falls between its full (4-wave) items.  One block of 1024 threads; items
-// are numbered as make_bucket_items numbers them (bucket-major, ceil(count / kVQ) each).
-__global__ __launch_bounds__(1024) void scan_split(const uint32_t *qoff, const uint32_t *len, uint32_t R,
-                                                   uint32_t *bounds, uint32_t *next) {
-  __shared__ unsigned long long sw[16];
-  __shared__ uint32_t si[16];
-  __shared__ unsigned long long carry_w;
-  __shared__ uint32_t carry_i;
-  const int t = threadIdx.x, w = t / kWave, l = lane_id();
-  auto bucket = [&](uint32_t b, unsigned long long &wt, uint32_t &ni, uint32_t &steps) {
-    wt = 0;
-    ni = 0;
-    steps = 0;
-    if (b < R) {
-      const uint32_t cnt = qoff[b + 1] - qoff[b];
-      ni = (cnt + kVQ - 1) / kVQ;
-      steps = (len[b] + kVP - 1) / kVP;
-      wt = (unsigned long long)steps * ((cnt + 16 * kQB - 1) / (16 * kQB) + ni);
-    }
-  };
-  // Total work.
-  unsigned long long tw = 0;
-  uint32_t ti = 0;
-  for (uint32_t b = t; b < R; b += 1024) {
-    unsigned long long wt;
-    uint32_t ni, steps;
-    bucket(b, wt, ni, steps);
-    tw += wt;
-    ti += ni;
-  }
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    tw += __shfl_xor(tw, o, kWave);
-    ti += __shfl_xor(ti, o, kWave);
-  }
-  if (l == 0) {
-    sw[w] = tw;
-    si[w] = ti;
-  }
-  __syncthreads();
-  unsigned long long W = 0;
-  uint32_t NI = 0;
-  for (int i = 0; i < 16; ++i) {
-    W += sw[i];
-    NI += si[i];
-  }
-  // Boundaries not placed below (no work at all) leave everything to XCD 0's range.
-  if (t <= kXCD) bounds[t] = t == 0 ? 0 : NI;
-  if (t < kXCD) next[t] = 0;
-  if (t == 0) {
-    carry_w = 0;
-    carry_i = 0;
-  }
-  __syncthreads();
-  // Exclusive prefix of work and items, 1024 buckets at a time; the bucket holding target
-  // x W / kXCD places boundary x.
-  for (uint32_t c0 = 0; c0 < R; c0 += 1024) {
-    unsigned long long wt;
-    uint32_t ni, steps;
-    bucket(c0 + t, wt, ni, steps);
-    unsigned long long pw = wt;
-    uint32_t pi = ni;
-    for (int o = 1; o < kWave; o <<= 1) {
-      const unsigned long long nw = __shfl_up(pw, o, kWave);
-      const uint32_t nn = __shfl_up(pi, o, kWave);
-      if (l >= o) {
-        pw += nw;
-        pi += nn;
-      }
-    }
-    __syncthreads();
-    if (l == kWave - 1) {
-      sw[w] = pw;
-      si[w] = pi;
-    }
-    __syncthreads();
-    unsigned long long bw = carry_w;
-    uint32_t bi = carry_i;
-    for (int i = 0; i < w; ++i) {
-      bw += sw[i];
-      bi += si[i];
-    }
-    pw = bw + pw - wt;  // exclusive
-    pi = bi + pi - ni;
-    if (wt > 0) {
-      for (int x = 1; x < kXCD; ++x) {
-        const unsigned long long T = W * (unsigned long long)x / kXCD;
-        if (T >= pw && T < pw + wt) {
-          const unsigned long long full = (unsigned long long)steps * (4 + 1);
-          const uint32_t j = (uint32_t)min((unsigned long long)(ni - 1), (T - pw) / full);
-          bounds[x] = pi + j;
-        }
-      }
-    }
-    __syncthreads();
-    if (t == 1023) {
-      carry_w = pw + wt;
-      carry_i = pi + ni;
-    }
-  }
 }
 
 template <int KS>
@@ -872,7 +766,7 @@ int bf16_scan(rbc_exact *h, const uint16_t *qb, const float *qn, const float *qe
     set_error("BF16 filter supports d <= 128 (d=%u)", h->d);
     return RBC_ERR_LIMIT;
   }
-  hipLaunchKernelGGL(scan_split, dim3(1), dim3(1024), 0, st, list_qoff, h->list_len.get(), h->R, sched,
+  hipLaunchKernelGGL((scan_split_kernel<kVQ, kVP, 16 * kQB>), dim3(1), dim3(1024), 0, st, list_qoff, h->list_len.get(), h->R, sched,
                      sched + kXCD + 1);
   RBC_HIP_TRY(hipGetLastError());
   Bf16ScanArgs a;

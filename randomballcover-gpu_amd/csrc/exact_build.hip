@@ -12,6 +12,7 @@
 #include <unordered_set>
 
 #include "exact.hpp"
+#include "exact_sketch.hpp"
 
 namespace rbc {
 
@@ -339,6 +340,7 @@ int rbc_exact_build(const float *x, uint32_t n, uint32_t d, uint32_t ldx, uint32
   }
   RBC_TRY(voronoi_denominators(h, st));
   RBC_TRY(rep_splits(h, st));
+  RBC_TRY(sketch_build(h, st));
   h->h_list_off.resize(R + 1);
   RBC_HIP_TRY(hipMemcpy(h->h_list_off.data(), h->list_off.get(), (R + 1) * 4, hipMemcpyDeviceToHost));
   RBC_HIP_TRY(hipDeviceSynchronize());
@@ -364,7 +366,7 @@ int rbc_exact_get_info(const rbc_exact *h, rbc_exact_info *info) {
   info->min_list = mn;
   size_t b = 0;
   b += h->xp.n * 4 + h->xb.n * 2 + h->xb_norm.n * 4 + h->perm.n * 4 + h->inv.n * 4 + h->owner.n * 4;
-  b += h->reps.n * 4 + h->reps_b.n * 2 + h->rr.n * 4 + h->ws.bytes();
+  b += h->reps.n * 4 + h->reps_b.n * 2 + h->rr.n * 4 + h->xs.n * 2 + h->ws.bytes();
   info->device_bytes = b;
   return RBC_OK;
 }

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "exact.hpp"
+#include "exact_sketch.hpp"
 
 namespace rbc {
 
@@ -245,6 +246,7 @@ int rbc_exact_load(const char *path, rbc_exact **out) {
   RBC_TRY(check_structure(h->n, h->R, perm, inv, owner, off, len, rep_ids, radius, qerr, nmax));
   RBC_TRY(voronoi_denominators(h.get(), nullptr));
   RBC_TRY(rep_splits(h.get(), nullptr));
+  RBC_TRY(sketch_build(h.get(), nullptr));
   h->h_list_off = std::move(off);
   *out = h.release();
   return RBC_OK;

@@ -35,6 +35,7 @@
 #include "exact.hpp"
 #include "exact_device.hpp"
 #include "exact_query.hpp"
+#include "exact_sketch.hpp"
 
 namespace rbc {
 
@@ -719,6 +720,20 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   }
   RBC_TRY(rep_distances(h, qd, ldq, m, precision, Dqr, qb, qn, qe, qn2, qerr, st));
   const bool bf16 = precision == RBC_PREC_BF16;
+  // BF16 filter on an index whose data lie close to a low-dimensional subspace: the scan
+  // tests the queries' and points' projected sketches (exact_sketch.hip).
+  const bool sketch = bf16 && h->metric == RBC_L2 && h->sk_active;
+  uint16_t *qs = nullptr;
+  float *qs_e = nullptr, *qs_n = nullptr;
+  if (sketch) {
+    RBC_TRY(ws.get(kQ + 40, size_t(m) * kSkW * 2, &p));
+    qs = static_cast<uint16_t *>(p);
+    RBC_TRY(ws.get(kQ + 41, size_t(m) * 4, &p));
+    qs_e = static_cast<float *>(p);
+    RBC_TRY(ws.get(kQ + 42, size_t(m) * 4, &p));
+    qs_n = static_cast<float *>(p);
+    RBC_TRY(sketch_queries(h, qd, ldq, m, qs, qs_e, qs_n, st));
+  }
   // The BF16 probe keeps at most 32 candidates, so it serves k <= 32.  It scans the Pp
   // lists of the smallest lane minima of each query's rep distances in one pass.
   const bool bprobe = bf16 && k <= 32;
@@ -879,8 +894,9 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   RBC_HIP_TRY(hipGetLastError());
   DevBuf<ScanItem> sitems;
   uint32_t nsitems = 0;
+  const uint32_t scan_tile = sketch ? kSkScanQ : bf16 ? kBf16ScanQ : (uint32_t)kTQ;
   RBC_TRY(make_bucket_items(lcnt, h->list_off.get(), h->list_len.get(), R, sitems, &nsitems, ws,
-                            kSlotScanItems, st, bf16 ? kBf16ScanQ : (uint32_t)kTQ));
+                            kSlotScanItems, st, scan_tile));
   tm.mark();
   // 5. scan with candidate emission; re-run with a larger buffer on overflow.
   uint32_t &cap_seen = h->cand_cap[bf16 ? 1 : 0];
@@ -896,7 +912,11 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
     RBC_TRY(ws.get(kQ + 18, size_t(m) * cap * 4, &p));
     cand_i = static_cast<uint32_t *>(p);
     RBC_TRY(fill_u32(cand_n, 0, m, st));
-    if (precision == RBC_PREC_BF16) {
+    if (sketch) {
+      RBC_TRY(ws.get(kQ + 35, 32 * 4, &p));
+      RBC_TRY(sketch_scan(h, qs, qs_n, qs_e, gam, qlist, lcnt, sitems.get(), nsitems, static_cast<uint32_t *>(p),
+                          cand_i, cand_n, cap, st));
+    } else if (precision == RBC_PREC_BF16) {
       RBC_TRY(ws.get(kQ + 35, 32 * 4, &p));
       RBC_TRY(bf16_scan(h, qb, qn, qe, gam, qlist, lcnt, sitems.get(), nsitems, static_cast<uint32_t *>(p),
                         cand_d, cand_i, cand_n, cap, st));
@@ -983,8 +1003,8 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
     double pp = 0, ps = 0, bytes = 0;
     // Algorithmic bytes of the scan: each item reads its list once and its queries once
     // (BF16 rows of ldb, or FP32 rows of d); emitted candidates are not counted.
-    const double row_bytes = bf16 ? 2.0 * h->ldb : 4.0 * d;
-    const double tile = bf16 ? (double)kBf16ScanQ : (double)kTQ;
+    const double row_bytes = sketch ? 2.0 * kSkW : bf16 ? 2.0 * h->ldb : 4.0 * d;
+    const double tile = (double)scan_tile;
     for (uint32_t j = 0; j < R; ++j) {
       const double len = h->h_list_off[j + 1] - h->h_list_off[j];
       const double cq = hl[j + 1] - hl[j];
@@ -998,6 +1018,8 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
     stats->items_scan = nsitems;
     stats->lists_scanned = total_pairs;
     stats->bytes_scan = bytes;
+    stats->flops_scan = sketch ? 2.0 * kSkW : bf16 ? 2.0 * h->ldb : 2.0 * d;
+    stats->sketch = sketch ? 1.0 : 0.0;
     stats->guard_fp64 = (double)hs[0];
     stats->fallbacks = fallbacks;
     stats->ms_qreps = tm.ms(0, 1);

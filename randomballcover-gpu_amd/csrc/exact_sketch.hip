@@ -1,0 +1,763 @@
+// Projected lower-bound filter ("sketch") of the BF16 exact query (L2).
+//
+// For a matrix P whose rows are orthonormal, |P (q - x)| <= |q - x|: distances between
+// projections bound the true distances from below.  The build takes P = the top
+// kSkDims principal directions of a sample of the database (FP64 covariance on the GPU,
+// Jacobi eigenvectors on the host, re-orthonormalised, stored in FP32), and keeps for
+// every point the 32-coordinate BF16 row
+//     x^ = (y~_0 .. y~_27, -n_hi, -n_lo, 1, 1),   y = P (x - mu),  y~ = bf16(y),
+// with n_hi + n_lo <= |y~|^2 / 2 (both rounded down).  A query gets (y~_q, 1, 1, s_hi,
+// s_lo) with s_hi + s_lo >= s (rounded up), s the emission threshold of its (query, list)
+// pair, so that ONE v_mfma_f32_16x16x32_bf16 product x^ . q^ equals
+//     y~_q . y~_x - |y~_x|^2 / 2 + s,
+// and a point is emitted when that is >= 0: the threshold test needs no vector-ALU work
+// besides a max over the accumulators.  The scan thus costs 64 FLOPs per pair instead of
+// 2 ldb; the exact FP32 re-rank decides every result, as with the full-dimension filter.
+//
+// Exactness.  With s_P >= |P|_2 (from P P^T - I, FP64), any x with |q - x| <= G has
+// |P (q - x)| <= s_P G.  The computed projection y^ (FP32, one FMA per coordinate after an
+// FP32 subtraction of mu) satisfies |y^_j - (P (x - mu))_j| <= gamma_{d+1} sum_i |P_ji|
+// |x_i - mu_i| (evaluated in FP64 per coordinate), and e_x = that vector bound + |y^ - y~|
+// (FP64, rounded up), so |y~_q - y~_x| <= s_P G + e_q + e_x =: G + eta in the notation of
+// the full-dimension filter, and the same threshold algebra applies with the stored norms
+// |y~|^2 and the per-list maxima of e_x and |y~_x|^2.
+//
+// The filter pays off only where the data lie close to a low-dimensional subspace: the
+// bound loses about 2 x (variance outside the kSkDims directions) in squared distance.
+// The build decides (RBC_SKETCH_AUTO) by comparing that loss with the mean squared
+// distance of a point to its representative.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "exact_query.hpp"
+#include "exact_sketch.hpp"
+#include "scan_sched.hpp"
+
+namespace rbc {
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i32x4;
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(3))) const i32x4 lds_i32x4;
+
+constexpr uint32_t kSkSample = 65536;  // rows sampled for the mean and the covariance
+constexpr uint32_t kSkChunk = 2048;    // sample rows per covariance partial
+constexpr uint16_t kBf16One = 0x3F80, kBf16NegInf = 0xFF80, kBf16PosInf = 0x7F80;
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+// Smallest BF16 >= f (f not NaN).
+__device__ __forceinline__ uint16_t bf16_up(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0xFFFFu) == 0u || f < 0.f) return (uint16_t)(u >> 16);  // exact, or truncation towards zero
+  return (uint16_t)((u >> 16) + 1u);                              // may round up to +inf
+}
+// Largest BF16 <= f for f >= 0.
+__device__ __forceinline__ uint16_t bf16_down_pos(float f) { return (uint16_t)(__float_as_uint(f) >> 16); }
+__device__ __forceinline__ float f32_down(double v) {
+  float f = (float)v;
+  if ((double)f > v) f = nextafterf(f, -INFINITY);
+  return f;
+}
+__device__ __forceinline__ float f32_up(double v) {
+  if (v >= (double)FLT_MAX) return INFINITY;
+  float f = (float)v;
+  if ((double)f < v) f = nextafterf(f, INFINITY);
+  return f;
+}
+
+// ---- build: mean, covariance and the representative distances of a row sample ---------
+// Sample row i (i < S) is permuted row floor(i n / S).
+__device__ __forceinline__ uint32_t sample_row(uint32_t i, uint32_t S, uint32_t n) {
+  return (uint32_t)(((unsigned long long)i * n) / S);
+}
+
+// Block c0/64: coordinates c0 .. c0+63; thread (part = t / 64, c = t % 64) sums sample
+// rows part, part + 4, ... in FP64; the four parts are added in a fixed order.
+__global__ __launch_bounds__(256) void sk_mean(const float *xp, size_t ld, uint32_t n, uint32_t S, uint32_t d,
+                                               double *mu) {
+  __shared__ double part[4][64];
+  const int t = threadIdx.x, c = t % 64, pt = t / 64;
+  const uint32_t col = blockIdx.x * 64 + c;
+  double s = 0.0;
+  if (col < d)
+    for (uint32_t i = pt; i < S; i += 4) s += (double)xp[size_t(sample_row(i, S, n)) * ld + col];
+  part[pt][c] = s;
+  __syncthreads();
+  if (pt == 0 && col < d) mu[col] = (((part[0][c] + part[1][c]) + part[2][c]) + part[3][c]) / (double)S;
+}
+
+// Partial covariance sums: block (tile ti, tile tj, chunk ch) adds, over the chunk's
+// sample rows, (x_i - mu_i)(x_j - mu_j) for the 32 x 32 entries of the tile into
+// part[ch][i][j]; thread t holds row ti*32 + t/8 and columns tj*32 + (t%8)*4 .. +3.
+__global__ __launch_bounds__(256) void sk_cov_part(const float *xp, size_t ld, uint32_t n, uint32_t S, uint32_t d,
+                                                   const double *mu, double *part) {
+  __shared__ double si[64][33], sj[64][33];
+  const int t = threadIdx.x;
+  const uint32_t i0 = blockIdx.x * 32, j0 = blockIdx.y * 32, ch = blockIdx.z;
+  const uint32_t r = t / 8, c4 = (t % 8) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const uint32_t s0 = ch * kSkChunk, s1 = min(S, s0 + kSkChunk);
+  for (uint32_t b = s0; b < s1; b += 64) {
+    __syncthreads();
+    for (int e = t; e < 64 * 32; e += 256) {
+      const uint32_t sr = e / 32, cc = e % 32, row = b + sr;
+      double vi = 0.0, vj = 0.0;
+      if (row < s1) {
+        const float *xr = xp + size_t(sample_row(row, S, n)) * ld;
+        if (i0 + cc < d) vi = (double)xr[i0 + cc] - mu[i0 + cc];
+        if (j0 + cc < d) vj = (double)xr[j0 + cc] - mu[j0 + cc];
+      }
+      si[sr][cc] = vi;
+      sj[sr][cc] = vj;
+    }
+    __syncthreads();
+    for (int sr = 0; sr < 64; ++sr) {
+      const double a = si[sr][r];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] += a * sj[sr][c4 + u];
+    }
+  }
+  const uint32_t i = i0 + r;
+  if (i < d)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (j0 + c4 + u < d) part[(size_t(ch) * d + i) * d + j0 + c4 + u] = acc[u];
+}
+
+// C[i][j] = sum over chunks (in chunk order) / S.
+__global__ void sk_cov_sum(const double *part, uint32_t nch, uint32_t d, uint32_t S, double *C) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d * d) return;
+  double s = 0.0;
+  for (uint32_t c = 0; c < nch; ++c) s += part[size_t(c) * d * d + e];
+  C[e] = s / (double)S;
+}
+
+// Squared FP64 distance of each sample row to its owning representative (the scale the
+// lower bound's loss is compared with).
+__global__ void sk_owner_d2(const float *xp, size_t ld, uint32_t n, uint32_t S, uint32_t d, const uint32_t *perm,
+                            const uint32_t *owner, const float *reps, double *out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= S) return;
+  const uint32_t p = sample_row(i, S, n);
+  const float *xr = xp + size_t(p) * ld, *rr = reps + size_t(owner[perm[p]]) * ld;
+  double s = 0.0;
+  for (uint32_t c = 0; c < d; ++c) {
+    const double t = (double)xr[c] - (double)rr[c];
+    s += t * t;
+  }
+  out[i] = s;
+}
+
+// ---- projection of rows (database at build, queries per call) --------------------------
+// Eight rows per block, 32 threads per row; thread j < kSkDims computes y_j = sum_i
+// P[j][i] (x_i - mu_i) with one FMA per coordinate in coordinate order (FP32, after an
+// FP32 subtraction), and the per-coordinate bound gamma_{d+1} sum_i |P_ji| |x_i - mu_i|
+// in FP64.  Outputs: the BF16 row (database: -n_hi, -n_lo, 1, 1 in the last four
+// coordinates; queries: 1, 1, 0, 0), e = |y^ - y~| + |bound vector| rounded up, and
+// |y~|^2 (database: rounded up; queries: rounded to nearest).
+__global__ __launch_bounds__(256) void sk_project(const float *x, size_t ldx, uint32_t rows, uint32_t d,
+                                                  const float *P, uint32_t ldP, const float *mu, double gam_d1,
+                                                  int database, uint16_t *out, float *err, float *n2) {
+  __shared__ float Pt[kSkMaxD][kSkW];
+  __shared__ float xs[8][kSkMaxD];
+  __shared__ float xa[8][kSkMaxD];
+  const int t = threadIdx.x, r = t / kSkW, j = t % kSkW;
+  for (uint32_t e = t; e < d * kSkW; e += 256) {
+    const uint32_t i = e / kSkW, jj = e % kSkW;
+    Pt[i][jj] = jj < (uint32_t)kSkDims ? P[size_t(jj) * ldP + i] : 0.f;
+  }
+  const uint32_t row = blockIdx.x * 8 + r;
+  for (uint32_t i = j; i < d; i += kSkW) {
+    const float v = row < rows ? x[size_t(row) * ldx + i] : 0.f;
+    xs[r][i] = v - mu[i];
+    // |x_i - mu_i| rounded up, for the bound (the FP32 difference may round down).
+    xa[r][i] = f32_up(fabs((double)v - (double)mu[i]));
+  }
+  __syncthreads();
+  float y = 0.f;
+  double a = 0.0;
+  for (uint32_t i = 0; i < d; ++i) {
+    y = __builtin_fmaf(Pt[i][j], xs[r][i], y);
+    a += fabs((double)Pt[i][j]) * (double)xa[r][i];
+  }
+  double q2 = 0.0, b2 = 0.0, y2 = 0.0;
+  uint16_t h = 0;
+  if (j < kSkDims) {
+    h = bf16_rne(y);
+    const double hv = (double)bf16_val(h), dq = (double)y - hv, bj = gam_d1 * a;
+    q2 = dq * dq;
+    b2 = bj * bj;
+    y2 = hv * hv;
+  }
+  // Sums over the row's 32 threads (lanes 32 (r % 2) .. +31 of the wave).
+  for (int o = kSkW / 2; o > 0; o >>= 1) {
+    q2 += __shfl_xor(q2, o, kSkW);
+    b2 += __shfl_xor(b2, o, kSkW);
+    y2 += __shfl_xor(y2, o, kSkW);
+  }
+  if (row >= rows) return;
+  if (j >= kSkDims) {
+    if (database) {
+      // -n_hi, -n_lo with n_hi + n_lo <= |y~|^2 / 2.
+      const double N = 0.5 * y2;
+      const uint16_t nh = bf16_down_pos(f32_down(N));
+      const double rem = N - (double)bf16_val(nh);
+      const uint16_t nl = rem > 0.0 ? bf16_down_pos(f32_down(rem)) : (uint16_t)0;
+      h = j == kSkDims ? (uint16_t)(nh | 0x8000u) : j == kSkDims + 1 ? (uint16_t)(nl | 0x8000u) : kBf16One;
+    } else {
+      h = j < kSkDims + 2 ? kBf16One : (uint16_t)0;
+    }
+  }
+  out[size_t(row) * kSkW + j] = h;
+  if (j == 0) {
+    err[row] = f32_up((sqrt(q2) + sqrt(b2)) * (1.0 + 1e-12));
+    n2[row] = database ? f32_up(y2) : (float)y2;
+  }
+}
+
+// One block per list: the largest e and the largest |y~|^2 of its rows.
+__global__ __launch_bounds__(256) void sk_list_stats(const uint32_t *off, const float *err, const float *n2,
+                                                     float *lqe, float *lnm) {
+  __shared__ float se[256], sn[256];
+  const uint32_t j = blockIdx.x;
+  float me = 0.f, mn = 0.f;
+  for (uint32_t p = off[j] + threadIdx.x; p < off[j + 1]; p += 256) {
+    me = fmaxf(me, err[p]);
+    mn = fmaxf(mn, n2[p]);
+  }
+  se[threadIdx.x] = me;
+  sn[threadIdx.x] = mn;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      se[threadIdx.x] = fmaxf(se[threadIdx.x], se[threadIdx.x + s]);
+      sn[threadIdx.x] = fmaxf(sn[threadIdx.x], sn[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    lqe[j] = se[0];
+    lnm[j] = sn[0];
+  }
+}
+
+// ---- the list scan on sketches ----------------------------------------------------------
+// Work item = (one ownership list, up to 64 QB queries).  Wave w keeps queries 16 QB w ..
+// as QB MFMA B operands (one 16-byte fragment each: K = 32); the list's sketch rows, the
+// A operands, stream through a four-stage LDS ring, 64 points (4 KB) per step, filled
+// three steps ahead by direct global->LDS loads (one 1 KB load per wave and step).  A
+// point block (16 points) is one ds_read_b128 per lane feeding QB MFMAs whose
+// accumulators start at 0; lane l then holds points 4 (l/16) .. +3 of query l % 16 of
+// every block.  The emission test is acc >= 0 on the float bits as signed integers (a
+// max3 tree over the block's 4 QB accumulators); a point block with a hit re-tests its
+// values one by one and appends (point, query slot) to an LDS list that is flushed to
+// the per-query candidate buffers after the item's last step.
+// Ring rows are 64 bytes (4 chunks of 16); chunk c of row r sits in slot c ^ swz(r), so
+// that the 16 rows of a point block read by one ds_read_b128 fall in distinct banks.
+constexpr int kSkPB = 4;                 // point blocks per step
+constexpr int kSkVP = 16 * kSkPB;        // points per step
+constexpr int kSkStages = 4;
+constexpr int kSkStageB = kSkVP * 64;    // bytes per ring stage
+constexpr int kSkCS = 2048;              // LDS candidate entries per item
+
+__device__ __forceinline__ uint32_t sk_swz(uint32_t r) { return ((r >> 1) & 3u) ^ ((r >> 3) & 1u); }
+
+__device__ __forceinline__ uint32_t lds_addr(const void *p) { return (uint32_t)(uintptr_t)(const lds_void *)p; }
+
+// One direct global->LDS load of 16 bytes per lane (global_load_lds_dwordx4: lane l
+// writes LDS bytes [16 l, 16 l + 16) after the wave-uniform address in M0), issued from
+// asm so that the compiler neither drains it before unrelated instructions nor moves LDS
+// reads across it; M0 is saved and restored inside the block (s_nop: the wait state
+// between writing M0 and the LDS DMA that reads it).
+__device__ __forceinline__ void sk_dma(const void *base, uint32_t off, uint32_t m0) {
+  uint32_t keep;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %2\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %1, %3\n\t"
+      "s_mov_b32 m0, %0"
+      : "=&s"(keep)
+      : "v"(off), "s"(m0), "s"(base)
+      : "memory");
+}
+
+// vmcnt(n) with the other counters left alone (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void sk_wait_vm() {
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+__device__ __forceinline__ int imax3(int a, int b, int c) { return max(max(a, b), c); }
+
+// Sketch threshold of a (query, list) pair, packed as (s_lo << 16) | s_hi with s_hi +
+// s_lo >= s, s = ((G + eta)^2 - nq) / 2 + margin rounded up.  The MFMA adds the 32
+// products y~_q,i y~_x,i, -n_hi, -n_lo, s_hi, s_lo exactly and accumulates in FP32: the
+// error is at most c_gemm (gamma_32 + 6u') times their absolute sum, <= |y~_q||y~_x| +
+// |y~_x|^2 / 2 + |s| (1 + 2^-7) <= nq / 2 + nxm + 1.01 |s|, which the margin c_gemm (2 |s0|
+// + nq + nxm) + u' (nq + nxm) covers (|s| exceeds |s0| only by the margin itself).
+__device__ __forceinline__ uint32_t sk_threshold_pack(double G, float qe, float qn, float lqe, float lnm, double c_gemm) {
+  const double eta = (double)qe + (double)lqe, nq = (double)qn, nxm = (double)lnm;
+  const double T0 = (G + eta) * (G + eta);
+  const double s0 = 0.5 * (T0 - nq);
+  const double up = 0x1p-23;
+  const double margin = c_gemm * (2.0 * fabs(s0) + nq + nxm) + up * (nq + nxm) + 1e-30;
+  const float hs = f32_up((s0 + margin) * (1.0 + 1e-9));
+  if (hs >= INFINITY) return (uint32_t)kBf16PosInf;
+  const uint16_t sh = bf16_up(hs);
+  const float rem = hs - bf16_val(sh);  // exact (Sterbenz): |sh| within 2^-7 of |hs|
+  const uint16_t sl = bf16_up(rem);
+  return ((uint32_t)sl << 16) | sh;
+}
+
+template <int QB>
+__global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
+  constexpr int QW = 16 * QB, VQ = 4 * QW;
+  __shared__ uint32_t qrow[VQ];
+  __shared__ uint32_t sthr[VQ];
+  __shared__ __attribute__((aligned(16))) unsigned char ring[kSkStages * kSkStageB];
+  __shared__ uint32_t cs_p[kSkCS];
+  __shared__ uint16_t cs_q[kSkCS];
+  __shared__ uint32_t cs_n;
+  __shared__ uint32_t s_item;
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  const uint32_t xcd = blockIdx.x % kXCD;
+  const uint32_t lo = a.bounds[xcd], hi = a.bounds[xcd + 1];
+  const uint32_t wu = __builtin_amdgcn_readfirstlane((uint32_t)w);
+  // Ring loads: wave w fills block w (rows 16 w .. +15) of a stage; lane l writes row l/4,
+  // slot l%4, with the row's chunk that belongs in that slot.
+  const uint32_t m0w = lds_addr(ring) + wu * 1024;
+  const uint32_t drow = (uint32_t)l / 4, dchunk = ((uint32_t)l % 4) ^ sk_swz(drow);
+  // Ring reads: lane l reads row l%16, chunk l/16 of a point block.
+  const uint32_t roff = ((uint32_t)l % 16) * 64 + ((((uint32_t)l / 16) ^ sk_swz((uint32_t)l % 16)) * 16);
+  for (;;) {
+    if (t == 0) {
+      uint32_t v = lo + atomicAdd(&a.next[xcd], 1u);
+      if (v >= hi) {
+        v = kDummy;
+        for (uint32_t dd = 1; dd < (uint32_t)kXCD; ++dd) {
+          const uint32_t y = (xcd + dd) % kXCD;
+          const uint32_t u = a.bounds[y] + atomicAdd(&a.next[y], 1u);
+          if (u < a.bounds[y + 1]) {
+            v = u;
+            break;
+          }
+        }
+      }
+      s_item = v;
+    }
+    __syncthreads();
+    const uint32_t item = s_item;
+    if (item == kDummy) break;
+    const ScanItem it = a.items[item];
+    const uint32_t x0 = __builtin_amdgcn_readfirstlane(it.x_begin);
+    const uint32_t np = __builtin_amdgcn_readfirstlane(it.x_count);
+    const uint32_t nsteps = (np + kSkVP - 1) / kSkVP;
+    // Rows past the list end load clamped copies of its last row (every lane issues its
+    // load, so each wave issues exactly one load per step); the exact test drops them.
+    auto issue = [&](uint32_t step) {
+      const uint32_t p0 = step * kSkVP, last = np - 1 - p0;
+      const uint16_t *src = a.xs + size_t(x0 + p0) * kSkW;
+      sk_dma(src, min(wu * 16 + drow, last) * 64 + dchunk * 16, m0w + (step % kSkStages) * kSkStageB);
+    };
+    if (nsteps > 0) issue(0);
+    if (nsteps > 1) issue(1);
+    if (nsteps > 2) issue(2);
+    // Query slots of wave w: QW w .. QW w + QW - 1; lane l fetches the ids of slots
+    // QW w + 64 h + l and computes their thresholds for this list.
+    uint32_t myq[QB / 4];
+#pragma unroll
+    for (int h = 0; h < QB / 4; ++h) {
+      const uint32_t slot = QW * w + 64 * h + l;
+      myq[h] = slot < it.q_count ? a.qids[it.q_begin + slot] : kDummy;
+    }
+    const bool wave_live = (uint32_t)(w * QW) < it.q_count;
+#pragma unroll
+    for (int h = 0; h < QB / 4; ++h) {
+      const uint32_t slot = QW * w + 64 * h + l, q = myq[h];
+      qrow[slot] = q;
+      sthr[slot] = q != kDummy ? sk_threshold_pack(a.sP * a.gam[q], a.qe[q], a.qn[q], a.lqe[it.aux], a.lnm[it.aux],
+                                                   a.c_gemm)
+                               : (uint32_t)kBf16NegInf;
+    }
+    bf16x8 B[QB];
+    const uint32_t koff = ((uint32_t)l / 16) * 8;
+#pragma unroll
+    for (int qb = 0; qb < QB; ++qb) {
+      const uint32_t row = (uint32_t)__shfl((int)myq[qb / 4], (qb % 4) * 16 + (l % 16), kWave);
+      i32x4 v = i32x4{0, 0, 0, 0};
+      if (row != kDummy) v = *reinterpret_cast<const i32x4 *>(a.qs + size_t(row) * kSkW + koff);
+      B[qb] = __builtin_bit_cast(bf16x8, v);
+    }
+    if (t == 0) cs_n = 0;
+    __syncthreads();
+    // Lanes 48..63 hold coordinates 24..31: coordinates 30, 31 carry the threshold of
+    // the (query, list) pair (dummy slots: -inf, so that they never emit).
+    if (l >= 48) {
+#pragma unroll
+      for (int qb = 0; qb < QB; ++qb) {
+        i32x4 v = __builtin_bit_cast(i32x4, B[qb]);
+        v[3] = (int)sthr[QW * w + 16 * qb + (l % 16)];
+        B[qb] = __builtin_bit_cast(bf16x8, v);
+      }
+    }
+    // The query operands must have arrived before the first MFMA; waiting here (for the
+    // ring's first loads too) keeps the compiler from placing that wait inside the loop.
+    sk_wait_vm<0>();
+    for (uint32_t s = 0; s < nsteps; ++s) {
+      // This wave's load of step s is done when at most those of steps s+1, s+2 (those
+      // that exist) are outstanding; the barrier then publishes every wave's.
+      if (s + 2 < nsteps)
+        sk_wait_vm<2>();
+      else if (s + 1 < nsteps)
+        sk_wait_vm<1>();
+      else
+        sk_wait_vm<0>();
+      asm volatile("s_barrier" ::: "memory");
+      if (s + 3 < nsteps) issue(s + 3);
+      if (!wave_live) continue;
+      const uint32_t so = (s % kSkStages) * kSkStageB;
+      const uint32_t left = np - s * kSkVP;
+#pragma unroll
+      for (int pb = 0; pb < kSkPB; ++pb) {
+        if ((uint32_t)pb * 16 >= left) break;
+        const i32x4 raw = *reinterpret_cast<const lds_i32x4 *>((uintptr_t)(lds_addr(ring) + so + pb * 1024 + roff));
+        const bf16x8 A = __builtin_bit_cast(bf16x8, raw);
+        f32x4 acc[QB];
+        __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb)
+          acc[qb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[qb], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        __builtin_amdgcn_s_setprio(0);
+        // max over the 4 QB accumulators as signed integers: >= 0 iff some float >= +0.
+        int mx[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) mx[c] = INT_MIN;
+#pragma unroll
+        for (int qb = 0; qb < QB; ++qb) {
+          mx[qb % 4] = imax3(mx[qb % 4], __float_as_int(acc[qb][0]), __float_as_int(acc[qb][1]));
+          mx[qb % 4] = imax3(mx[qb % 4], __float_as_int(acc[qb][2]), __float_as_int(acc[qb][3]));
+        }
+        const int m = max(imax3(mx[0], mx[1], mx[2]), mx[3]);
+        if (__any(m >= 0)) {
+          const uint32_t p0 = s * kSkVP + pb * 16 + ((uint32_t)l / 16) * 4;
+#pragma unroll
+          for (int qb = 0; qb < QB; ++qb) {
+            const int mq = max(imax3(__float_as_int(acc[qb][0]), __float_as_int(acc[qb][1]),
+                                     __float_as_int(acc[qb][2])),
+                               __float_as_int(acc[qb][3]));
+            if (mq < 0) continue;
+            const uint32_t qi = QW * w + 16 * qb + ((uint32_t)l % 16);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const uint32_t p = p0 + r;
+              if (__float_as_int(acc[qb][r]) >= 0 && p < np) {
+                const uint32_t slot = atomicAdd(&cs_n, 1u);
+                if (slot < (uint32_t)kSkCS) {
+                  cs_p[slot] = p;
+                  cs_q[slot] = (uint16_t)qi;
+                } else {
+                  const uint32_t qr = qrow[qi];
+                  const uint32_t gs = atomicAdd(&a.cand_n[qr], 1u);
+                  if (gs < a.cap) a.cand_i[size_t(qr) * a.cap + gs] = a.perm[RBC_CHECKED(x0 + p, a.n)];
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const uint32_t nc = min(cs_n, (uint32_t)kSkCS);
+    for (uint32_t i = t; i < nc; i += 256) {
+      const uint32_t qr = qrow[cs_q[i]];
+      const uint32_t gs = atomicAdd(&a.cand_n[qr], 1u);
+      if (gs < a.cap) a.cand_i[size_t(qr) * a.cap + gs] = a.perm[RBC_CHECKED(x0 + cs_p[i], a.n)];
+    }
+    // The next item rewrites qrow, sthr, cs_* and s_item.
+    __syncthreads();
+  }
+}
+
+// ---- host: symmetric eigenvectors (cyclic Jacobi) -----------------------------------------
+// A (n x n, row-major, symmetric) is diagonalised in place; V receives the eigenvectors as
+// columns.  Rotations zero a_pq with t = sign(th) / (|th| + sqrt(th^2 + 1)), th = (a_qq -
+// a_pp) / (2 a_pq); sweeps stop when the off-diagonal mass is below 1e-30 of the total.
+void jacobi_eigen(std::vector<double> &A, uint32_t n, std::vector<double> &V) {
+  V.assign(size_t(n) * n, 0.0);
+  for (uint32_t i = 0; i < n; ++i) V[size_t(i) * n + i] = 1.0;
+  double total = 0.0;
+  for (double v : A) total += v * v;
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0.0;
+    for (uint32_t p = 0; p < n; ++p)
+      for (uint32_t q = p + 1; q < n; ++q) off += 2.0 * A[size_t(p) * n + q] * A[size_t(p) * n + q];
+    if (off <= 1e-30 * total) break;
+    for (uint32_t p = 0; p + 1 < n; ++p)
+      for (uint32_t q = p + 1; q < n; ++q) {
+        const double apq = A[size_t(p) * n + q];
+        if (fabs(apq) <= 1e-300) continue;
+        const double th = (A[size_t(q) * n + q] - A[size_t(p) * n + p]) / (2.0 * apq);
+        const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        const double c = 1.0 / sqrt(tt * tt + 1.0), s = tt * c;
+        for (uint32_t k = 0; k < n; ++k) {  // columns p, q
+          const double akp = A[size_t(k) * n + p], akq = A[size_t(k) * n + q];
+          A[size_t(k) * n + p] = c * akp - s * akq;
+          A[size_t(k) * n + q] = s * akp + c * akq;
+        }
+        for (uint32_t k = 0; k < n; ++k) {  // rows p, q
+          const double apk = A[size_t(p) * n + k], aqk = A[size_t(q) * n + k];
+          A[size_t(p) * n + k] = c * apk - s * aqk;
+          A[size_t(q) * n + k] = s * apk + c * aqk;
+        }
+        for (uint32_t k = 0; k < n; ++k) {
+          const double vkp = V[size_t(k) * n + p], vkq = V[size_t(k) * n + q];
+          V[size_t(k) * n + p] = c * vkp - s * vkq;
+          V[size_t(k) * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+}
+
+}  // namespace
+
+int sketch_build(rbc_exact *h, hipStream_t st) {
+  h->sk_dims = 0;
+  h->sk_active = false;
+  // Only where the sketch row (32 BF16) is shorter than the full one, for L2.
+  if (h->metric != RBC_L2 || h->ldb < 64 || h->d > (uint32_t)kSkMaxD) return RBC_OK;
+  const uint32_t n = h->n, d = h->d, ld = h->ld, R = h->R;
+  const uint32_t S = std::min(n, kSkSample);
+  DevBuf<double> mu64, part, C64, od2;
+  const uint32_t nch = cdiv(S, kSkChunk);
+  RBC_TRY(mu64.alloc(d));
+  RBC_TRY(part.alloc(size_t(nch) * d * d));
+  RBC_TRY(C64.alloc(size_t(d) * d));
+  RBC_TRY(od2.alloc(S));
+  hipLaunchKernelGGL(sk_mean, dim3(cdiv(d, 64)), dim3(256), 0, st, h->xp.get(), (size_t)ld, n, S, d, mu64.get());
+  hipLaunchKernelGGL(sk_cov_part, dim3(cdiv(d, 32), cdiv(d, 32), nch), dim3(256), 0, st, h->xp.get(), (size_t)ld, n,
+                     S, d, mu64.get(), part.get());
+  hipLaunchKernelGGL(sk_cov_sum, dim3(cdiv(d * d, 256)), dim3(256), 0, st, part.get(), nch, d, S, C64.get());
+  hipLaunchKernelGGL(sk_owner_d2, dim3(cdiv(S, 256)), dim3(256), 0, st, h->xp.get(), (size_t)ld, n, S, d,
+                     h->perm.get(), h->owner.get(), h->reps.get(), od2.get());
+  RBC_HIP_TRY(hipGetLastError());
+  std::vector<double> mu(d), C(size_t(d) * d), o2(S);
+  RBC_HIP_TRY(hipMemcpyAsync(mu.data(), mu64.get(), d * 8, hipMemcpyDeviceToHost, st));
+  RBC_HIP_TRY(hipMemcpyAsync(C.data(), C64.get(), size_t(d) * d * 8, hipMemcpyDeviceToHost, st));
+  RBC_HIP_TRY(hipMemcpyAsync(o2.data(), od2.get(), size_t(S) * 8, hipMemcpyDeviceToHost, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  double trace = 0.0, mo2 = 0.0;
+  for (uint32_t i = 0; i < d; ++i) trace += C[size_t(i) * d + i];
+  for (double v : o2) mo2 += v;
+  mo2 /= (double)S;
+  std::vector<double> V;
+  jacobi_eigen(C, d, V);
+  std::vector<uint32_t> order(d);
+  for (uint32_t i = 0; i < d; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t b) { return C[size_t(a) * d + a] > C[size_t(b) * d + b]; });
+  const uint32_t D = kSkDims;
+  double kept = 0.0;
+  for (uint32_t r = 0; r < D; ++r) kept += C[size_t(order[r]) * d + order[r]];
+  // Rows of P: the D leading eigenvectors, re-orthonormalised (modified Gram-Schmidt,
+  // twice) in FP64, then rounded to FP32.
+  std::vector<double> P64(size_t(D) * d);
+  for (uint32_t r = 0; r < D; ++r)
+    for (uint32_t i = 0; i < d; ++i) P64[size_t(r) * d + i] = V[size_t(i) * d + order[r]];
+  for (int pass = 0; pass < 2; ++pass)
+    for (uint32_t r = 0; r < D; ++r) {
+      double *pr = &P64[size_t(r) * d];
+      for (uint32_t s = 0; s < r; ++s) {
+        const double *ps = &P64[size_t(s) * d];
+        double dot = 0.0;
+        for (uint32_t i = 0; i < d; ++i) dot += pr[i] * ps[i];
+        for (uint32_t i = 0; i < d; ++i) pr[i] -= dot * ps[i];
+      }
+      double nr = 0.0;
+      for (uint32_t i = 0; i < d; ++i) nr += pr[i] * pr[i];
+      nr = sqrt(nr);
+      for (uint32_t i = 0; i < d; ++i) pr[i] /= nr;
+    }
+  std::vector<float> Pf(size_t(D) * ld, 0.f), muf(ld, 0.f);
+  for (uint32_t r = 0; r < D; ++r)
+    for (uint32_t i = 0; i < d; ++i) Pf[size_t(r) * ld + i] = (float)P64[size_t(r) * d + i];
+  for (uint32_t i = 0; i < d; ++i) muf[i] = (float)mu[i];
+  // |P_f|_2^2 = |P_f P_f^T|_2 <= 1 + |P_f P_f^T - I|_F; the FP64 products of FP32 values
+  // are exact and the sums of d of them are within d 2^-53 of their absolute values.
+  double g2 = 0.0;
+  for (uint32_t r = 0; r < D; ++r)
+    for (uint32_t s = 0; s < D; ++s) {
+      double dot = 0.0;
+      for (uint32_t i = 0; i < d; ++i) dot += (double)Pf[size_t(r) * ld + i] * (double)Pf[size_t(s) * ld + i];
+      const double g = dot - (r == s ? 1.0 : 0.0);
+      g2 += g * g;
+    }
+  h->sk_scale = sqrt(1.0 + sqrt(g2) + 1e-12 + d * 0x1p-52) * (1.0 + 1e-12);
+  h->sk_resid = std::max(0.0, trace - kept);
+  h->sk_total = trace;
+  h->sk_owner_d2 = mo2;
+  RBC_TRY(h->sk_P.alloc(size_t(D) * ld));
+  RBC_TRY(h->sk_mu.alloc(ld));
+  RBC_HIP_TRY(hipMemcpyAsync(h->sk_P.get(), Pf.data(), Pf.size() * 4, hipMemcpyHostToDevice, st));
+  RBC_HIP_TRY(hipMemcpyAsync(h->sk_mu.get(), muf.data(), muf.size() * 4, hipMemcpyHostToDevice, st));
+  // Sketch rows of the permuted database and their per-list maxima.
+  RBC_TRY(h->xs.alloc(size_t(n) * kSkW));
+  DevBuf<float> err, n2;
+  RBC_TRY(err.alloc(n));
+  RBC_TRY(n2.alloc(n));
+  hipLaunchKernelGGL(sk_project, dim3(cdiv(n, 8)), dim3(256), 0, st, h->xp.get(), (size_t)ld, n, d, h->sk_P.get(), ld,
+                     h->sk_mu.get(), sketch_proj_err(d), 1, h->xs.get(), err.get(), n2.get());
+  RBC_TRY(h->sk_lqe.alloc(R));
+  RBC_TRY(h->sk_lnm.alloc(R));
+  hipLaunchKernelGGL(sk_list_stats, dim3(R), dim3(256), 0, st, h->list_off.get(), err.get(), n2.get(),
+                     h->sk_lqe.get(), h->sk_lnm.get());
+  RBC_HIP_TRY(hipGetLastError());
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  h->sk_dims = D;
+  sketch_set_mode(h, h->sk_mode);
+  return RBC_OK;
+}
+
+double sketch_proj_err(uint32_t d) {
+  const double u = 0x1p-24, N = (double)d + 1.0;
+  return N * u / (1.0 - N * u) * 1.01;
+}
+
+void sketch_set_mode(rbc_exact *h, int mode) {
+  h->sk_mode = mode;
+  const bool worth = 2.0 * h->sk_resid <= kSkAutoFrac * h->sk_owner_d2;
+  h->sk_active = h->sk_dims > 0 && (mode == RBC_SKETCH_ON || (mode == RBC_SKETCH_AUTO && worth));
+}
+
+int sketch_queries(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint16_t *qs, float *qe, float *qn,
+                   hipStream_t st) {
+  RBC_ARG_CHECK(h->sk_dims > 0, "the index has no sketch (L2, 32 < d <= %d only)", kSkMaxD);
+  if (m == 0) return RBC_OK;
+  hipLaunchKernelGGL(sk_project, dim3(cdiv(m, 8)), dim3(256), 0, st, q, ldq, m, h->d, h->sk_P.get(), h->ld,
+                     h->sk_mu.get(), sketch_proj_err(h->d), 0, qs, qe, qn);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+double sketch_gemm_err() {
+  const double up = 0x1p-23, K = (double)kSkW;
+  return K * up / (1.0 - K * up) + 6.0 * up;
+}
+
+template <int QB>
+static int launch_sk_scan(const SkScanArgs &a, uint32_t nitems, hipStream_t st) {
+  static int per_cu = 0, cus = 0;
+  if (per_cu == 0) {
+    int dev = 0;
+    RBC_HIP_TRY(hipGetDevice(&dev));
+    RBC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RBC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sk_scan_kernel<QB>, 256, 0));
+    per_cu = std::max(per_cu, 1);
+  }
+  uint32_t grid = (uint32_t)(cus * per_cu);
+  grid = std::min(grid, nitems + kXCD);
+  grid = cdiv(std::max(grid, (uint32_t)kXCD), kXCD) * kXCD;
+  hipLaunchKernelGGL(sk_scan_kernel<QB>, dim3(grid), dim3(256), 0, st, a);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
+
+int sketch_scan(rbc_exact *h, const uint16_t *qs, const float *qn, const float *qe, const double *gam,
+                const uint32_t *qlist, const uint32_t *list_qoff, const ScanItem *items, uint32_t nitems,
+                uint32_t *sched, uint32_t *cand_i, uint32_t *cand_n, uint32_t cap, hipStream_t st) {
+  if (nitems == 0) return RBC_OK;
+  constexpr uint32_t QB = kSkScanQ / 64;
+  hipLaunchKernelGGL((scan_split_kernel<kSkScanQ, kSkVP, 16 * QB>), dim3(1), dim3(1024), 0, st, list_qoff,
+                     h->list_len.get(), h->R, sched, sched + kXCD + 1);
+  RBC_HIP_TRY(hipGetLastError());
+  SkScanArgs a;
+  a.items = items;
+  a.bounds = sched;
+  a.next = sched + kXCD + 1;
+  a.qids = qlist;
+  a.qs = qs;
+  a.qn = qn;
+  a.qe = qe;
+  a.gam = gam;
+  a.sP = h->sk_scale * (1.0 + 1e-12);
+  a.c_gemm = sketch_gemm_err();
+  a.lqe = h->sk_lqe.get();
+  a.lnm = h->sk_lnm.get();
+  a.xs = h->xs.get();
+  a.perm = h->perm.get();
+  a.n = h->n;
+  a.cand_i = cand_i;
+  a.cand_n = cand_n;
+  a.cap = cap;
+  return launch_sk_scan<QB>(a, nitems, st);
+}
+
+}  // namespace rbc
+
+using namespace rbc;
+
+extern "C" {
+
+int rbc_exact_set_sketch(rbc_exact *h, int mode) {
+  RBC_ARG_CHECK(h, "NULL handle");
+  RBC_ARG_CHECK(mode == RBC_SKETCH_AUTO || mode == RBC_SKETCH_OFF || mode == RBC_SKETCH_ON, "unknown sketch mode %d",
+                mode);
+  RBC_ARG_CHECK(mode != RBC_SKETCH_ON || h->sk_dims > 0, "the index has no sketch (L2, 32 < d <= %d only)", kSkMaxD);
+  sketch_set_mode(h, mode);
+  return RBC_OK;
+}
+
+int rbc_exact_sketch_info(const rbc_exact *h, rbc_sketch_info *info) {
+  RBC_ARG_CHECK(h && info, "NULL argument");
+  info->dims = h->sk_dims;
+  info->active = h->sk_active ? 1u : 0u;
+  info->norm_bound = h->sk_scale;
+  info->resid_var = h->sk_resid;
+  info->total_var = h->sk_total;
+  info->mean_owner_d2 = h->sk_owner_d2;
+  return RBC_OK;
+}
+
+int rbc_exact_sketch_rows(rbc_exact *h, const float *x, uint32_t m, uint32_t ldx, int mem, uint16_t *out, float *err,
+                          float *n2) {
+  RBC_ARG_CHECK(h && (x || m == 0), "NULL argument");
+  RBC_ARG_CHECK(h->sk_dims > 0, "the index has no sketch (L2, 32 < d <= %d only)", kSkMaxD);
+  RBC_ARG_CHECK(ldx >= h->d, "ldx=%u below d=%u", ldx, h->d);
+  if (m == 0) return RBC_OK;
+  hipStream_t st = nullptr;
+  DevMat X;
+  RBC_TRY(X.stage(x, m, h->d, ldx, mem, st));
+  DevBuf<uint16_t> o;
+  DevBuf<float> e, n;
+  RBC_TRY(o.alloc(size_t(m) * kSkW));
+  RBC_TRY(e.alloc(m));
+  RBC_TRY(n.alloc(m));
+  RBC_TRY(sketch_queries(h, X.p, X.ld, m, o.get(), e.get(), n.get(), st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  if (out) RBC_TRY(copy_out(out, o.get(), size_t(m) * kSkW * 2, mem, st));
+  if (err) RBC_TRY(copy_out(err, e.get(), size_t(m) * 4, mem, st));
+  if (n2) RBC_TRY(copy_out(n2, n.get(), size_t(m) * 4, mem, st));
+  RBC_HIP_TRY(hipStreamSynchronize(st));
+  return RBC_OK;
+}
+
+int rbc_exact_sketch_basis(const rbc_exact *h, float *P, float *mu) {
+  RBC_ARG_CHECK(h, "NULL handle");
+  RBC_ARG_CHECK(h->sk_dims > 0, "the index has no sketch (L2, 32 < d <= %d only)", kSkMaxD);
+  if (P)
+    RBC_HIP_TRY(hipMemcpy2D(P, size_t(h->d) * 4, h->sk_P.get(), size_t(h->ld) * 4, size_t(h->d) * 4, h->sk_dims,
+                            hipMemcpyDeviceToHost));
+  if (mu) RBC_HIP_TRY(hipMemcpy(mu, h->sk_mu.get(), size_t(h->d) * 4, hipMemcpyDeviceToHost));
+  return RBC_OK;
+}
+
+}  // extern "C"

@@ -23,6 +23,7 @@ DUMMY_IDX = 0xFFFFFFFF
 L2, L1 = 0, 1
 MEM_HOST, MEM_DEVICE = 0, 1
 PREC_FP32, PREC_BF16 = 0, 1
+SKETCH_AUTO, SKETCH_OFF, SKETCH_ON = 0, 1, 2
 
 _u32 = ctypes.c_uint32
 _int = ctypes.c_int
@@ -52,10 +53,16 @@ class QueryStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "pairs_probe", "pairs_scan", "pairs_rerank", "items_scan", "lists_scanned", "bytes_scan",
         "guard_fp64", "fallbacks", "ms_qreps", "ms_probe", "ms_plan", "ms_scan", "ms_select",
-        "ms_total")]
+        "ms_total", "flops_scan", "sketch")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class SketchInfo(ctypes.Structure):
+    _fields_ = [("dims", _u32), ("active", _u32), ("norm_bound", ctypes.c_double),
+                ("resid_var", ctypes.c_double), ("total_var", ctypes.c_double),
+                ("mean_owner_d2", ctypes.c_double)]
 
 
 class ExactInfo(ctypes.Structure):
@@ -99,6 +106,10 @@ _SIGS = {
                                    _int, _vp]),
     "rbc_exact_get_lists": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rbc_exact_destroy": (_int, [_vp]),
+    "rbc_exact_set_sketch": (_int, [_vp, _int]),
+    "rbc_exact_sketch_info": (_int, [_vp, ctypes.POINTER(SketchInfo)]),
+    "rbc_exact_sketch_rows": (_int, [_vp, _vp, _u32, _u32, _int, _vp, _vp, _vp]),
+    "rbc_exact_sketch_basis": (_int, [_vp, _vp, _vp]),
     "rbc_exact_save": (_int, [_vp, ctypes.c_char_p]),
     "rbc_exact_load": (_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "rbc_exact_set_proposal": (_int, [_vp, _vp]),
@@ -369,6 +380,34 @@ class ExactRBC:
         _check(lib().rbc_exact_rep_dists(self.h, qp, m, q.shape[1], precision, D.ctypes.data, self.num_reps,
                                          ae.ctypes.data, ctypes.byref(rel), MEM_HOST, None))
         return D, ae, rel.value
+
+    def set_sketch(self, mode):
+        """rbc_exact_set_sketch: SKETCH_AUTO, SKETCH_OFF or SKETCH_ON."""
+        _check(lib().rbc_exact_set_sketch(self.h, mode))
+
+    def sketch_info(self):
+        inf = SketchInfo()
+        _check(lib().rbc_exact_sketch_info(self.h, ctypes.byref(inf)))
+        return {n: getattr(inf, n) for n, _ in inf._fields_}
+
+    def sketch_rows(self, x):
+        """rbc_exact_sketch_rows on a host array: (BF16 bits [m, 32] as uint16, err [m], n2 [m])."""
+        x, xp = _host(x, np.float32)
+        m = x.shape[0]
+        out = np.empty((m, 32), np.uint16)
+        err = np.empty(m, np.float32)
+        n2 = np.empty(m, np.float32)
+        _check(lib().rbc_exact_sketch_rows(self.h, xp, m, x.shape[1], MEM_HOST, out.ctypes.data,
+                                           err.ctypes.data, n2.ctypes.data))
+        return out, err, n2
+
+    def sketch_basis(self):
+        """rbc_exact_sketch_basis: (P [dims, d], mu [d]) as stored (FP32)."""
+        dims = self.sketch_info()["dims"]
+        P = np.empty((dims, self.d), np.float32)
+        mu = np.empty(self.d, np.float32)
+        _check(lib().rbc_exact_sketch_basis(self.h, P.ctypes.data, mu.ctypes.data))
+        return P, mu
 
     def info(self):
         inf = ExactInfo()
