@@ -353,6 +353,7 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
 // others; the per-query values are wave-uniform.
 constexpr uint32_t kPlanCQ = 64;  // plan_count: queries per block (fewer list_cnt atomics)
 
+template <int NPR>
 __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, uint32_t R,
                                                   const uint32_t *near,
                                                   const uint32_t *key1, const double *gam,
@@ -406,16 +407,16 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
             // argmin over all reps), so the nearest rep and each probed rep give a lower
             // bound, and the largest decides.
             // All loads first (independent of each other), then the tests.
-            uint32_t ji[1 + kPlaneReps];
-            float Dqi[1 + kPlaneReps], den[1 + kPlaneReps];
+            uint32_t ji[1 + NPR];
+            float Dqi[1 + NPR], den[1 + NPR];
             ji[0] = j0;
 #pragma unroll
-            for (uint32_t pi = 0; pi < kPlaneReps; ++pi) {
+            for (uint32_t pi = 0; pi < (uint32_t)NPR; ++pi) {
               const uint32_t r = pi < P ? planeR[size_t(q) * P + pi] : R;
               ji[1 + pi] = r < R && r != j0 ? r : R;
             }
 #pragma unroll
-            for (uint32_t pi = 0; pi <= kPlaneReps; ++pi) {
+            for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) {
               const bool ok = ji[pi] < R && ji[pi] != j;
               Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
               const float dn = ok ? rr_den[size_t(ji[pi]) * R + j] : 0.f;
@@ -435,7 +436,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
             const float cb1 = (float)(1.0 + e + 0x1p-21);
             bool far = false;
 #pragma unroll
-            for (uint32_t pi = 0; pi <= kPlaneReps; ++pi)
+            for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi)
               far |= A > __builtin_fmaf(Gf, den[pi], __builtin_fmaf(Dqi[pi], cb1, Eu));
             if (far) keep = false;
           }
@@ -684,7 +685,7 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   // An early error return must not leave the proposal's inference running on the side
   // stream over workspace that a later call reuses.
   SideStreamGuard side_guard;
-  const uint32_t Pprop = std::min<uint32_t>(kProbeLists, R);
+  const uint32_t Pprop = std::min<uint32_t>(h->probe_lists, R);
   if (h->proposal && precision == RBC_PREC_BF16 && k <= 32) {
     RBC_TRY(ws.get(kQ + 23, size_t(m) * Pprop * 4, &p));
     plists = static_cast<uint32_t *>(p);
@@ -738,14 +739,14 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   // lists of the smallest lane minima of each query's rep distances in one pass.
   const bool bprobe = bf16 && k <= 32;
   const uint32_t KpUse = bprobe ? std::min<uint32_t>(32, k + 8) : Kp;
-  const uint32_t Pp = bprobe ? std::min<uint32_t>(kProbeLists, R) : 0;
+  const uint32_t Pp = bprobe ? std::min<uint32_t>(h->probe_lists, R) : 0;
   // 2. row stats
   RBC_TRY(ws.get(kQ + 1, size_t(m) * 4, &p));
   uint32_t *near = static_cast<uint32_t *>(p);
   RBC_TRY(ws.get(kQ + 2, size_t(m) * 4, &p));
   float *rep_kth = static_cast<float *>(p);
   uint32_t *nearP = nullptr, *planeR = nullptr;
-  const uint32_t P2 = bprobe ? std::min<uint32_t>(kPlaneReps, R) : 0;
+  const uint32_t P2 = bprobe ? std::min<uint32_t>(h->plane_reps, R) : 0;
   if (bprobe) {
     RBC_TRY(ws.get(kQ + 31, size_t(m) * Pp * 4, &p));
     nearP = static_cast<uint32_t *>(p);
@@ -878,9 +879,19 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   uint32_t *cursor = static_cast<uint32_t *>(p);
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
-  hipLaunchKernelGGL(plan_count, dim3(cdiv(m, kPlanCQ), cdiv(R, kWave)), dim3(256), 0, st, Dqr, m, R, near,
-                     key1, gam, cert, h->radius.get(), h->rr_den.get(), e, qerr, h->metric, planeR, P2, Wq,
-                     bits, lcnt);
+  {
+    const dim3 pg(cdiv(m, kPlanCQ), cdiv(R, kWave));
+#define RBC_PLAN(NPR)                                                                                          \
+  hipLaunchKernelGGL((plan_count<NPR>), pg, dim3(256), 0, st, Dqr, m, R, near, key1, gam, cert, h->radius.get(), \
+                     h->rr_den.get(), e, qerr, h->metric, planeR, P2, Wq, bits, lcnt)
+    if (P2 <= 4)
+      RBC_PLAN(4);
+    else if (P2 <= 8)
+      RBC_PLAN(8);
+    else
+      RBC_PLAN(12);
+#undef RBC_PLAN
+  }
   RBC_HIP_TRY(hipGetLastError());
   RBC_TRY(exclusive_scan_u32(lcnt, lcnt, R, ws, kSlotPlanScan, st));
   RBC_TRY(ws.get(kQ + 13, 4, &p));
@@ -914,8 +925,10 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
     RBC_TRY(fill_u32(cand_n, 0, m, st));
     if (sketch) {
       RBC_TRY(ws.get(kQ + 35, 32 * 4, &p));
-      RBC_TRY(sketch_scan(h, qs, qs_n, qs_e, gam, qlist, lcnt, sitems.get(), nsitems, static_cast<uint32_t *>(p),
-                          cand_i, cand_n, cap, st));
+      uint32_t *sched = static_cast<uint32_t *>(p);
+      RBC_TRY(ws.get(kQ + 43, size_t(m) * 4, &p));
+      RBC_TRY(sketch_scan(h, m, qs, qs_e, qs_n, gam, static_cast<uint32_t *>(p), qlist, lcnt, sitems.get(), nsitems,
+                          sched, cand_i, cand_n, cap, st));
     } else if (precision == RBC_PREC_BF16) {
       RBC_TRY(ws.get(kQ + 35, 32 * 4, &p));
       RBC_TRY(bf16_scan(h, qb, qn, qe, gam, qlist, lcnt, sitems.get(), nsitems, static_cast<uint32_t *>(p),
@@ -1044,6 +1057,16 @@ int exact_query_common(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, ui
 }  // namespace rbc
 
 using namespace rbc;
+
+extern "C" int rbc_exact_set_search_params(rbc_exact *h, uint32_t probe_lists, uint32_t plane_reps) {
+  RBC_ARG_CHECK(h, "NULL handle");
+  RBC_ARG_CHECK(probe_lists >= 1 && probe_lists <= kProbeLists, "probe_lists=%u outside [1, %u]", probe_lists,
+                kProbeLists);
+  RBC_ARG_CHECK(plane_reps <= kPlaneReps, "plane_reps=%u above %u", plane_reps, kPlaneReps);
+  h->probe_lists = probe_lists;
+  h->plane_reps = plane_reps;
+  return RBC_OK;
+}
 
 extern "C" int rbc_exact_set_proposal_mass(rbc_exact *h, float mass) {
   RBC_ARG_CHECK(h, "NULL handle");

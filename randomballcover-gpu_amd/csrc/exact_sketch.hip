@@ -46,32 +46,7 @@ typedef __attribute__((address_space(3))) const i32x4 lds_i32x4;
 
 constexpr uint32_t kSkSample = 65536;  // rows sampled for the mean and the covariance
 constexpr uint32_t kSkChunk = 2048;    // sample rows per covariance partial
-constexpr uint16_t kBf16One = 0x3F80, kBf16NegInf = 0xFF80, kBf16PosInf = 0x7F80;
 
-__device__ __forceinline__ uint16_t bf16_rne(float f) {
-  const uint32_t u = __float_as_uint(f);
-  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
-}
-__device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
-// Smallest BF16 >= f (f not NaN).
-__device__ __forceinline__ uint16_t bf16_up(float f) {
-  const uint32_t u = __float_as_uint(f);
-  if ((u & 0xFFFFu) == 0u || f < 0.f) return (uint16_t)(u >> 16);  // exact, or truncation towards zero
-  return (uint16_t)((u >> 16) + 1u);                              // may round up to +inf
-}
-// Largest BF16 <= f for f >= 0.
-__device__ __forceinline__ uint16_t bf16_down_pos(float f) { return (uint16_t)(__float_as_uint(f) >> 16); }
-__device__ __forceinline__ float f32_down(double v) {
-  float f = (float)v;
-  if ((double)f > v) f = nextafterf(f, -INFINITY);
-  return f;
-}
-__device__ __forceinline__ float f32_up(double v) {
-  if (v >= (double)FLT_MAX) return INFINITY;
-  float f = (float)v;
-  if ((double)f < v) f = nextafterf(f, INFINITY);
-  return f;
-}
 
 // ---- build: mean, covariance and the representative distances of a row sample ---------
 // Sample row i (i < S) is permuted row floor(i n / S).
@@ -252,22 +227,21 @@ __global__ __launch_bounds__(256) void sk_list_stats(const uint32_t *off, const 
 
 // ---- the list scan on sketches ----------------------------------------------------------
 // Work item = (one ownership list, up to 64 QB queries).  Wave w keeps queries 16 QB w ..
-// as QB MFMA B operands (one 16-byte fragment each: K = 32); the list's sketch rows, the
-// A operands, stream through a four-stage LDS ring, 64 points (4 KB) per step, filled
-// three steps ahead by direct global->LDS loads (one 1 KB load per wave and step).  A
-// point block (16 points) is one ds_read_b128 per lane feeding QB MFMAs whose
-// accumulators start at 0; lane l then holds points 4 (l/16) .. +3 of query l % 16 of
-// every block.  The emission test is acc >= 0 on the float bits as signed integers (a
-// max3 tree over the block's 4 QB accumulators); a point block with a hit re-tests its
-// values one by one and appends (point, query slot) to an LDS list that is flushed to
-// the per-query candidate buffers after the item's last step.
+// as QB MFMA B operands (one 16-byte fragment each: K = 32, the query's threshold in
+// coordinates 30 and 31); the list's sketch rows, the A operands, stream through an
+// NST-stage LDS ring, 16 PB points per step, filled NST - 1 steps ahead by direct
+// global->LDS loads (PB / 4 loads of 1 KB per wave and step).  A point block (16 points)
+// is one ds_read_b128 per lane feeding QB MFMAs whose accumulators start at 0 (the next
+// block's fragment is read while they run); lane l then holds points 4 (l/16) .. +3 of
+// query l % 16 of every block.  The emission test is acc >= 0 on the float bits as signed
+// integers (a max3 tree over the block's 4 QB accumulators); a point block with a hit
+// re-tests its values one by one and appends (point, query slot) to an LDS list that is
+// flushed to the per-query candidate buffers after the item's last step.
 // Ring rows are 64 bytes (4 chunks of 16); chunk c of row r sits in slot c ^ swz(r), so
 // that the 16 rows of a point block read by one ds_read_b128 fall in distinct banks.
-constexpr int kSkPB = 4;                 // point blocks per step
-constexpr int kSkVP = 16 * kSkPB;        // points per step
-constexpr int kSkStages = 4;
-constexpr int kSkStageB = kSkVP * 64;    // bytes per ring stage
-constexpr int kSkCS = 2048;              // LDS candidate entries per item
+// Measured on C3-shaped data: 8 blocks of 16 queries per wave (4 waves per SIMD) beat 16
+// (2 waves per SIMD); steps of 128 points beat 64 (half the barriers); waves loading
+// their fragments from L2 without a shared ring were 15-20 % slower.
 
 __device__ __forceinline__ uint32_t sk_swz(uint32_t r) { return ((r >> 1) & 3u) ^ ((r >> 3) & 1u); }
 
@@ -299,34 +273,33 @@ __device__ __forceinline__ void sk_wait_vm() {
 
 __device__ __forceinline__ int imax3(int a, int b, int c) { return max(max(a, b), c); }
 
-// Sketch threshold of a (query, list) pair, packed as (s_lo << 16) | s_hi with s_hi +
-// s_lo >= s, s = ((G + eta)^2 - nq) / 2 + margin rounded up.  The MFMA adds the 32
-// products y~_q,i y~_x,i, -n_hi, -n_lo, s_hi, s_lo exactly and accumulates in FP32: the
-// error is at most c_gemm (gamma_32 + 6u') times their absolute sum, <= |y~_q||y~_x| +
-// |y~_x|^2 / 2 + |s| (1 + 2^-7) <= nq / 2 + nxm + 1.01 |s|, which the margin c_gemm (2 |s0|
-// + nq + nxm) + u' (nq + nxm) covers (|s| exceeds |s0| only by the margin itself).
-__device__ __forceinline__ uint32_t sk_threshold_pack(double G, float qe, float qn, float lqe, float lnm, double c_gemm) {
-  const double eta = (double)qe + (double)lqe, nq = (double)qn, nxm = (double)lnm;
-  const double T0 = (G + eta) * (G + eta);
-  const double s0 = 0.5 * (T0 - nq);
-  const double up = 0x1p-23;
-  const double margin = c_gemm * (2.0 * fabs(s0) + nq + nxm) + up * (nq + nxm) + 1e-30;
-  const float hs = f32_up((s0 + margin) * (1.0 + 1e-9));
-  if (hs >= INFINITY) return (uint32_t)kBf16PosInf;
-  const uint16_t sh = bf16_up(hs);
-  const float rem = hs - bf16_val(sh);  // exact (Sterbenz): |sh| within 2^-7 of |hs|
-  const uint16_t sl = bf16_up(rem);
-  return ((uint32_t)sl << 16) | sh;
+
+// Waits until this wave has at most n ring loads outstanding (n < 8).
+__device__ __forceinline__ void sk_wait_upto(uint32_t n) {
+  switch (n) {
+    case 0: sk_wait_vm<0>(); break;
+    case 1: sk_wait_vm<1>(); break;
+    case 2: sk_wait_vm<2>(); break;
+    case 3: sk_wait_vm<3>(); break;
+    case 4: sk_wait_vm<4>(); break;
+    case 5: sk_wait_vm<5>(); break;
+    default: sk_wait_vm<6>(); break;
+  }
 }
 
-template <int QB>
+// QB: 16-query blocks per wave; CS: LDS candidate entries; PF: read the next point
+// block's fragment ahead; PB: point blocks (of 16) per step (4 or 8: each wave loads PB/4
+// KB per step); NST: ring stages (prefetch distance NST - 1 steps).
+template <int QB, int CS, bool PF, int PB, int NST>
 __global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
   constexpr int QW = 16 * QB, VQ = 4 * QW;
+  constexpr int kSkPB = PB, kSkVP = 16 * PB, kSkStages = NST, kSkStageB = kSkVP * 64, LW = PB / 4;
+  static_assert(PB % 4 == 0 && LW * (NST - 2) <= 6, "ring shape");
   __shared__ uint32_t qrow[VQ];
   __shared__ uint32_t sthr[VQ];
   __shared__ __attribute__((aligned(16))) unsigned char ring[kSkStages * kSkStageB];
-  __shared__ uint32_t cs_p[kSkCS];
-  __shared__ uint16_t cs_q[kSkCS];
+  __shared__ uint32_t cs_p[CS];
+  __shared__ uint16_t cs_q[CS];
   __shared__ uint32_t cs_n;
   __shared__ uint32_t s_item;
   const int t = threadIdx.x, w = t / kWave, l = lane_id();
@@ -367,28 +340,26 @@ __global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
     auto issue = [&](uint32_t step) {
       const uint32_t p0 = step * kSkVP, last = np - 1 - p0;
       const uint16_t *src = a.xs + size_t(x0 + p0) * kSkW;
-      sk_dma(src, min(wu * 16 + drow, last) * 64 + dchunk * 16, m0w + (step % kSkStages) * kSkStageB);
+#pragma unroll
+      for (int j = 0; j < LW; ++j)
+        sk_dma(src, min((wu + 4 * j) * 16 + drow, last) * 64 + dchunk * 16,
+               m0w + j * 4096 + (step % kSkStages) * kSkStageB);
     };
-    if (nsteps > 0) issue(0);
-    if (nsteps > 1) issue(1);
-    if (nsteps > 2) issue(2);
+#pragma unroll
+    for (int s0 = 0; s0 < NST - 1; ++s0)
+      if ((uint32_t)s0 < nsteps) issue(s0);
     // Query slots of wave w: QW w .. QW w + QW - 1; lane l fetches the ids of slots
     // QW w + 64 h + l and computes their thresholds for this list.
     uint32_t myq[QB / 4];
 #pragma unroll
     for (int h = 0; h < QB / 4; ++h) {
       const uint32_t slot = QW * w + 64 * h + l;
-      myq[h] = slot < it.q_count ? a.qids[it.q_begin + slot] : kDummy;
+      const bool live = slot < it.q_count;
+      myq[h] = live ? a.qids[it.q_begin + slot] : kDummy;
+      qrow[slot] = myq[h];
+      sthr[slot] = live ? a.qthr[myq[h]] : (uint32_t)kBf16NegInf;
     }
     const bool wave_live = (uint32_t)(w * QW) < it.q_count;
-#pragma unroll
-    for (int h = 0; h < QB / 4; ++h) {
-      const uint32_t slot = QW * w + 64 * h + l, q = myq[h];
-      qrow[slot] = q;
-      sthr[slot] = q != kDummy ? sk_threshold_pack(a.sP * a.gam[q], a.qe[q], a.qn[q], a.lqe[it.aux], a.lnm[it.aux],
-                                                   a.c_gemm)
-                               : (uint32_t)kBf16NegInf;
-    }
     bf16x8 B[QB];
     const uint32_t koff = ((uint32_t)l / 16) * 8;
 #pragma unroll
@@ -414,23 +385,27 @@ __global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
     // ring's first loads too) keeps the compiler from placing that wait inside the loop.
     sk_wait_vm<0>();
     for (uint32_t s = 0; s < nsteps; ++s) {
-      // This wave's load of step s is done when at most those of steps s+1, s+2 (those
-      // that exist) are outstanding; the barrier then publishes every wave's.
-      if (s + 2 < nsteps)
-        sk_wait_vm<2>();
-      else if (s + 1 < nsteps)
-        sk_wait_vm<1>();
-      else
-        sk_wait_vm<0>();
+      // This wave's loads of step s are done when at most those of the following NST - 2
+      // steps (those that exist) are outstanding; the barrier then publishes every wave's.
+      sk_wait_upto(LW * min((uint32_t)(NST - 2), nsteps - 1 - s));
       asm volatile("s_barrier" ::: "memory");
-      if (s + 3 < nsteps) issue(s + 3);
+      if (s + NST - 1 < nsteps) issue(s + NST - 1);
       if (!wave_live) continue;
       const uint32_t so = (s % kSkStages) * kSkStageB;
       const uint32_t left = np - s * kSkVP;
+      const uint32_t rbase = lds_addr(ring) + so + roff;
+      i32x4 rawn = *reinterpret_cast<const lds_i32x4 *>((uintptr_t)rbase);
 #pragma unroll
       for (int pb = 0; pb < kSkPB; ++pb) {
         if ((uint32_t)pb * 16 >= left) break;
-        const i32x4 raw = *reinterpret_cast<const lds_i32x4 *>((uintptr_t)(lds_addr(ring) + so + pb * 1024 + roff));
+        i32x4 raw;
+        if constexpr (PF) {
+          // The next block's fragment is read while this block's MFMAs run.
+          raw = rawn;
+          if (pb + 1 < kSkPB) rawn = *reinterpret_cast<const lds_i32x4 *>((uintptr_t)(rbase + (pb + 1) * 1024));
+        } else {
+          raw = pb == 0 ? rawn : *reinterpret_cast<const lds_i32x4 *>((uintptr_t)(rbase + pb * 1024));
+        }
         const bf16x8 A = __builtin_bit_cast(bf16x8, raw);
         f32x4 acc[QB];
         __builtin_amdgcn_s_setprio(1);
@@ -462,7 +437,7 @@ __global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
               const uint32_t p = p0 + r;
               if (__float_as_int(acc[qb][r]) >= 0 && p < np) {
                 const uint32_t slot = atomicAdd(&cs_n, 1u);
-                if (slot < (uint32_t)kSkCS) {
+                if (slot < (uint32_t)CS) {
                   cs_p[slot] = p;
                   cs_q[slot] = (uint16_t)qi;
                 } else {
@@ -477,7 +452,7 @@ __global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
       }
     }
     __syncthreads();
-    const uint32_t nc = min(cs_n, (uint32_t)kSkCS);
+    const uint32_t nc = min(cs_n, (uint32_t)CS);
     for (uint32_t i = t; i < nc; i += 256) {
       const uint32_t qr = qrow[cs_q[i]];
       const uint32_t gs = atomicAdd(&a.cand_n[qr], 1u);
@@ -621,7 +596,12 @@ int sketch_build(rbc_exact *h, hipStream_t st) {
   hipLaunchKernelGGL(sk_list_stats, dim3(R), dim3(256), 0, st, h->list_off.get(), err.get(), n2.get(),
                      h->sk_lqe.get(), h->sk_lnm.get());
   RBC_HIP_TRY(hipGetLastError());
+  std::vector<float> lqe(R), lnm(R);
+  RBC_HIP_TRY(hipMemcpyAsync(lqe.data(), h->sk_lqe.get(), size_t(R) * 4, hipMemcpyDeviceToHost, st));
+  RBC_HIP_TRY(hipMemcpyAsync(lnm.data(), h->sk_lnm.get(), size_t(R) * 4, hipMemcpyDeviceToHost, st));
   RBC_HIP_TRY(hipStreamSynchronize(st));
+  h->sk_gqe = *std::max_element(lqe.begin(), lqe.end());
+  h->sk_gnm = *std::max_element(lnm.begin(), lnm.end());
   h->sk_dims = D;
   sketch_set_mode(h, h->sk_mode);
   return RBC_OK;
@@ -653,30 +633,27 @@ double sketch_gemm_err() {
   return K * up / (1.0 - K * up) + 6.0 * up;
 }
 
-template <int QB>
-static int launch_sk_scan(const SkScanArgs &a, uint32_t nitems, hipStream_t st) {
-  static int per_cu = 0, cus = 0;
-  if (per_cu == 0) {
-    int dev = 0;
-    RBC_HIP_TRY(hipGetDevice(&dev));
-    RBC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    RBC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sk_scan_kernel<QB>, 256, 0));
-    per_cu = std::max(per_cu, 1);
-  }
-  uint32_t grid = (uint32_t)(cus * per_cu);
-  grid = std::min(grid, nitems + kXCD);
-  grid = cdiv(std::max(grid, (uint32_t)kXCD), kXCD) * kXCD;
-  hipLaunchKernelGGL(sk_scan_kernel<QB>, dim3(grid), dim3(256), 0, st, a);
-  RBC_HIP_TRY(hipGetLastError());
-  return RBC_OK;
+// Per query: the packed emission threshold (sk_threshold_pack) with the index-wide largest
+// point error bound and sketch norm, which bound every list's.
+__global__ void sk_query_thr(uint32_t m, const double *gam, const float *qe, const float *qn, double sP, float gqe,
+                             float gnm, double c_gemm, uint32_t *thr) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < m) thr[q] = sk_threshold_pack(sP * gam[q], qe[q], qn[q], gqe, gnm, c_gemm);
 }
 
-int sketch_scan(rbc_exact *h, const uint16_t *qs, const float *qn, const float *qe, const double *gam,
-                const uint32_t *qlist, const uint32_t *list_qoff, const ScanItem *items, uint32_t nitems,
-                uint32_t *sched, uint32_t *cand_i, uint32_t *cand_n, uint32_t cap, hipStream_t st) {
+// The scan's shape: 16-query blocks per wave, LDS candidate entries, next-fragment read,
+// point blocks per step, ring stages.
+constexpr int kSkQB = (int)kSkScanQ / 64, kSkCS = 1024, kSkPBStep = 8, kSkStagesN = 3;
+using SkScan = void (*)(SkScanArgs);
+constexpr SkScan kSkKernel = sk_scan_kernel<kSkQB, kSkCS, true, kSkPBStep, kSkStagesN>;
+
+int sketch_scan(rbc_exact *h, uint32_t m, const uint16_t *qs, const float *qe, const float *qn, const double *gam,
+                uint32_t *qthr, const uint32_t *qlist, const uint32_t *list_qoff, const ScanItem *items,
+                uint32_t nitems, uint32_t *sched, uint32_t *cand_i, uint32_t *cand_n, uint32_t cap, hipStream_t st) {
   if (nitems == 0) return RBC_OK;
-  constexpr uint32_t QB = kSkScanQ / 64;
-  hipLaunchKernelGGL((scan_split_kernel<kSkScanQ, kSkVP, 16 * QB>), dim3(1), dim3(1024), 0, st, list_qoff,
+  hipLaunchKernelGGL(sk_query_thr, dim3(cdiv(m, 256)), dim3(256), 0, st, m, gam, qe, qn,
+                     h->sk_scale * (1.0 + 1e-12), h->sk_gqe, h->sk_gnm, sketch_gemm_err(), qthr);
+  hipLaunchKernelGGL((scan_split_kernel<kSkScanQ, 16 * kSkPBStep, 16 * kSkQB>), dim3(1), dim3(1024), 0, st, list_qoff,
                      h->list_len.get(), h->R, sched, sched + kXCD + 1);
   RBC_HIP_TRY(hipGetLastError());
   SkScanArgs a;
@@ -684,21 +661,30 @@ int sketch_scan(rbc_exact *h, const uint16_t *qs, const float *qn, const float *
   a.bounds = sched;
   a.next = sched + kXCD + 1;
   a.qids = qlist;
+  a.qthr = qthr;
   a.qs = qs;
-  a.qn = qn;
-  a.qe = qe;
-  a.gam = gam;
-  a.sP = h->sk_scale * (1.0 + 1e-12);
-  a.c_gemm = sketch_gemm_err();
-  a.lqe = h->sk_lqe.get();
-  a.lnm = h->sk_lnm.get();
   a.xs = h->xs.get();
   a.perm = h->perm.get();
   a.n = h->n;
   a.cand_i = cand_i;
   a.cand_n = cand_n;
   a.cap = cap;
-  return launch_sk_scan<QB>(a, nitems, st);
+  // One persistent block per resident slot: (CUs x blocks per CU), a multiple of kXCD so
+  // that every XCD gets blocks.
+  static int per_cu = 0, cus = 0;
+  if (per_cu == 0) {
+    int dev = 0;
+    RBC_HIP_TRY(hipGetDevice(&dev));
+    RBC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    RBC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kSkKernel, 256, 0));
+    per_cu = std::max(per_cu, 1);
+  }
+  uint32_t grid = (uint32_t)(cus * per_cu);
+  grid = std::min(grid, nitems + kXCD);
+  grid = cdiv(std::max(grid, (uint32_t)kXCD), kXCD) * kXCD;
+  hipLaunchKernelGGL(kSkKernel, dim3(grid), dim3(256), 0, st, a);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
 }
 
 }  // namespace rbc

@@ -13,11 +13,11 @@ constexpr int kXCD = 8;  // XCDs of an MI355X; workgroup b is dispatched to XCD 
 // clears the per-range counters.  The work of an item is its number of steps times (its
 // live waves + 1): the live waves issue the MFMAs, and every item pays the ring loads and
 // barriers of all its steps.  A list's items all have the same steps, so a range boundary
-// inside a list falls between its full (4-wave) items.  One block of 1024 threads; items
+// inside a list falls between its full items (VQ / QW waves each).  One block of 1024 threads; items
 // are numbered as make_bucket_items numbers them (bucket-major, ceil(count / VQ) each).
 template <uint32_t VQ, uint32_t VP, uint32_t QW>
 __global__ __launch_bounds__(1024) void scan_split_kernel(const uint32_t *qoff, const uint32_t *len, uint32_t R,
-                                                   uint32_t *bounds, uint32_t *next) {
+                                                          uint32_t *bounds, uint32_t *next) {
   __shared__ unsigned long long sw[16];
   __shared__ uint32_t si[16];
   __shared__ unsigned long long carry_w;
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(1024) void scan_split_kernel(const uint32_t *qoff, 
       for (int x = 1; x < kXCD; ++x) {
         const unsigned long long T = W * (unsigned long long)x / kXCD;
         if (T >= pw && T < pw + wt) {
-          const unsigned long long full = (unsigned long long)steps * (4 + 1);
+          const unsigned long long full = (unsigned long long)steps * (VQ / QW + 1);
           const uint32_t j = (uint32_t)min((unsigned long long)(ni - 1), (T - pw) / full);
           bounds[x] = pi + j;
         }

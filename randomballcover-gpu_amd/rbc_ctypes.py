@@ -107,6 +107,7 @@ _SIGS = {
     "rbc_exact_get_lists": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "rbc_exact_destroy": (_int, [_vp]),
     "rbc_exact_set_sketch": (_int, [_vp, _int]),
+    "rbc_exact_set_search_params": (_int, [_vp, _u32, _u32]),
     "rbc_exact_sketch_info": (_int, [_vp, ctypes.POINTER(SketchInfo)]),
     "rbc_exact_sketch_rows": (_int, [_vp, _vp, _u32, _u32, _int, _vp, _vp, _vp]),
     "rbc_exact_sketch_basis": (_int, [_vp, _vp, _vp]),
@@ -384,6 +385,10 @@ class ExactRBC:
     def set_sketch(self, mode):
         """rbc_exact_set_sketch: SKETCH_AUTO, SKETCH_OFF or SKETCH_ON."""
         _check(lib().rbc_exact_set_sketch(self.h, mode))
+
+    def set_search_params(self, probe_lists=8, plane_reps=12):
+        """rbc_exact_set_search_params (BF16 query; results do not depend on them)."""
+        _check(lib().rbc_exact_set_search_params(self.h, probe_lists, plane_reps))
 
     def sketch_info(self):
         inf = SketchInfo()
