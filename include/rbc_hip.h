@@ -366,7 +366,8 @@ int rbc_mlp_propose_lists(rbc_mlp *m, const float *x, uint32_t rows, uint32_t ld
 int rbc_exact_set_proposal(rbc_exact *h, rbc_mlp *mlp);
 /* Search parameters of the BF16 k-query (no effect on results): the number of lists each
  * query probes for its first bound (1..8, default 8) and the number of nearby reps whose
- * Voronoi planes bound the lists in the plan besides the nearest rep (0..12, default 12). */
+ * Voronoi planes bound the lists in the plan besides the nearest rep (0..12; default 8
+ * when the scan runs on sketches, 12 otherwise). */
 int rbc_exact_set_search_params(rbc_exact *h, uint32_t probe_lists, uint32_t plane_reps);
 /* The proposal mass of the BF16 probe set, in (0, 1]. */
 int rbc_exact_set_proposal_mass(rbc_exact *h, float mass);

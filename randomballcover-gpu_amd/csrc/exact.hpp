@@ -51,8 +51,10 @@ struct rbc_exact {
   rbc::DevBuf<float> sk_lqe, sk_lnm;
   float sk_gqe = 0.f, sk_gnm = 0.f;  // largest sketch error bound and |y~|^2 over all points
   // BF16 query: lists probed per query (1..8) and reps whose Voronoi planes bound the lists
-  // in the plan (0..12), both beside the nearest rep (rbc_exact_set_search_params).
-  uint32_t probe_lists = 8, plane_reps = 12;
+  // in the plan (0..12), both beside the nearest rep (rbc_exact_set_search_params);
+  // plane_reps = kPlaneRepsAuto: 8 when the scan runs on sketches (cheap pairs), else 12.
+  static constexpr uint32_t kPlaneRepsAuto = 0xFFFFFFFFu;
+  uint32_t probe_lists = 8, plane_reps = kPlaneRepsAuto;
   std::vector<uint32_t> h_list_off;
   rbc::Workspace ws;
   // Candidate-buffer capacity per query and distance filter (FP32, BF16), grown to 1.25x

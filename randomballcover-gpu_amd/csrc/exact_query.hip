@@ -746,7 +746,8 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   RBC_TRY(ws.get(kQ + 2, size_t(m) * 4, &p));
   float *rep_kth = static_cast<float *>(p);
   uint32_t *nearP = nullptr, *planeR = nullptr;
-  const uint32_t P2 = bprobe ? std::min<uint32_t>(h->plane_reps, R) : 0;
+  const uint32_t prep = h->plane_reps != rbc_exact::kPlaneRepsAuto ? h->plane_reps : sketch ? 8u : 12u;
+  const uint32_t P2 = bprobe ? std::min<uint32_t>(prep, R) : 0;
   if (bprobe) {
     RBC_TRY(ws.get(kQ + 31, size_t(m) * Pp * 4, &p));
     nearP = static_cast<uint32_t *>(p);
@@ -817,7 +818,10 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
     uint32_t *eid = static_cast<uint32_t *>(p);
     RBC_TRY(fill_f32(ev, FLT_MAX, ne * Ke, st));
     RBC_TRY(fill_u32(eid, kDummy, ne * Ke, st));
-    RBC_TRY(bf16_probe(h, qb, qn, order, Pp, pitems.get(), npitems, Ke, eid, ev, st));
+    if (sketch)
+      RBC_TRY(sketch_probe(h, qs, qs_n, order, Pp, pitems.get(), npitems, Ke, eid, ev, st));
+    else
+      RBC_TRY(bf16_probe(h, qb, qn, order, Pp, pitems.get(), npitems, Ke, eid, ev, st));
     RBC_TRY(ws.get(kQ + 5, size_t(m) * KpUse * 4, &p));
     probe_d = static_cast<float *>(p);
     RBC_TRY(ws.get(kQ + 6, size_t(m) * KpUse * 4, &p));

@@ -463,6 +463,157 @@ __global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
   }
 }
 
+// ---- the probe on sketches -------------------------------------------------------------
+// Work item = (one list, up to 64 probe entries; entry e belongs to query e / P).  Wave w
+// keeps entries 16 w .. 16 w + 15 as one MFMA B operand: the query's sketch row with
+// coordinates 30, 31 set to about -|y~_q|^2 / 2 (the same value for all of a query's
+// entries), so that the MFMA gives acc = -D~/2, D~ the squared sketch distance (up to a
+// per-query constant).  acc <= 0 for every point but a coincident one, and the float bits
+// of non-positive values order by magnitude as unsigned integers, so a point's key is
+// its accumulator's bits with the low bits replaced by the point's slot (point block,
+// row) -- one v_and_or per pair -- and each lane keeps its kSkPL smallest keys with the
+// branch-free median insertion.  The list's sketch rows come through the scan's ring
+// (direct global->LDS loads, 128 points per step).  After the last step one thread per
+// entry merges the four lanes that saw disjoint quarters of the list into the entry's
+// ascending list of up to 4 kSkPL (id, D~) pairs.  Only a bound is taken from them: the
+// caller re-ranks its best candidates exactly.
+constexpr int kSkPL = 4;                     // keys kept per lane
+constexpr uint32_t kSkNoKey = 0xFF800000u;   // keys at or above this hold no point
+
+__device__ __forceinline__ uint32_t sk_med3(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// Inserts key into the ascending list t (the largest drops out): slot j takes the median
+// of (t[j-1], key, t[j]).
+__device__ __forceinline__ void sk_insert(uint32_t (&t)[kSkPL], uint32_t key) {
+#pragma unroll
+  for (int j = kSkPL - 1; j > 0; --j) t[j] = sk_med3(t[j - 1], key, t[j]);
+  t[0] = min(t[0], key);
+}
+
+template <int PB, int NST>
+__global__ __launch_bounds__(256) void sk_probe_kernel(SkProbeArgs a) {
+  constexpr int VP = 16 * PB, StageB = VP * 64, LW = PB / 4;
+  static_assert(PB % 4 == 0 && LW * (NST - 2) <= 6, "ring shape");
+  constexpr int kE = 64;  // entries per item
+  __shared__ uint32_t qent[kE];
+  __shared__ __attribute__((aligned(16))) unsigned char ring[NST * StageB];
+  const int t = threadIdx.x, w = t / kWave, l = lane_id();
+  // Workgroup b runs on XCD b % kXCD; each XCD takes a contiguous run of the list-major
+  // items, so the items of one list re-read it from the same L2.
+  const uint32_t per_xcd = (a.nitems + kXCD - 1) / kXCD;
+  const uint32_t item = (blockIdx.x % kXCD) * per_xcd + blockIdx.x / kXCD;
+  if (item >= a.nitems) return;
+  const ScanItem it = a.items[item];
+  const uint32_t x0 = __builtin_amdgcn_readfirstlane(it.x_begin);
+  const uint32_t np = __builtin_amdgcn_readfirstlane(it.x_count);
+  const uint32_t nsteps = (np + VP - 1) / VP;
+  const uint32_t wu = __builtin_amdgcn_readfirstlane((uint32_t)w);
+  const uint32_t m0w = lds_addr(ring) + wu * 1024;
+  const uint32_t drow = (uint32_t)l / 4, dchunk = ((uint32_t)l % 4) ^ sk_swz(drow);
+  const uint32_t roff = ((uint32_t)l % 16) * 64 + ((((uint32_t)l / 16) ^ sk_swz((uint32_t)l % 16)) * 16);
+  auto issue = [&](uint32_t step) {
+    const uint32_t p0 = step * VP, last = np - 1 - p0;
+    const uint16_t *src = a.xs + size_t(x0 + p0) * kSkW;
+#pragma unroll
+    for (int j = 0; j < LW; ++j)
+      sk_dma(src, min((wu + 4 * j) * 16 + drow, last) * 64 + dchunk * 16, m0w + j * 4096 + (step % NST) * StageB);
+  };
+#pragma unroll
+  for (int s0 = 0; s0 < NST - 1; ++s0)
+    if ((uint32_t)s0 < nsteps) issue(s0);
+  if (t < kE) qent[t] = (uint32_t)t < it.q_count ? a.qids[it.q_begin + t] : kDummy;
+  __syncthreads();
+  const uint32_t myent = qent[w * 16 + (l % 16)];
+  const uint32_t myrow = myent != kDummy ? myent / a.P : kDummy;
+  const bool wave_live = (uint32_t)(w * 16) < it.q_count;
+  i32x4 bv = i32x4{0, 0, 0, 0};
+  if (myrow != kDummy) bv = *reinterpret_cast<const i32x4 *>(a.qs + size_t(myrow) * kSkW + ((uint32_t)l / 16) * 8);
+  if (l >= 48) {
+    uint32_t sp = (uint32_t)kBf16NegInf;
+    if (myrow != kDummy) {
+      const float s = -0.5f * a.qn[myrow];
+      const uint16_t sh = bf16_rne(s);
+      sp = ((uint32_t)bf16_rne(s - bf16_val(sh)) << 16) | sh;
+    }
+    bv[3] = (int)sp;
+  }
+  const bf16x8 B = __builtin_bit_cast(bf16x8, bv);
+  const uint32_t vmask = ~((1u << a.idx_bits) - 1u);
+  uint32_t tk[kSkPL];
+#pragma unroll
+  for (int j = 0; j < kSkPL; ++j) tk[j] = 0xFFFFFFFFu;
+  sk_wait_vm<0>();
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    sk_wait_upto(LW * min((uint32_t)(NST - 2), nsteps - 1 - s));
+    asm volatile("s_barrier" ::: "memory");
+    if (s + NST - 1 < nsteps) issue(s + NST - 1);
+    if (!wave_live) continue;
+    const uint32_t rbase = lds_addr(ring) + (s % NST) * StageB + roff;
+    const uint32_t left = np - s * VP;
+#pragma unroll
+    for (int pb = 0; pb < PB; ++pb) {
+      if ((uint32_t)pb * 16 >= left) break;
+      const i32x4 raw = *reinterpret_cast<const lds_i32x4 *>((uintptr_t)(rbase + pb * 1024));
+      const f32x4 acc =
+          __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), B, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      const uint32_t slot0 = (s * PB + pb) * 4;
+      if ((uint32_t)pb * 16 + 16 <= left) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sk_insert(tk, (__float_as_uint(acc[r]) & vmask) | (slot0 + r));
+      } else {
+        // The list's last block: rows past its end hold copies of its last row.
+        const uint32_t p0 = (uint32_t)pb * 16 + ((uint32_t)l / 16) * 4;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t key = p0 + r < left ? (__float_as_uint(acc[r]) & vmask) | (slot0 + r) : 0xFFFFFFFFu;
+          sk_insert(tk, key);
+        }
+      }
+    }
+  }
+  // Hand the per-lane lists to one thread per entry through LDS (the ring is free now).
+  __syncthreads();
+  uint32_t *lk = reinterpret_cast<uint32_t *>(ring);  // [kE][4][kSkPL]
+  if (wave_live) {
+    const int e = w * 16 + (l % 16), g = l / 16;
+#pragma unroll
+    for (int j = 0; j < kSkPL; ++j) lk[(e * 4 + g) * kSkPL + j] = tk[j];
+  }
+  __syncthreads();
+  if (t >= kE || qent[t] == kDummy) return;
+  const uint32_t ent = qent[t];
+  const uint32_t smask = ~vmask;
+  uint32_t head[4] = {0, 0, 0, 0};
+  for (uint32_t j = 0; j < a.Kp; ++j) {
+    int best = -1;
+    uint32_t bk = 0xFFFFFFFFu;
+    for (int g = 0; g < 4; ++g) {
+      if (head[g] >= (uint32_t)kSkPL) continue;
+      const uint32_t key = lk[(t * 4 + g) * kSkPL + head[g]];
+      if (key < kSkNoKey && key < bk) {
+        best = g;
+        bk = key;
+      }
+    }
+    uint32_t id = kDummy;
+    float v = FLT_MAX;
+    if (best >= 0) {
+      ++head[best];
+      // Slot = point block * 4 + row; the lane's quarter of the block is g = best.
+      const uint32_t slot = bk & smask;
+      const uint32_t p = (slot >> 2) * 16 + best * 4 + (slot & 3u);
+      id = a.perm[RBC_CHECKED(x0 + p, a.n)];
+      v = -2.f * __uint_as_float(bk & vmask);
+    }
+    a.out_i[size_t(ent) * a.Kp + j] = id;
+    a.out_v[size_t(ent) * a.Kp + j] = v;
+  }
+}
+
 // ---- host: symmetric eigenvectors (cyclic Jacobi) -----------------------------------------
 // A (n x n, row-major, symmetric) is diagonalised in place; V receives the eigenvectors as
 // columns.  Rotations zero a_pq with t = sign(th) / (|th| + sqrt(th^2 + 1)), th = (a_qq -
@@ -646,6 +797,39 @@ __global__ void sk_query_thr(uint32_t m, const double *gam, const float *qe, con
 constexpr int kSkQB = (int)kSkScanQ / 64, kSkCS = 1024, kSkPBStep = 8, kSkStagesN = 3;
 using SkScan = void (*)(SkScanArgs);
 constexpr SkScan kSkKernel = sk_scan_kernel<kSkQB, kSkCS, true, kSkPBStep, kSkStagesN>;
+
+int sketch_probe(rbc_exact *h, const uint16_t *qs, const float *qn, const uint32_t *entries, uint32_t P,
+                 const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v, hipStream_t st) {
+  if (nitems == 0) return RBC_OK;
+  RBC_ARG_CHECK(Kp <= 4u * kSkPL, "sketch probe keeps at most %d candidates per entry (K'=%u)", 4 * kSkPL, Kp);
+  SkProbeArgs a;
+  a.items = items;
+  a.nitems = nitems;
+  a.qids = entries;
+  a.qs = qs;
+  a.qn = qn;
+  a.xs = h->xs.get();
+  a.perm = h->perm.get();
+  a.n = h->n;
+  a.Kp = Kp;
+  a.P = P;
+  // Slot bits: 4 slots per 16-point block of the longest list.
+  uint32_t max_len = 0;
+  for (uint32_t r = 0; r < h->R; ++r) max_len = std::max(max_len, h->h_list_off[r + 1] - h->h_list_off[r]);
+  const uint32_t slots = std::max(1u, (max_len + 15) / 16) * 4;
+  uint32_t bits = 2;
+  while ((1u << bits) < slots) ++bits;
+  if (bits > 22) {
+    set_error("sketch probe: a list of %u points is too long for its keys", max_len);
+    return RBC_ERR_LIMIT;
+  }
+  a.idx_bits = bits;
+  a.out_i = out_i;
+  a.out_v = out_v;
+  hipLaunchKernelGGL((sk_probe_kernel<8, 3>), dim3(kXCD * ((nitems + kXCD - 1) / kXCD)), dim3(256), 0, st, a);
+  RBC_HIP_TRY(hipGetLastError());
+  return RBC_OK;
+}
 
 int sketch_scan(rbc_exact *h, uint32_t m, const uint16_t *qs, const float *qe, const float *qn, const double *gam,
                 uint32_t *qthr, const uint32_t *qlist, const uint32_t *list_qoff, const ScanItem *items,

@@ -79,6 +79,20 @@ struct SkScanArgs {
   uint32_t cap;
 };
 
+struct SkProbeArgs {
+  const ScanItem *items;
+  uint32_t nitems;
+  const uint32_t *qids;    // list-major probe entries (entry e belongs to query e / P)
+  const uint16_t *qs;      // query sketch rows
+  const float *qn;         // |y~_q|^2
+  const uint16_t *xs;      // permuted database sketch rows
+  const uint32_t *perm;
+  uint32_t n, Kp, P;
+  uint32_t idx_bits;       // low key bits holding a point's slot (4 per 16-point block)
+  uint32_t *out_i;         // [entries][Kp] original ids, ascending approximate distance
+  float *out_v;            // [entries][Kp] approximate squared sketch distances (up to a per-query constant)
+};
+
 // Builds the projection, the database sketch rows and their per-list maxima from the
 // index's permuted rows (h->xp, lists, reps); no sketch (sk_dims = 0) unless L2 and
 // 32 < d <= kSkMaxD.
@@ -96,6 +110,10 @@ int sketch_queries(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, u
 // The list scan on sketches: items from make_bucket_items over list_qoff with tile
 // kSkScanQ; sched holds 2 kXCD + 1 uint32 of device scratch.  The emission threshold of a
 // query is computed once per call (with the index-wide largest point error and norm).
+// The probe on sketches: for every entry (query, list) of items (tile kBf16ProbeQ), its
+// Kp (<= 16) closest points by sketch distance, ascending, as bf16_probe returns them.
+int sketch_probe(rbc_exact *h, const uint16_t *qs, const float *qn, const uint32_t *entries, uint32_t P,
+                 const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v, hipStream_t st);
 // qthr is device scratch of m uint32 (one per query of the call).
 int sketch_scan(rbc_exact *h, uint32_t m, const uint16_t *qs, const float *qe, const float *qn, const double *gam,
                 uint32_t *qthr, const uint32_t *qlist, const uint32_t *list_qoff, const ScanItem *items,
