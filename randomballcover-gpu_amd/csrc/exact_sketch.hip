@@ -290,11 +290,11 @@ __device__ __forceinline__ void sk_wait_upto(uint32_t n) {
 // QB: 16-query blocks per wave; CS: LDS candidate entries; PF: read the next point
 // block's fragment ahead; PB: point blocks (of 16) per step (4 or 8: each wave loads PB/4
 // KB per step); NST: ring stages (prefetch distance NST - 1 steps).
-template <int QB, int CS, bool PF, int PB, int NST>
-__global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
-  constexpr int QW = 16 * QB, VQ = 4 * QW;
-  constexpr int kSkPB = PB, kSkVP = 16 * PB, kSkStages = NST, kSkStageB = kSkVP * 64, LW = PB / 4;
-  static_assert(PB % 4 == 0 && LW * (NST - 2) <= 6, "ring shape");
+template <int QB, int CS, bool PF, int PB, int NST, int NW>
+__global__ __launch_bounds__(64 * NW) void sk_scan_kernel(SkScanArgs a) {
+  constexpr int QW = 16 * QB, VQ = NW * QW, NT = 64 * NW;
+  constexpr int kSkPB = PB, kSkVP = 16 * PB, kSkStages = NST, kSkStageB = kSkVP * 64, LW = PB / NW;
+  static_assert(PB % NW == 0 && LW * (NST - 2) <= 6, "ring shape");
   __shared__ uint32_t qrow[VQ];
   __shared__ uint32_t sthr[VQ];
   __shared__ __attribute__((aligned(16))) unsigned char ring[kSkStages * kSkStageB];
@@ -342,8 +342,8 @@ __global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
       const uint16_t *src = a.xs + size_t(x0 + p0) * kSkW;
 #pragma unroll
       for (int j = 0; j < LW; ++j)
-        sk_dma(src, min((wu + 4 * j) * 16 + drow, last) * 64 + dchunk * 16,
-               m0w + j * 4096 + (step % kSkStages) * kSkStageB);
+        sk_dma(src, min((wu + NW * j) * 16 + drow, last) * 64 + dchunk * 16,
+               m0w + j * (NW * 1024) + (step % kSkStages) * kSkStageB);
     };
 #pragma unroll
     for (int s0 = 0; s0 < NST - 1; ++s0)
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(256) void sk_scan_kernel(SkScanArgs a) {
     }
     __syncthreads();
     const uint32_t nc = min(cs_n, (uint32_t)CS);
-    for (uint32_t i = t; i < nc; i += 256) {
+    for (uint32_t i = t; i < nc; i += NT) {
       const uint32_t qr = qrow[cs_q[i]];
       const uint32_t gs = atomicAdd(&a.cand_n[qr], 1u);
       if (gs < a.cap) a.cand_i[size_t(qr) * a.cap + gs] = a.perm[RBC_CHECKED(x0 + cs_p[i], a.n)];
@@ -793,10 +793,11 @@ __global__ void sk_query_thr(uint32_t m, const double *gam, const float *qe, con
 }
 
 // The scan's shape: 16-query blocks per wave, LDS candidate entries, next-fragment read,
-// point blocks per step, ring stages.
-constexpr int kSkQB = (int)kSkScanQ / 64, kSkCS = 1024, kSkPBStep = 8, kSkStagesN = 3;
+// point blocks per step, ring stages, waves per workgroup.
+constexpr int kSkNW = (int)(kSkScanQ / 128), kSkQB = 8, kSkCS = 1024, kSkPBStep = 8, kSkStagesN = 3;
+static_assert(kSkNW * 16 * kSkQB == (int)kSkScanQ, "scan items are cut at kSkScanQ queries");
 using SkScan = void (*)(SkScanArgs);
-constexpr SkScan kSkKernel = sk_scan_kernel<kSkQB, kSkCS, true, kSkPBStep, kSkStagesN>;
+constexpr SkScan kSkKernel = sk_scan_kernel<kSkQB, kSkCS, true, kSkPBStep, kSkStagesN, kSkNW>;
 
 int sketch_probe(rbc_exact *h, const uint16_t *qs, const float *qn, const uint32_t *entries, uint32_t P,
                  const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v, hipStream_t st) {
@@ -860,13 +861,13 @@ int sketch_scan(rbc_exact *h, uint32_t m, const uint16_t *qs, const float *qe, c
     int dev = 0;
     RBC_HIP_TRY(hipGetDevice(&dev));
     RBC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    RBC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kSkKernel, 256, 0));
+    RBC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kSkKernel, 64 * kSkNW, 0));
     per_cu = std::max(per_cu, 1);
   }
   uint32_t grid = (uint32_t)(cus * per_cu);
   grid = std::min(grid, nitems + kXCD);
   grid = cdiv(std::max(grid, (uint32_t)kXCD), kXCD) * kXCD;
-  hipLaunchKernelGGL(kSkKernel, dim3(grid), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(kSkKernel, dim3(grid), dim3(64 * kSkNW), 0, st, a);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
