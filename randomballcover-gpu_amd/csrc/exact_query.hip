@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
                                                   const uint32_t *near,
                                                   const uint32_t *key1, const double *gam,
                                                   const uint32_t *cert, const float *radius,
-                                                  const float *rr_den, double e, const float *qerr, int metric,
+                                                  const uint16_t *rr_den, double e, const float *qerr, int metric,
                                                   const uint32_t *planeR, uint32_t P,
                                                   uint32_t Wq, uint32_t *bits, uint32_t *list_cnt) {
   static_assert(kPlanCQ == 64, "four waves of 16 queries make two 32-query words");
@@ -419,7 +419,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
             for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) {
               const bool ok = ji[pi] < R && ji[pi] != j;
               Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
-              const float dn = ok ? rr_den[size_t(ji[pi]) * R + j] : 0.f;
+              const float dn = ok ? (float)__builtin_bit_cast(_Float16, rr_den[size_t(ji[pi]) * R + j]) : 0.f;
               den[pi] = dn > 0.f ? dn : INFINITY;  // no plane: never prunes
             }
             // The plane test num > G' den in FP32, with every rounding pushed towards
@@ -463,12 +463,15 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
 // Pass C: scatter query ids list-major.  One wave per (list blockIdx.y, 64 words = 2048
 // queries): lane l reads word w0 + l of the list's row (one 256-byte segment), a wave
 // prefix sum of the popcounts places every lane's ids after those of the lanes before it,
-// and one atomic per wave reserves the wave's range of the list.
+// and one atomic per wave reserves the wave's range of the list.  The ids are placed in
+// the wave's LDS buffer first and then copied out with coalesced stores.
 __global__ __launch_bounds__(256) void plan_scatter(uint32_t Wq, const uint32_t *bits,
                                                     const uint32_t *list_qoff, uint32_t *cursor,
                                                     uint32_t *qlist, uint32_t cap) {
+  __shared__ uint32_t stage[4][32 * kWave];
   const uint32_t j = blockIdx.y;
-  const uint32_t w0 = (blockIdx.x * 4 + threadIdx.x / kWave) * kWave;
+  const int wv = threadIdx.x / kWave;
+  const uint32_t w0 = (blockIdx.x * 4 + wv) * kWave;
   if (w0 >= Wq) return;
   const int l = lane_id();
   const uint32_t t = w0 + l;
@@ -484,12 +487,14 @@ __global__ __launch_bounds__(256) void plan_scatter(uint32_t Wq, const uint32_t 
   if (total == 0) return;
   uint32_t base = 0;
   if (l == 0) base = list_qoff[j] + atomicAdd(&cursor[j], total);
-  uint32_t pos = __shfl(base, 0, kWave) + incl - c;
+  base = __shfl(base, 0, kWave);
+  uint32_t pos = incl - c;
   while (word) {
-    qlist[RBC_CHECKED(pos, cap)] = t * 32 + (__ffs(word) - 1);
-    ++pos;
+    stage[wv][pos++] = t * 32 + (__ffs(word) - 1);
     word &= word - 1;
   }
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t i = l; i < total; i += kWave) qlist[RBC_CHECKED(base + i, cap)] = stage[wv][i];
 }
 
 }  // namespace
@@ -810,8 +815,9 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
     RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws,
                               kSlotProbeItems, st,
                               kBf16ProbeQ));
-    // Each entry holds at most kBf16ProbeKeep candidates; the merge reads that many.
-    const uint32_t Ke = std::min(KpUse, kBf16ProbeKeep);
+    // Each entry holds at most kBf16ProbeKeep candidates (the sketch probe kSkProbeKeep);
+    // the merge reads that many.
+    const uint32_t Ke = std::min(KpUse, sketch ? kSkProbeKeep : kBf16ProbeKeep);
     RBC_TRY(ws.get(kQ + 32, ne * Ke * 4, &p));
     float *ev = static_cast<float *>(p);
     RBC_TRY(ws.get(kQ + 33, ne * Ke * 4, &p));

@@ -312,25 +312,26 @@ __global__ __launch_bounds__(64 * NW) void sk_scan_kernel(SkScanArgs a) {
   const uint32_t drow = (uint32_t)l / 4, dchunk = ((uint32_t)l % 4) ^ sk_swz(drow);
   // Ring reads: lane l reads row l%16, chunk l/16 of a point block.
   const uint32_t roff = ((uint32_t)l % 16) * 64 + ((((uint32_t)l / 16) ^ sk_swz((uint32_t)l % 16)) * 16);
-  for (;;) {
-    if (t == 0) {
-      uint32_t v = lo + atomicAdd(&a.next[xcd], 1u);
-      if (v >= hi) {
-        v = kDummy;
-        for (uint32_t dd = 1; dd < (uint32_t)kXCD; ++dd) {
-          const uint32_t y = (xcd + dd) % kXCD;
-          const uint32_t u = a.bounds[y] + atomicAdd(&a.next[y], 1u);
-          if (u < a.bounds[y + 1]) {
-            v = u;
-            break;
-          }
-        }
-      }
-      s_item = v;
+  // An item index from this XCD's range, else (when it is exhausted) from the others, in
+  // XCD order after this one (so the XCDs finish together); kDummy when all are done.
+  auto take = [&](uint32_t v) -> uint32_t {
+    if (v < hi) return v;
+    for (uint32_t dd = 1; dd < (uint32_t)kXCD; ++dd) {
+      const uint32_t y = (xcd + dd) % kXCD;
+      const uint32_t u = a.bounds[y] + atomicAdd(&a.next[y], 1u);
+      if (u < a.bounds[y + 1]) return u;
     }
-    __syncthreads();
+    return kDummy;
+  };
+  if (t == 0) s_item = take(lo + atomicAdd(&a.next[xcd], 1u));
+  __syncthreads();
+  for (;;) {
     const uint32_t item = s_item;
     if (item == kDummy) break;
+    // The following item is requested now and read after this one (its atomic's latency
+    // hides behind the item's work).
+    uint32_t nxt = 0;
+    if (t == 0) nxt = lo + atomicAdd(&a.next[xcd], 1u);
     const ScanItem it = a.items[item];
     const uint32_t x0 = __builtin_amdgcn_readfirstlane(it.x_begin);
     const uint32_t np = __builtin_amdgcn_readfirstlane(it.x_count);
@@ -458,7 +459,8 @@ __global__ __launch_bounds__(64 * NW) void sk_scan_kernel(SkScanArgs a) {
       const uint32_t gs = atomicAdd(&a.cand_n[qr], 1u);
       if (gs < a.cap) a.cand_i[size_t(qr) * a.cap + gs] = a.perm[RBC_CHECKED(x0 + cs_p[i], a.n)];
     }
-    // The next item rewrites qrow, sthr, cs_* and s_item.
+    if (t == 0) s_item = take(nxt);
+    // The next item rewrites qrow, sthr and cs_*, after every thread has read them.
     __syncthreads();
   }
 }
@@ -477,7 +479,7 @@ __global__ __launch_bounds__(64 * NW) void sk_scan_kernel(SkScanArgs a) {
 // entry merges the four lanes that saw disjoint quarters of the list into the entry's
 // ascending list of up to 4 kSkPL (id, D~) pairs.  Only a bound is taken from them: the
 // caller re-ranks its best candidates exactly.
-constexpr int kSkPL = 4;                     // keys kept per lane
+constexpr int kSkPL = 2;                     // keys kept per lane
 constexpr uint32_t kSkNoKey = 0xFF800000u;   // keys at or above this hold no point
 
 __device__ __forceinline__ uint32_t sk_med3(uint32_t a, uint32_t b, uint32_t c) {
@@ -802,7 +804,8 @@ constexpr SkScan kSkKernel = sk_scan_kernel<kSkQB, kSkCS, true, kSkPBStep, kSkSt
 int sketch_probe(rbc_exact *h, const uint16_t *qs, const float *qn, const uint32_t *entries, uint32_t P,
                  const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v, hipStream_t st) {
   if (nitems == 0) return RBC_OK;
-  RBC_ARG_CHECK(Kp <= 4u * kSkPL, "sketch probe keeps at most %d candidates per entry (K'=%u)", 4 * kSkPL, Kp);
+  static_assert(4 * kSkPL == (int)kSkProbeKeep, "sketch probe entries hold 4 lanes x kSkPL keys");
+  RBC_ARG_CHECK(Kp <= kSkProbeKeep, "sketch probe keeps at most %u candidates per entry (K'=%u)", kSkProbeKeep, Kp);
   SkProbeArgs a;
   a.items = items;
   a.nitems = nitems;

@@ -13,6 +13,7 @@ constexpr int kSkDims = 28;    // projection rows (principal directions)
 constexpr int kSkW = 32;       // BF16 coordinates per sketch row (28 + 4 carrying norm and threshold)
 constexpr int kSkMaxD = 128;   // largest d with a sketch (the BF16 filter's limit)
 constexpr uint32_t kSkScanQ = 1024;  // queries per scan item (8 waves x 8 blocks of 16)
+constexpr uint32_t kSkProbeKeep = 8;  // candidates the sketch probe finds per (query, list) entry
 // RBC_SKETCH_AUTO uses the sketch when twice the variance outside the projection is at
 // most this fraction of the mean squared point-to-representative distance.
 constexpr double kSkAutoFrac = 0.05;
