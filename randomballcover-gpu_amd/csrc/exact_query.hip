@@ -353,21 +353,41 @@ __global__ __launch_bounds__(256) void probe_merge(const float *Q, size_t ldq, c
 // others; the per-query values are wave-uniform.
 constexpr uint32_t kPlanCQ = 64;  // plan_count: queries per block (fewer list_cnt atomics)
 
-template <int NPR>
+// Lane l of a block covers the V adjacent lists jb + V l .. + V - 1 (V = 4 when rows of
+// R floats keep 16-byte alignment), so that every distance and plane-denominator load
+// moves V values: the kernel is bound by the number of load instructions.
+template <int V>
+__device__ __forceinline__ void load_v(const float *p, float (&o)[V]) {
+  if constexpr (V == 4) {
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    o[0] = t.x, o[1] = t.y, o[2] = t.z, o[3] = t.w;
+  } else {
+    o[0] = p[0];
+  }
+}
+
+template <int NPR, int V>
 __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, uint32_t R,
                                                   const uint32_t *near,
                                                   const uint32_t *key1, const double *gam,
                                                   const uint32_t *cert, const float *radius,
-                                                  const uint16_t *rr_den, double e, const float *qerr, int metric,
+                                                  const float *rr_den, double e, const float *qerr, int metric,
                                                   const uint32_t *planeR, uint32_t P,
                                                   uint32_t Wq, uint32_t *bits, uint32_t *list_cnt) {
   static_assert(kPlanCQ == 64, "four waves of 16 queries make two 32-query words");
-  __shared__ uint32_t red[4][kWave];
+  __shared__ uint32_t red[4][kWave][V];
   const int l = lane_id(), w = threadIdx.x / kWave;
-  const uint32_t jb = blockIdx.y * kWave, j = jb + l;
-  const double psi = j < R ? (double)radius[j] : 0.0;
-  const double slack = 2.0 * e * psi * psi;
-  uint32_t mask = 0;
+  const uint32_t jl = blockIdx.y * (kWave * V) + V * l;  // first list of this lane
+  const bool live = jl < R;  // V divides R when V > 1
+  double psi[V], slack[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    psi[v] = live ? (double)radius[jl + v] : 0.0;
+    slack[v] = 2.0 * e * psi[v] * psi[v];
+  }
+  uint32_t mask[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) mask[v] = 0;
   for (uint32_t qi = 0; qi < 16; ++qi) {
     const uint32_t q = blockIdx.x * kPlanCQ + 16 * w + qi;
     if (q >= m) break;
@@ -375,88 +395,110 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
     const uint32_t j1 = key1 ? key1[q] : R;
     const uint32_t cb = cert[q];
     const double Eq = qerr ? (double)qerr[q] : 0.0;  // wave-uniform, loaded with the others
-    const double Eud = Eq * (1.0 + 0x1p-21);
-    float Eu = (float)Eud;
-    if ((double)Eu < Eud) Eu = nextafterf(Eu, INFINITY);
-    bool keep = false;
-    if (j < R) {
+    const double G = gam[q];
+    if (!live) continue;
+    float dq[V];
+    load_v<V>(Dqr + size_t(q) * R + jl, dq);
+    bool keep[V], tri = false;
+    double Dlo[V];
+    const double Gp = G * (1.0 + 2e-12);
+    const double Gd = Gp * (1.0 + 1e-13);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const uint32_t j = jl + v;
+      Dlo[v] = 0.0;
       if (j == j0) {
-        keep = (cb & 1u) == 0;  // probed; rescan only if the probe was not complete
+        keep[v] = (cb & 1u) == 0;  // probed; rescan only if the probe was not complete
       } else if (j == j1) {
-        keep = (cb & 2u) == 0;
+        keep[v] = (cb & 2u) == 0;
+      } else if (metric == 0) {
+        // list_lower_bound's test without square roots or divisions: the triangle bound
+        // sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as Dqj (1-e) <= (G' + psi)^2,
+        // and each plane bound num / den <= G' as num <= G' den with den rounded up; both
+        // keep at least what it keeps.  With split-BF16 rep distances, D~ is within qerr[q]
+        // of the true value (absolute), and Dlo and the upper values B_i below carry that
+        // term too.
+        const double rhs = Gp + psi[v];
+        Dlo[v] = (double)dq[v] * (1.0 - e) - Eq;
+        keep[v] = Dlo[v] <= rhs * rhs * (1.0 + 1e-13);
+        tri |= keep[v];
       } else {
-        const double Dqj = (double)Dqr[size_t(q) * R + j];
-        const double G = gam[q];
-        if (metric == 0) {
-          // list_lower_bound's test without square roots or divisions: the triangle
-          // bound sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as
-          // Dqj (1-e) <= (G' + psi)^2, and each plane bound num / den <= G' as
-          // num <= G' den with den rounded up; both keep at least what it keeps.
-          // With split-BF16 rep distances, D~ is within qerr[q] of the true value
-          // (absolute), and Dlo and the upper values B_i below carry that term too.
-          const double Gp = G * (1.0 + 2e-12);
-          const double Gd = Gp * (1.0 + 1e-13);
-          const double rhs = Gp + psi;
-          const double Dlo = Dqj * (1.0 - e) - Eq;
-          keep = Dlo <= rhs * rhs * (1.0 + 1e-13);
-          // The planes only matter for lists the triangle bound keeps; their rows of
-          // rr_den are not read for the others.
-          if (keep) {
-            // The plane between r_j and r_i: an x owned by r_j satisfies
-            // |x-r_j|^2 <= |x-r_i|^2 + 2e psi^2 for EVERY rep r_i (its owner is the FP32
-            // argmin over all reps), so the nearest rep and each probed rep give a lower
-            // bound, and the largest decides.
-            // All loads first (independent of each other), then the tests.
-            uint32_t ji[1 + NPR];
-            float Dqi[1 + NPR], den[1 + NPR];
-            ji[0] = j0;
-#pragma unroll
-            for (uint32_t pi = 0; pi < (uint32_t)NPR; ++pi) {
-              const uint32_t r = pi < P ? planeR[size_t(q) * P + pi] : R;
-              ji[1 + pi] = r < R && r != j0 ? r : R;
-            }
-#pragma unroll
-            for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) {
-              const bool ok = ji[pi] < R && ji[pi] != j;
-              Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
-              const float dn = ok ? (float)__builtin_bit_cast(_Float16, rr_den[size_t(ji[pi]) * R + j]) : 0.f;
-              den[pi] = dn > 0.f ? dn : INFINITY;  // no plane: never prunes
-            }
-            // The plane test num > G' den in FP32, with every rounding pushed towards
-            // keeping: A <= Dlo - slack (rounded down), and fl(Gf den + B_i) >= G' den +
-            // Dqi (1+e) + Eq, since Gf >= G' (1 + 8u) and B_i = fl(Dqi fl(1 + e + 8u) + Eu)
-            // >= (Dqi (1+e) + Eq)(1 + 6u) with Eu >= Eq (1 + 8u) (u = 2^-24).  A > that
-            // implies the exact test.
-            const double Ad = Dlo - slack;
-            float A = (float)Ad;
-            if ((double)A > Ad) A = nextafterf(A, -INFINITY);
-            const double Gu = Gd * (1.0 + 0x1p-21);
-            float Gf = (float)Gu;
-            if ((double)Gf < Gu) Gf = nextafterf(Gf, INFINITY);
-            const float cb1 = (float)(1.0 + e + 0x1p-21);
-            bool far = false;
-#pragma unroll
-            for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi)
-              far |= A > __builtin_fmaf(Gf, den[pi], __builtin_fmaf(Dqi[pi], cb1, Eu));
-            if (far) keep = false;
-          }
-        } else {
-          const double lb = list_lower_bound(Dqj, 0.0, 0.0, psi, e, true, metric);
-          keep = lb <= G;
-        }
+        const double lb = list_lower_bound((double)dq[v], 0.0, 0.0, psi[v], e, true, metric);
+        keep[v] = lb <= G;
       }
     }
-    mask |= uint32_t(keep) << qi;
+    // The planes only matter for lists the triangle bound keeps; their rr_den segments are
+    // not read when none of the lane's lists was kept by it.
+    if (tri) {
+      // The plane between r_j and r_i: an x owned by r_j satisfies |x-r_j|^2 <= |x-r_i|^2 +
+      // 2e psi^2 for EVERY rep r_i (its owner is the FP32 argmin over all reps), so the
+      // nearest rep and each plane rep give a lower bound, and the largest decides.  All
+      // loads first (independent of each other), then the tests.
+      uint32_t ji[1 + NPR];
+      float Dqi[1 + NPR], den[1 + NPR][V];
+      ji[0] = j0;
+#pragma unroll
+      for (uint32_t pi = 0; pi < (uint32_t)NPR; ++pi) {
+        const uint32_t r = pi < P ? planeR[size_t(q) * P + pi] : R;
+        ji[1 + pi] = r < R && r != j0 ? r : R;
+      }
+#pragma unroll
+      for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) {
+        const bool ok = ji[pi] < R;
+        Dqi[pi] = ok ? Dqr[size_t(q) * R + ji[pi]] : 0.f;
+        if (ok) {
+          load_v<V>(rr_den + size_t(ji[pi]) * R + jl, den[pi]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v) den[pi][v] = 0.f;
+        }
+      }
+      // The plane test num > G' den in FP32, with every rounding pushed towards keeping:
+      // A <= Dlo - slack (rounded down), and fl(Gf den + B_i) >= G' den + Dqi (1+e) + Eq,
+      // since Gf >= G' (1 + 8u) and B_i = fl(Dqi fl(1 + e + 8u) + Eu) >= (Dqi (1+e) + Eq)
+      // (1 + 6u) with Eu >= Eq (1 + 8u) (u = 2^-24).  A > that implies the exact test.
+      // A rep's plane against its own list (ji == j) is skipped (no bound).
+      const double Eud = Eq * (1.0 + 0x1p-21);
+      float Eu = (float)Eud;
+      if ((double)Eu < Eud) Eu = nextafterf(Eu, INFINITY);
+      const double Gu = Gd * (1.0 + 0x1p-21);
+      float Gf = (float)Gu;
+      if ((double)Gf < Gu) Gf = nextafterf(Gf, INFINITY);
+      const float cb1 = (float)(1.0 + e + 0x1p-21);
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        if (!keep[v] || jl + v == j0 || jl + v == j1) continue;
+        const double Ad = Dlo[v] - slack[v];
+        float A = (float)Ad;
+        if ((double)A > Ad) A = nextafterf(A, -INFINITY);
+        bool far = false;
+#pragma unroll
+        for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) {
+          const bool ok = ji[pi] < R && ji[pi] != jl + v;
+          const float dn = ok ? den[pi][v] : 0.f;
+          const float dd = dn > 0.f ? dn : INFINITY;  // no plane: never prunes
+          far |= A > __builtin_fmaf(Gf, dd, __builtin_fmaf(ok ? Dqi[pi] : 0.f, cb1, Eu));
+        }
+        if (far) keep[v] = false;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) mask[v] |= uint32_t(keep[v]) << qi;
   }
-  red[w][l] = mask;
+#pragma unroll
+  for (int v = 0; v < V; ++v) red[w][l][v] = mask[v];
   __syncthreads();
-  if (w == 0 && j < R) {
-    const uint32_t w0 = red[0][l] | (red[1][l] << 16), w1 = red[2][l] | (red[3][l] << 16);
+  if (w == 0 && live) {
     const uint32_t t0 = blockIdx.x * 2;
-    if (t0 < Wq) bits[size_t(j) * Wq + t0] = w0;
-    if (t0 + 1 < Wq) bits[size_t(j) * Wq + t0 + 1] = w1;
-    const uint32_t tot = __popc(w0) + __popc(w1);
-    if (tot) atomicAdd(&list_cnt[j], tot);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const uint32_t j = jl + v;
+      const uint32_t w0 = red[0][l][v] | (red[1][l][v] << 16), w1 = red[2][l][v] | (red[3][l][v] << 16);
+      if (t0 < Wq) bits[size_t(j) * Wq + t0] = w0;
+      if (t0 + 1 < Wq) bits[size_t(j) * Wq + t0 + 1] = w1;
+      const uint32_t tot = __popc(w0) + __popc(w1);
+      if (tot) atomicAdd(&list_cnt[j], tot);
+    }
   }
 }
 
@@ -890,16 +932,27 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   RBC_TRY(fill_u32(lcnt, 0, R + 1, st));
   RBC_TRY(fill_u32(cursor, 0, R + 1, st));
   {
-    const dim3 pg(cdiv(m, kPlanCQ), cdiv(R, kWave));
-#define RBC_PLAN(NPR)                                                                                          \
-  hipLaunchKernelGGL((plan_count<NPR>), pg, dim3(256), 0, st, Dqr, m, R, near, key1, gam, cert, h->radius.get(), \
-                     h->rr_den.get(), e, qerr, h->metric, planeR, P2, Wq, bits, lcnt)
-    if (P2 <= 4)
-      RBC_PLAN(4);
-    else if (P2 <= 8)
-      RBC_PLAN(8);
-    else
-      RBC_PLAN(12);
+    // Four lists per lane when rows of R floats stay 16-byte aligned.
+    const bool v4 = R % 4 == 0;
+    const dim3 pg(cdiv(m, kPlanCQ), cdiv(R, kWave * (v4 ? 4 : 1)));
+#define RBC_PLAN(NPR, V)                                                                                         \
+  hipLaunchKernelGGL((plan_count<NPR, V>), pg, dim3(256), 0, st, Dqr, m, R, near, key1, gam, cert,              \
+                     h->radius.get(), h->rr_den.get(), e, qerr, h->metric, planeR, P2, Wq, bits, lcnt)
+    if (v4) {
+      if (P2 <= 4)
+        RBC_PLAN(4, 4);
+      else if (P2 <= 8)
+        RBC_PLAN(8, 4);
+      else
+        RBC_PLAN(12, 4);
+    } else {
+      if (P2 <= 4)
+        RBC_PLAN(4, 1);
+      else if (P2 <= 8)
+        RBC_PLAN(8, 1);
+      else
+        RBC_PLAN(12, 1);
+    }
 #undef RBC_PLAN
   }
   RBC_HIP_TRY(hipGetLastError());
