@@ -283,12 +283,15 @@ __global__ __launch_bounds__(256) void cand_stats(const uint32_t *cnt, uint32_t 
 // Multi-list probe: a query whose proposal reached the mass threshold within its P lists
 // (count c > 0) probes the network's c best lists instead of its geometric set; the other
 // slots get key R (no list, dropped by the counting sort).
+// A query probes at least minc of the network's lists, so that its probe entries can hold
+// its k candidates.
 __global__ void proposal_lists(const uint32_t *plists, const uint32_t *pcnt, uint32_t m, uint32_t P, uint32_t R,
-                               uint32_t *keys) {
+                               uint32_t minc, uint32_t *keys) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= m) return;
-  const uint32_t c = pcnt[q];
+  uint32_t c = pcnt[q];
   if (c == 0) return;
+  c = min(max(c, minc), P);
   for (uint32_t s = 0; s < P; ++s) {
     const uint32_t j = plists[size_t(q) * P + s];
     keys[size_t(q) * P + s] = s < c && j < R ? j : R;
@@ -843,9 +846,14 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   uint32_t *probe_i = nullptr, *probe_i1 = nullptr, *key1 = nullptr;
   if (bprobe) {
     // Entries e = q * Pp + s (query q, probe slot s), keyed by list, sorted list-major.
+    // Each entry holds at most Ke candidates (the sketch probe 8, the full-row probe 16);
+    // the merge reads that many.  A query taking the network's lists probes enough of them
+    // to hold its k (ceil(k / Ke)).
+    const uint32_t Ke = std::min(KpUse, sketch ? 8u : kBf16ProbeKeep);
     if (plists) {
       RBC_HIP_TRY(hipStreamWaitEvent(st, h->ev_out, 0));
-      hipLaunchKernelGGL(proposal_lists, dim3(cdiv(m, 256)), dim3(256), 0, st, plists, pcnt, m, Pp, R, nearP);
+      hipLaunchKernelGGL(proposal_lists, dim3(cdiv(m, 256)), dim3(256), 0, st, plists, pcnt, m, Pp, R,
+                         cdiv(k, Ke), nearP);
       RBC_HIP_TRY(hipGetLastError());
     }
     const size_t ne = size_t(m) * Pp;
@@ -857,9 +865,6 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
     RBC_TRY(make_bucket_items(off, h->list_off.get(), h->list_len.get(), R, pitems, &npitems, ws,
                               kSlotProbeItems, st,
                               kBf16ProbeQ));
-    // Each entry holds at most kBf16ProbeKeep candidates (the sketch probe kSkProbeKeep);
-    // the merge reads that many.
-    const uint32_t Ke = std::min(KpUse, sketch ? kSkProbeKeep : kBf16ProbeKeep);
     RBC_TRY(ws.get(kQ + 32, ne * Ke * 4, &p));
     float *ev = static_cast<float *>(p);
     RBC_TRY(ws.get(kQ + 33, ne * Ke * 4, &p));

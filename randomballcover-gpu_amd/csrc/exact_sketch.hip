@@ -473,13 +473,12 @@ __global__ __launch_bounds__(64 * NW) void sk_scan_kernel(SkScanArgs a) {
 // per-query constant).  acc <= 0 for every point but a coincident one, and the float bits
 // of non-positive values order by magnitude as unsigned integers, so a point's key is
 // its accumulator's bits with the low bits replaced by the point's slot (point block,
-// row) -- one v_and_or per pair -- and each lane keeps its kSkPL smallest keys with the
+// row) -- one v_and_or per pair -- and each lane keeps its PL smallest keys with the
 // branch-free median insertion.  The list's sketch rows come through the scan's ring
 // (direct global->LDS loads, 128 points per step).  After the last step one thread per
 // entry merges the four lanes that saw disjoint quarters of the list into the entry's
-// ascending list of up to 4 kSkPL (id, D~) pairs.  Only a bound is taken from them: the
+// ascending list of up to 4 PL (id, D~) pairs.  Only a bound is taken from them: the
 // caller re-ranks its best candidates exactly.
-constexpr int kSkPL = 2;                     // keys kept per lane
 constexpr uint32_t kSkNoKey = 0xFF800000u;   // keys at or above this hold no point
 
 __device__ __forceinline__ uint32_t sk_med3(uint32_t a, uint32_t b, uint32_t c) {
@@ -490,13 +489,15 @@ __device__ __forceinline__ uint32_t sk_med3(uint32_t a, uint32_t b, uint32_t c) 
 
 // Inserts key into the ascending list t (the largest drops out): slot j takes the median
 // of (t[j-1], key, t[j]).
-__device__ __forceinline__ void sk_insert(uint32_t (&t)[kSkPL], uint32_t key) {
+template <int PL>
+__device__ __forceinline__ void sk_insert(uint32_t (&t)[PL], uint32_t key) {
 #pragma unroll
-  for (int j = kSkPL - 1; j > 0; --j) t[j] = sk_med3(t[j - 1], key, t[j]);
+  for (int j = PL - 1; j > 0; --j) t[j] = sk_med3(t[j - 1], key, t[j]);
   t[0] = min(t[0], key);
 }
 
-template <int PB, int NST>
+// PL: keys kept per lane (entries hold 4 PL candidates).
+template <int PB, int NST, int PL>
 __global__ __launch_bounds__(256) void sk_probe_kernel(SkProbeArgs a) {
   constexpr int VP = 16 * PB, StageB = VP * 64, LW = PB / 4;
   static_assert(PB % 4 == 0 && LW * (NST - 2) <= 6, "ring shape");
@@ -545,9 +546,9 @@ __global__ __launch_bounds__(256) void sk_probe_kernel(SkProbeArgs a) {
   }
   const bf16x8 B = __builtin_bit_cast(bf16x8, bv);
   const uint32_t vmask = ~((1u << a.idx_bits) - 1u);
-  uint32_t tk[kSkPL];
+  uint32_t tk[PL];
 #pragma unroll
-  for (int j = 0; j < kSkPL; ++j) tk[j] = 0xFFFFFFFFu;
+  for (int j = 0; j < PL; ++j) tk[j] = 0xFFFFFFFFu;
   sk_wait_vm<0>();
   for (uint32_t s = 0; s < nsteps; ++s) {
     sk_wait_upto(LW * min((uint32_t)(NST - 2), nsteps - 1 - s));
@@ -579,11 +580,11 @@ __global__ __launch_bounds__(256) void sk_probe_kernel(SkProbeArgs a) {
   }
   // Hand the per-lane lists to one thread per entry through LDS (the ring is free now).
   __syncthreads();
-  uint32_t *lk = reinterpret_cast<uint32_t *>(ring);  // [kE][4][kSkPL]
+  uint32_t *lk = reinterpret_cast<uint32_t *>(ring);  // [kE][4][PL]
   if (wave_live) {
     const int e = w * 16 + (l % 16), g = l / 16;
 #pragma unroll
-    for (int j = 0; j < kSkPL; ++j) lk[(e * 4 + g) * kSkPL + j] = tk[j];
+    for (int j = 0; j < PL; ++j) lk[(e * 4 + g) * PL + j] = tk[j];
   }
   __syncthreads();
   if (t >= kE || qent[t] == kDummy) return;
@@ -594,8 +595,8 @@ __global__ __launch_bounds__(256) void sk_probe_kernel(SkProbeArgs a) {
     int best = -1;
     uint32_t bk = 0xFFFFFFFFu;
     for (int g = 0; g < 4; ++g) {
-      if (head[g] >= (uint32_t)kSkPL) continue;
-      const uint32_t key = lk[(t * 4 + g) * kSkPL + head[g]];
+      if (head[g] >= (uint32_t)PL) continue;
+      const uint32_t key = lk[(t * 4 + g) * PL + head[g]];
       if (key < kSkNoKey && key < bk) {
         best = g;
         bk = key;
@@ -804,8 +805,7 @@ constexpr SkScan kSkKernel = sk_scan_kernel<kSkQB, kSkCS, true, kSkPBStep, kSkSt
 int sketch_probe(rbc_exact *h, const uint16_t *qs, const float *qn, const uint32_t *entries, uint32_t P,
                  const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v, hipStream_t st) {
   if (nitems == 0) return RBC_OK;
-  static_assert(4 * kSkPL == (int)kSkProbeKeep, "sketch probe entries hold 4 lanes x kSkPL keys");
-  RBC_ARG_CHECK(Kp <= kSkProbeKeep, "sketch probe keeps at most %u candidates per entry (K'=%u)", kSkProbeKeep, Kp);
+  RBC_ARG_CHECK(Kp >= 1 && Kp <= 16, "sketch probe entries hold at most 16 candidates (K'=%u)", Kp);
   SkProbeArgs a;
   a.items = items;
   a.nitems = nitems;
@@ -830,7 +830,11 @@ int sketch_probe(rbc_exact *h, const uint16_t *qs, const float *qn, const uint32
   a.idx_bits = bits;
   a.out_i = out_i;
   a.out_v = out_v;
-  hipLaunchKernelGGL((sk_probe_kernel<8, 3>), dim3(kXCD * ((nitems + kXCD - 1) / kXCD)), dim3(256), 0, st, a);
+  const dim3 grid(kXCD * ((nitems + kXCD - 1) / kXCD));
+  if (Kp <= 8)
+    hipLaunchKernelGGL((sk_probe_kernel<8, 3, 2>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((sk_probe_kernel<8, 3, 4>), grid, dim3(256), 0, st, a);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }

@@ -13,7 +13,6 @@ constexpr int kSkDims = 28;    // projection rows (principal directions)
 constexpr int kSkW = 32;       // BF16 coordinates per sketch row (28 + 4 carrying norm and threshold)
 constexpr int kSkMaxD = 128;   // largest d with a sketch (the BF16 filter's limit)
 constexpr uint32_t kSkScanQ = 1024;  // queries per scan item (8 waves x 8 blocks of 16)
-constexpr uint32_t kSkProbeKeep = 8;  // candidates the sketch probe finds per (query, list) entry
 // RBC_SKETCH_AUTO uses the sketch when twice the variance outside the projection is at
 // most this fraction of the mean squared point-to-representative distance.
 constexpr double kSkAutoFrac = 0.05;
@@ -112,7 +111,8 @@ int sketch_queries(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, u
 // kSkScanQ; sched holds 2 kXCD + 1 uint32 of device scratch.  The emission threshold of a
 // query is computed once per call (with the index-wide largest point error and norm).
 // The probe on sketches: for every entry (query, list) of items (tile kBf16ProbeQ), its
-// Kp (<= 16) closest points by sketch distance, ascending, as bf16_probe returns them.
+// Kp (<= 16) closest points by sketch distance, ascending, as bf16_probe returns them (the
+// entries of Kp <= 8 come from 2 keys per lane, of more from 4).
 int sketch_probe(rbc_exact *h, const uint16_t *qs, const float *qn, const uint32_t *entries, uint32_t P,
                  const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v, hipStream_t st);
 // qthr is device scratch of m uint32 (one per query of the call).

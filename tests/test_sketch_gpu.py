@@ -131,3 +131,31 @@ def test_sketch_survives_save_and_load(tmp_path):
     ids2, ds2 = ld.kquery(q, 10, precision=rbc.PREC_BF16)
     np.testing.assert_array_equal(ids, ids2)
     np.testing.assert_array_equal(ds, ds2)
+
+
+def test_sketch_probe_under_a_confident_proposal():
+    # A proposal that often gives a query a single list (low mass threshold): that list's
+    # probe entry alone must still hold k candidates, or the query falls back to the loose
+    # representative bound and floods the re-rank.  Results stay identical either way.
+    n, d, R, k = 1 << 17, 128, 512, 10
+    x = datagen.lowrank(n, d, 0)
+    idx = rbc.ExactRBC(x, R, seed=0)
+    assert idx.sketch_info()["active"] == 1
+    owner = idx.lists()["owner"]
+    qt = datagen.lowrank(1 << 15, d, 2)
+    nn, _ = idx.kquery(qt, k, precision=rbc.PREC_BF16)
+    net = rbc.ProposalMLP(d, R, seed=0)
+    net.train(qt, owner[nn], epochs=4, batch=1024, lr=1e-3)
+    q = datagen.lowrank(4096, d, 1)
+    i0, d0, s0 = idx.kquery(q, k, precision=rbc.PREC_BF16, stats=True)
+    idx.set_proposal(net, mass=0.3)
+    i1, d1, s1 = idx.kquery(q, k, precision=rbc.PREC_BF16, stats=True)
+    idx.set_proposal(None)
+    _, counts = net.propose_lists(q, 8, 0.3)
+    assert (counts == 1).mean() > 0.1, "the proposal should give many queries one list"
+    np.testing.assert_array_equal(i0, i1)
+    np.testing.assert_array_equal(d0, d1)
+    # A single list's bound is looser than eight lists' (about twice the candidates here);
+    # the representative bound would give orders of magnitude more.
+    assert s1["pairs_rerank"] < 4.0 * s0["pairs_rerank"]
+    assert s1["pairs_scan"] < 2.0 * s0["pairs_scan"]
