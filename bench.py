@@ -500,7 +500,7 @@ def main():
     if prec == rbc.PREC_BF16:
         roof = {"bound": "mfma", "achieved": round(tflops, 2), "peak": MFMA_BF16_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-                "traffic": None, "kernel": "sk_scan_kernel<16>" if st["sketch"] else "bf16_scan_kernel<4>",
+                "traffic": None, "kernel": "sk_scan_kernel" if st["sketch"] else "bf16_scan_kernel<4>",
                 "flops_per_pair": st["flops_scan"],
                 "flops_per_launch": flops, "pairs_per_launch": st["pairs_scan"],
                 "alg_bytes_per_launch": st["bytes_scan"],
