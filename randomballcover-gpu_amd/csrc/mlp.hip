@@ -108,19 +108,32 @@ __device__ __forceinline__ void gemm_epilogue(const GemmArgs &g, const f32x4 (&a
       }
 }
 
-// KC: both operands FP16 and contiguous along k (a_cs == b_rs == 1, 16-byte aligned rows;
-// the forward pass).  64-wide k chunks: every thread stages two 16-byte pieces of one row
-// of A and of B per chunk, and issues the next chunk's loads before the current chunk's
-// MFMAs so that their latency overlaps the math.  Row pitch 72 halves: the 16 rows of a
-// read pass fall on distinct banks.
+// KC: operands contiguous along k (a_cs == b_rs == 1, 16-byte aligned rows; the forward
+// pass), B FP16, A FP16 or FP32 (the queries of the first layer, rounded to FP16 as they
+// are staged, as the general kernel rounds them).  64-wide k chunks: every thread stages
+// two 8-element pieces of one row of A and of B per chunk, and issues the next chunk's
+// loads before the current chunk's MFMAs so that their latency overlaps the math.  Row
+// pitch 72 halves: the 16 rows of a read pass fall on distinct banks.
 constexpr int kKC = 64, kKCPad = kKC + 8;
 
+template <typename TA>
+__device__ __forceinline__ f16x8 load8_h(const TA *p) {
+  if constexpr (sizeof(TA) == 2) {
+    return *reinterpret_cast<const f16x8 *>(p);
+  } else {
+    const float4 u = reinterpret_cast<const float4 *>(p)[0], v = reinterpret_cast<const float4 *>(p)[1];
+    return f16x8{(_Float16)u.x, (_Float16)u.y, (_Float16)u.z, (_Float16)u.w,
+                 (_Float16)v.x, (_Float16)v.y, (_Float16)v.z, (_Float16)v.w};
+  }
+}
+
+template <typename TA>
 __global__ __launch_bounds__(256) void gemm_f16_kc(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) _Float16 As[kGT][kKCPad];
   __shared__ __attribute__((aligned(16))) _Float16 Bs[kGT][kKCPad];  // Bs[j][k]
   const int t = threadIdx.x, w = t / 64, l = t % 64;
   const int i0 = blockIdx.y * kGT, j0 = blockIdx.x * kGT;
-  const _Float16 *A = static_cast<const _Float16 *>(g.A);
+  const TA *A = static_cast<const TA *>(g.A);
   const _Float16 *B = static_cast<const _Float16 *>(g.B);
   const int lr = t / 4, lk = (t % 4) * 8;  // row; k offsets lk and lk + 32 of the chunk
   const int i = i0 + lr, j = j0 + lr;
@@ -129,7 +142,7 @@ __global__ __launch_bounds__(256) void gemm_f16_kc(GemmArgs g) {
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const int k = k0 + lk + 32 * e;
-      ra[e] = (i < g.M && k < g.K) ? *reinterpret_cast<const f16x8 *>(A + int64_t(i) * g.a_rs + k) : f16x8{};
+      ra[e] = (i < g.M && k < g.K) ? load8_h(A + int64_t(i) * g.a_rs + k) : f16x8{};
       rb[e] = (j < g.N && k < g.K) ? *reinterpret_cast<const f16x8 *>(B + int64_t(j) * g.b_cs + k) : f16x8{};
     }
   };
@@ -331,47 +344,65 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
         for (int bi = 0; bi < 2; ++bi)
           acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wv, af[bi][ks], acc[bi][bj], 0, 0, 0);
       }
-    // Lane l holds row rb + bi*16 + l%16, columns j0 + cb + bj*16 + 4(l/16) + r.
+    // Lane l holds row rb + bi*16 + l%16, columns j0 + cb + bj*16 + 4(l/16) + r: value
+    // e = 4 bj + r of a query block is column j0 + cb + 16 (e/4) + 4(l/16) + e%4, so a
+    // lane's values come in ascending column order.
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-      const int jc = cb + bj * 16 + (l / 16) * 4;
+    for (int bi = 0; bi < 2; ++bi) {
+      float v[8];
+      float m8 = -INFINITY;
 #pragma unroll
-      for (int bi = 0; bi < 2; ++bi) {
-        float v[4];
-        float m4 = -INFINITY;
+      for (int e = 0; e < 8; ++e) {
+        const int jc = cb + (e / 4) * 16 + (l / 16) * 4 + e % 4;
+        v[e] = j0 + jc < g.N ? alpha * acc[bi][e / 4][e % 4] + bs[jc] : -INFINITY;
+        if (v[e] != v[e]) v[e] = -INFINITY;
+        m8 = fmaxf(m8, v[e]);
+      }
+      if (m8 > mx[bi]) {
+        se[bi] = mx[bi] == -INFINITY ? 0.f : se[bi] * __expf(mx[bi] - m8);
+        mx[bi] = m8;
+      }
+      if (mx[bi] != -INFINITY) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          v[r] = j0 + jc + r < g.N ? alpha * acc[bi][bj][r] + bs[jc + r] : -INFINITY;
-          if (v[r] != v[r]) v[r] = -INFINITY;
-          m4 = fmaxf(m4, v[r]);
+        for (int e = 0; e < 8; ++e) se[bi] += __expf(v[e] - mx[bi]);  // exp(-inf) = 0
+      }
+      // The 4 lanes of a row (l, l^16, l^32, l^48) each hold P values >= their own P-th,
+      // so a value below another lane's P-th is not among the row's P best; nor is one
+      // below the smallest of the 4 lanes' (P/4)-th values (those lanes hold P distinct
+      // columns at or above it).  A value equal to such a bound may be (a lower column
+      // wins a tie); one equal to the lane's own P-th is not (the lane's entries have
+      // lower columns).  Only values that pass these tests are inserted, one per lane per
+      // pass, so a wave runs as many insertion passes as its busiest lane has hits rather
+      // than one per value.
+      const float own = tv[bi][P - 1], q4 = tv[bi][P / 4 - 1];
+      const float oth = fmaxf(
+          fmaxf(__shfl_xor(own, 16, 64), fmaxf(__shfl_xor(own, 32, 64), __shfl_xor(own, 48, 64))),
+          fminf(fminf(q4, __shfl_xor(q4, 16, 64)), fminf(__shfl_xor(q4, 32, 64), __shfl_xor(q4, 48, 64))));
+      uint32_t pend = 0;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) pend |= (v[e] > own && v[e] >= oth) ? (1u << e) : 0u;
+      while (pend) {
+        const int e = __builtin_ctz(pend);
+        pend &= pend - 1;
+        float x = v[0];
+#pragma unroll
+        for (int s = 1; s < 8; ++s) x = e == s ? v[s] : x;
+        if (!(x > tv[bi][P - 1])) continue;
+        const int j = j0 + cb + (e / 4) * 16 + (l / 16) * 4 + e % 4;
+        // Branch-free insertion from the bottom: slot s takes slot s-1's entry when x
+        // belongs above it, x itself when it belongs exactly there, else keeps its own.
+#pragma unroll
+        for (int s = P - 1; s > 0; --s) {
+          const bool above = x > tv[bi][s - 1];
+          const bool here = !above && x > tv[bi][s];
+          const float nv = above ? tv[bi][s - 1] : (here ? x : tv[bi][s]);
+          const int ni = above ? ti[bi][s - 1] : (here ? j : ti[bi][s]);
+          tv[bi][s] = nv;
+          ti[bi][s] = ni;
         }
-        if (m4 > mx[bi]) {
-          se[bi] = mx[bi] == -INFINITY ? 0.f : se[bi] * __expf(mx[bi] - m4);
-          mx[bi] = m4;
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (v[r] == -INFINITY) continue;
-          se[bi] += __expf(v[r] - mx[bi]);
-          // Branch-free insertion from the bottom: slot s takes slot s-1's entry when v
-          // belongs above it, v itself when it belongs exactly there, else keeps its own.
-          // A lane sees its columns in ascending order, so an equal value goes after.
-          if (v[r] > tv[bi][P - 1]) {
-            const int j = j0 + jc + r;
-#pragma unroll
-            for (int s = P - 1; s > 0; --s) {
-              const bool above = v[r] > tv[bi][s - 1];
-              const bool here = !above && v[r] > tv[bi][s];
-              const float nv = above ? tv[bi][s - 1] : (here ? v[r] : tv[bi][s]);
-              const int ni = above ? ti[bi][s - 1] : (here ? j : ti[bi][s]);
-              tv[bi][s] = nv;
-              ti[bi][s] = ni;
-            }
-            if (v[r] > tv[bi][0]) {
-              tv[bi][0] = v[r];
-              ti[bi][0] = j;
-            }
-          }
+        if (x > tv[bi][0]) {
+          tv[bi][0] = x;
+          ti[bi][0] = j;
         }
       }
     }
@@ -471,10 +502,10 @@ __global__ __launch_bounds__(256) void gemm_f16(GemmArgs g) {
 template <typename TA, typename TB>
 int gemm(const GemmArgs &g, hipStream_t st) {
   dim3 grid(cdiv(g.N, kGT), cdiv(g.M, kGT));
-  const bool kc = sizeof(TA) == 2 && sizeof(TB) == 2 && g.a_cs == 1 && g.b_rs == 1 && g.K % 8 == 0 &&
-                  g.a_rs % 8 == 0 && g.b_cs % 8 == 0 && reinterpret_cast<uintptr_t>(g.A) % 16 == 0 &&
-                  reinterpret_cast<uintptr_t>(g.B) % 16 == 0;
-  if (kc && g.amax && g.K % 32 == 0 && g.K <= 256) {
+  const bool kc = sizeof(TB) == 2 && g.a_cs == 1 && g.b_rs == 1 && g.K % 8 == 0 &&
+                  (g.a_rs * (int64_t)sizeof(TA)) % 16 == 0 && g.b_cs % 8 == 0 &&
+                  reinterpret_cast<uintptr_t>(g.A) % 16 == 0 && reinterpret_cast<uintptr_t>(g.B) % 16 == 0;
+  if (kc && sizeof(TA) == 2 && g.amax && g.K % 32 == 0 && g.K <= 256) {
     const dim3 rows(cdiv(g.M, kGT));
     switch (g.K / 32) {
       case 1: hipLaunchKernelGGL(gemm_f16_argmax<1>, rows, dim3(256), 0, st, g); break;
@@ -487,7 +518,7 @@ int gemm(const GemmArgs &g, hipStream_t st) {
       default: hipLaunchKernelGGL(gemm_f16_argmax<8>, rows, dim3(256), 0, st, g); break;
     }
   } else if (kc)
-    hipLaunchKernelGGL(gemm_f16_kc, grid, dim3(256), 0, st, g);
+    hipLaunchKernelGGL(gemm_f16_kc<TA>, grid, dim3(256), 0, st, g);
   else
     hipLaunchKernelGGL((gemm_f16<TA, TB>), grid, dim3(256), 0, st, g);
   RBC_HIP_TRY(hipGetLastError());
