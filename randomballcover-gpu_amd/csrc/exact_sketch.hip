@@ -133,12 +133,17 @@ __global__ void sk_owner_d2(const float *xp, size_t ld, uint32_t n, uint32_t S, 
 }
 
 // ---- projection of rows (database at build, queries per call) --------------------------
-// Eight rows per block, 32 threads per row; thread j < kSkDims computes y_j = sum_i
-// P[j][i] (x_i - mu_i) with one FMA per coordinate in coordinate order (FP32, after an
-// FP32 subtraction), and the per-coordinate bound gamma_{d+1} sum_i |P_ji| |x_i - mu_i|
-// in FP64.  Outputs: the BF16 row (database: -n_hi, -n_lo, 1, 1 in the last four
-// coordinates; queries: 1, 1, 0, 0), e = |y^ - y~| + |bound vector| rounded up, and
-// |y~|^2 (database: rounded up; queries: rounded to nearest).
+// 32 rows per block, in 4 rounds of 8 rows with 32 threads per row (P is staged once per
+// block); thread j < kSkDims computes y_j = sum_i P[j][i] (x_i - mu_i) with one FMA per
+// coordinate in coordinate order (FP32, after an FP32 subtraction), and the bound
+// gamma_{d+1} a_j, a_j = sum_i |P_ji| |x_i - mu_i|: a_j is summed in FP32 from |x_i - mu_i|
+// rounded up, and a sum of d non-negative terms with one FMA each is within gamma_d of its
+// exact value, so (1 + 2.02 d 2^-24) times it bounds a_j from above.  Outputs: the BF16 row
+// (database: -n_hi, -n_lo, 1, 1 in the last four coordinates; queries: 1, 1, 0, 0), e =
+// |y^ - y~| + |bound vector| rounded up, and |y~|^2 (database: rounded up; queries: rounded
+// to nearest).
+constexpr uint32_t kSkProjRows = 32;
+
 __global__ __launch_bounds__(256) void sk_project(const float *x, size_t ldx, uint32_t rows, uint32_t d,
                                                   const float *P, uint32_t ldP, const float *mu, double gam_d1,
                                                   int database, uint16_t *out, float *err, float *n2) {
@@ -150,52 +155,56 @@ __global__ __launch_bounds__(256) void sk_project(const float *x, size_t ldx, ui
     const uint32_t i = e / kSkW, jj = e % kSkW;
     Pt[i][jj] = jj < (uint32_t)kSkDims ? P[size_t(jj) * ldP + i] : 0.f;
   }
-  const uint32_t row = blockIdx.x * 8 + r;
-  for (uint32_t i = j; i < d; i += kSkW) {
-    const float v = row < rows ? x[size_t(row) * ldx + i] : 0.f;
-    xs[r][i] = v - mu[i];
-    // |x_i - mu_i| rounded up, for the bound (the FP32 difference may round down).
-    xa[r][i] = f32_up(fabs((double)v - (double)mu[i]));
-  }
-  __syncthreads();
-  float y = 0.f;
-  double a = 0.0;
-  for (uint32_t i = 0; i < d; ++i) {
-    y = __builtin_fmaf(Pt[i][j], xs[r][i], y);
-    a += fabs((double)Pt[i][j]) * (double)xa[r][i];
-  }
-  double q2 = 0.0, b2 = 0.0, y2 = 0.0;
-  uint16_t h = 0;
-  if (j < kSkDims) {
-    h = bf16_rne(y);
-    const double hv = (double)bf16_val(h), dq = (double)y - hv, bj = gam_d1 * a;
-    q2 = dq * dq;
-    b2 = bj * bj;
-    y2 = hv * hv;
-  }
-  // Sums over the row's 32 threads (lanes 32 (r % 2) .. +31 of the wave).
-  for (int o = kSkW / 2; o > 0; o >>= 1) {
-    q2 += __shfl_xor(q2, o, kSkW);
-    b2 += __shfl_xor(b2, o, kSkW);
-    y2 += __shfl_xor(y2, o, kSkW);
-  }
-  if (row >= rows) return;
-  if (j >= kSkDims) {
-    if (database) {
-      // -n_hi, -n_lo with n_hi + n_lo <= |y~|^2 / 2.
-      const double N = 0.5 * y2;
-      const uint16_t nh = bf16_down_pos(f32_down(N));
-      const double rem = N - (double)bf16_val(nh);
-      const uint16_t nl = rem > 0.0 ? bf16_down_pos(f32_down(rem)) : (uint16_t)0;
-      h = j == kSkDims ? (uint16_t)(nh | 0x8000u) : j == kSkDims + 1 ? (uint16_t)(nl | 0x8000u) : kBf16One;
-    } else {
-      h = j < kSkDims + 2 ? kBf16One : (uint16_t)0;
+  const double afac = 1.0 + 2.02 * (double)d * 0x1p-24;
+  for (uint32_t round = 0; round < kSkProjRows / 8; ++round) {
+    const uint32_t row = blockIdx.x * kSkProjRows + round * 8 + r;
+    __syncthreads();  // P staged; the previous round's rows read
+    for (uint32_t i = j; i < d; i += kSkW) {
+      const float v = row < rows ? x[size_t(row) * ldx + i] : 0.f;
+      xs[r][i] = v - mu[i];
+      // |x_i - mu_i| rounded up, for the bound (the FP32 difference may round down).
+      xa[r][i] = f32_up(fabs((double)v - (double)mu[i]));
     }
-  }
-  out[size_t(row) * kSkW + j] = h;
-  if (j == 0) {
-    err[row] = f32_up((sqrt(q2) + sqrt(b2)) * (1.0 + 1e-12));
-    n2[row] = database ? f32_up(y2) : (float)y2;
+    __syncthreads();
+    float y = 0.f, af = 0.f;
+    for (uint32_t i = 0; i < d; ++i) {
+      const float p = Pt[i][j];
+      y = __builtin_fmaf(p, xs[r][i], y);
+      af = __builtin_fmaf(fabsf(p), xa[r][i], af);
+    }
+    double q2 = 0.0, b2 = 0.0, y2 = 0.0;
+    uint16_t h = 0;
+    if (j < kSkDims) {
+      h = bf16_rne(y);
+      const double hv = (double)bf16_val(h), dq = (double)y - hv, bj = gam_d1 * ((double)af * afac);
+      q2 = dq * dq;
+      b2 = bj * bj;
+      y2 = hv * hv;
+    }
+    // Sums over the row's 32 threads (lanes 32 (r % 2) .. +31 of the wave).
+    for (int o = kSkW / 2; o > 0; o >>= 1) {
+      q2 += __shfl_xor(q2, o, kSkW);
+      b2 += __shfl_xor(b2, o, kSkW);
+      y2 += __shfl_xor(y2, o, kSkW);
+    }
+    if (row >= rows) continue;
+    if (j >= kSkDims) {
+      if (database) {
+        // -n_hi, -n_lo with n_hi + n_lo <= |y~|^2 / 2.
+        const double N = 0.5 * y2;
+        const uint16_t nh = bf16_down_pos(f32_down(N));
+        const double rem = N - (double)bf16_val(nh);
+        const uint16_t nl = rem > 0.0 ? bf16_down_pos(f32_down(rem)) : (uint16_t)0;
+        h = j == kSkDims ? (uint16_t)(nh | 0x8000u) : j == kSkDims + 1 ? (uint16_t)(nl | 0x8000u) : kBf16One;
+      } else {
+        h = j < kSkDims + 2 ? kBf16One : (uint16_t)0;
+      }
+    }
+    out[size_t(row) * kSkW + j] = h;
+    if (j == 0) {
+      err[row] = f32_up((sqrt(q2) + sqrt(b2)) * (1.0 + 1e-12));
+      n2[row] = database ? f32_up(y2) : (float)y2;
+    }
   }
 }
 
@@ -745,7 +754,7 @@ int sketch_build(rbc_exact *h, hipStream_t st) {
   DevBuf<float> err, n2;
   RBC_TRY(err.alloc(n));
   RBC_TRY(n2.alloc(n));
-  hipLaunchKernelGGL(sk_project, dim3(cdiv(n, 8)), dim3(256), 0, st, h->xp.get(), (size_t)ld, n, d, h->sk_P.get(), ld,
+  hipLaunchKernelGGL(sk_project, dim3(cdiv(n, kSkProjRows)), dim3(256), 0, st, h->xp.get(), (size_t)ld, n, d, h->sk_P.get(), ld,
                      h->sk_mu.get(), sketch_proj_err(d), 1, h->xs.get(), err.get(), n2.get());
   RBC_TRY(h->sk_lqe.alloc(R));
   RBC_TRY(h->sk_lnm.alloc(R));
@@ -778,7 +787,7 @@ int sketch_queries(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, u
                    hipStream_t st) {
   RBC_ARG_CHECK(h->sk_dims > 0, "the index has no sketch (L2, 32 < d <= %d only)", kSkMaxD);
   if (m == 0) return RBC_OK;
-  hipLaunchKernelGGL(sk_project, dim3(cdiv(m, 8)), dim3(256), 0, st, q, ldq, m, h->d, h->sk_P.get(), h->ld,
+  hipLaunchKernelGGL(sk_project, dim3(cdiv(m, kSkProjRows)), dim3(256), 0, st, q, ldq, m, h->d, h->sk_P.get(), h->ld,
                      h->sk_mu.get(), sketch_proj_err(h->d), 0, qs, qe, qn);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;

@@ -107,14 +107,14 @@ double sketch_gemm_err();
 // qe and |y~_q|^2 (qn).
 int sketch_queries(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint16_t *qs, float *qe, float *qn,
                    hipStream_t st);
-// The list scan on sketches: items from make_bucket_items over list_qoff with tile
-// kSkScanQ; sched holds 2 kXCD + 1 uint32 of device scratch.  The emission threshold of a
-// query is computed once per call (with the index-wide largest point error and norm).
 // The probe on sketches: for every entry (query, list) of items (tile kBf16ProbeQ), its
 // Kp (<= 16) closest points by sketch distance, ascending, as bf16_probe returns them (the
 // entries of Kp <= 8 come from 2 keys per lane, of more from 4).
 int sketch_probe(rbc_exact *h, const uint16_t *qs, const float *qn, const uint32_t *entries, uint32_t P,
                  const ScanItem *items, uint32_t nitems, uint32_t Kp, uint32_t *out_i, float *out_v, hipStream_t st);
+// The list scan on sketches: items from make_bucket_items over list_qoff with tile
+// kSkScanQ; sched holds 2 kXCD + 1 uint32 of device scratch.  The emission threshold of a
+// query is computed once per call (with the index-wide largest point error and norm);
 // qthr is device scratch of m uint32 (one per query of the call).
 int sketch_scan(rbc_exact *h, uint32_t m, const uint16_t *qs, const float *qe, const float *qn, const double *gam,
                 uint32_t *qthr, const uint32_t *qlist, const uint32_t *list_qoff, const ScanItem *items,
