@@ -8,7 +8,8 @@ learned proposal leaving results unchanged.
 * C1: n 65,536, d 16, U[0,1) (seed 0), m 8,192 (seed 1), 256 reps, k 1 -- one-shot
   query against the oracle's restatement with the same reps, exact query against FP64.
 * C2: n 2^20, d 64, 64-component mixture, m 65,536, 1,024 reps, k 32, L2 and L1.
-* C3: n 2^22, d 128, low-rank model, m 131,072, 2,048 reps, k 10.
+* C3: n 2^22, d 128, low-rank model, m 131,072, 2,048 reps, k 10; every query is also
+  checked against the FP32 brute force (another kernel path) through FP64 distances.
 * C4: 256 databases (n log-uniform in [2^10, 2^18], d in {32, 96}, U[0,1)), query
   batches log-uniform in [1, 4096], ceil(sqrt(n)) reps, k 16, one batched launch.
 * C5: n 2^22, d 96, 1,024-component Dirichlet mixture, m 262,144 held-out queries,
@@ -92,6 +93,21 @@ def test_c3_exact_k10_bf16_and_fp32():
     np.testing.assert_array_equal(i32, ids)
     np.testing.assert_array_equal(d32, ds)
     check_sample(x, q, ids, ds, 10, 0, sample_rows(m, 96))
+    # Every query against a path that shares nothing with the exact search (no lists, no
+    # filters, no re-rank kernel): the FP32 brute force's 10 nearest.  Their FP64 distances
+    # must equal the exact result's, query by query; ids may differ only among points
+    # whose distances tie to within FP32 rounding.
+    bi, _ = rbc.bruteK(x, q, 10)
+    mism = 0
+    for c0 in range(0, m, 8192):
+        qc = q[c0:c0 + 8192].astype(np.float64)[:, None, :]
+        e64 = np.sqrt(((x[ids[c0:c0 + 8192]].astype(np.float64) - qc) ** 2).sum(-1))
+        b64 = np.sort(np.sqrt(((x[bi[c0:c0 + 8192]].astype(np.float64) - qc) ** 2).sum(-1)), axis=1)
+        assert (np.diff(e64, axis=1) >= -1e-12 * e64[:, 1:]).all()
+        assert (e64 <= b64 * (1 + REL_TOL)).all(), "the brute force found a closer point"
+        assert (np.abs(e64 - b64) <= REL_TOL * b64).all()
+        mism += int((np.sort(ids[c0:c0 + 8192], 1) != np.sort(bi[c0:c0 + 8192], 1)).any(1).sum())
+    assert mism <= m // 1000, f"{mism} queries differ in ids from the brute force"
 
 
 def test_c4_batched_256_databases():
