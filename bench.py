@@ -10,9 +10,9 @@ learned list proposal (MLP d -> 256 -> 256 -> R, FP16, trained on the GPU on 2^2
 itself).  Beside it, once each on the same index and queries: the same search without the
 proposal, and the plain FP32 exact RBC.
 
-``roofline`` is for the dominant kernel, the BF16 list scan: FLOPs the scan evaluates
-(2 d per scanned (query, point) pair, counted on the device) over its hipEvent-timed
-duration on the query stream.  ``traffic`` is its HBM bytes per launch from two
+``roofline`` is for the dominant kernel, the list scan: FLOPs the scan evaluates per
+scanned (query, point) pair (64 on the 32-coordinate sketches, 2 ldb on full BF16 rows;
+pairs counted on the device) over its hipEvent-timed duration on the query stream.  ``traffic`` is its HBM bytes per launch from two
 ``rocprofv3 --pmc`` passes (FETCH_SIZE, WRITE_SIZE) of a child run of the same query,
 each scaled by what the counter reads for a 1 GiB device copy.
 
@@ -156,28 +156,30 @@ def bench_batched(args, rbc, datagen, torch, peaks):
     st = stats[-1]
     log(f"{BATCHED_WORKLOAD}: {len(ns)} databases, {int(ns.sum())} points, {m_total} queries, k={k}; "
         f"build {build_s:.1f}s; first call {first['ms_graph']:.2f} ms (autotune {first['tune_ms']}, "
-        f"{first['tile_queries']} queries per tile)")
+        f"{first['tile_queries']} queries per tile, {first['blocks_per_cu']} workgroups per CU)")
     log(f"graph replay: {1e3 * wall / args.steps:.3f} ms/step ({st['ms_graph']:.3f} ms on the GPU), "
         f"{qps:,.0f} q/s, replayed {bool(st['graph_replayed'])}, same results as the first call: {same}")
     # Roofline of the persistent kernel: the FLOPs it evaluates (2 d per (query, point)
     # and per (query, rep) distance, counted on the device) over the graph's hipEvent time
-    # (which also holds the re-rank kernel), against the FP32 peak: the kernel runs its
-    # distances on the FP32 vector ALUs.
+    # (which also holds the re-rank kernel).  For L2 (every instance here) the wave kernel
+    # takes its distances on split-BF16 MFMA, three products per distance product, so the
+    # peak for these FLOPs is a third of the dense BF16 rate.
     gflop = st["flops"] / 1e9
     ach = gflop / st["ms_graph"]  # TFLOP/s
-    roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(ach / FP32_PEAK_TFLOPS, 4), "traffic": None,
-            "kernel": "batched_exact_kernel<0> + batched_select (one graph replay)",
+    split_peak = MFMA_BF16_PEAK_TFLOPS / 3.0
+    roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": round(split_peak, 1), "unit": "TFLOP/s",
+            "frac": round(ach / split_peak, 4), "traffic": None,
+            "kernel": "batched_wave_kernel + batched_select (one graph replay)",
             "flops_per_launch": st["flops"], "pairs_per_launch": st["pairs_scan"],
             "pairs_frac_of_brute": round(st["pairs_scan"] / float((ns.astype(np.float64) * ms).sum()), 4),
-            "peak_note": "FP32 vector FMA rate (the kernel's distances are FP32 VALU)",
-            "frac_of_measured_fp32": round(ach / peaks["fp32_fma_tflops"], 4)}
+            "peak_note": "dense BF16 MFMA rate / 3 (split-BF16: hi.hi + hi.lo + lo.hi per product)",
+            "frac_of_measured_mfma": round(ach / (peaks["bf16_mfma_tflops"] / 3.0), 4)}
     log(f"batched kernel: {st['steps']:.4e} 64-point steps over all tiles, longest tile "
         f"{st['longest_tile_ms']:.2f} ms")
     roof["steps"] = st["steps"]
     roof["longest_tile_ms"] = round(st["longest_tile_ms"], 3)
     log(f"batched kernel: {st['pairs_scan']:.4e} pairs, {gflop:.1f} GFLOP in {st['ms_graph']:.2f} ms = "
-        f"{ach:.1f} TFLOP/s ({ach / FP32_PEAK_TFLOPS:.1%} of the FP32 peak)")
+        f"{ach:.1f} TFLOP/s ({ach / split_peak:.1%} of the split-BF16 MFMA peak)")
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -208,15 +210,22 @@ def bench_batched(args, rbc, datagen, torch, peaks):
         "metric": "exact k-NN queries/sec, 256 databases in one batched launch (k=16)",
         "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
         "data": "synthetic",
         "config": {"workload": BATCHED_WORKLOAD, "instances": len(ns), "n_total": int(ns.sum()),
                    "m_total": m_total, "k": k, "tile_queries": int(st["tile_queries"]),
                    "build_s": round(build_s, 2)},
         "roofline": roof, "cpu_baseline": cpu,
         "graph_ms": round(st["ms_graph"], 3), "graph_replayed": bool(st["graph_replayed"]),
-        "autotune_ms": {"16": round(first["tune_ms"][0], 3), "32": round(first["tune_ms"][1], 3),
-                        "64": round(first["tune_ms"][2], 3)},
+        # L2 instances run the wave kernel (16-query tiles), whose tuner tries a quarter, half
+        # and all of the workgroups per CU that fit; the tile kernel's tries 16-, 32- and
+        # 64-query tiles.
+        "autotune_ms": ({"quarter_of_resident_workgroups": round(first["tune_ms"][0], 3),
+                         "half": round(first["tune_ms"][1], 3), "all": round(first["tune_ms"][2], 3)}
+                        if first["blocks_per_cu"] > 1 else
+                        {"16": round(first["tune_ms"][0], 3), "32": round(first["tune_ms"][1], 3),
+                         "64": round(first["tune_ms"][2], 3)}),
+        "blocks_per_cu": int(first["blocks_per_cu"]),
         "uncertified_queries": int(st["uncertified"]), "replays_match_first_call": same,
         "measured_peaks": peaks,
     }
@@ -482,6 +491,7 @@ def main():
             "scan_ms": round(float(np.mean([s_["ms_scan"] for s_ in s_ns_all])), 3),
             "scan_tflops": round(s_ns["flops_scan"] * s_ns["pairs_scan"] /
                                  (float(np.mean([s_["ms_scan"] for s_ in s_ns_all])) * 1e-3) / 1e12, 2),
+            "pairs_scan": s_ns["pairs_scan"], "pairs_probe": s_ns["pairs_probe"], "proposal": use_mlp,
             "same_results": bool(torch.equal(ref_ids2, ids) and torch.equal(ref_ds2, ds))}
         extra["sketch_gain"] = round(qps / (m * args.steps / w_ns), 4)
         extra["sketch"] = idx.sketch_info()

@@ -54,19 +54,30 @@ __device__ __forceinline__ uint32_t sample_row(uint32_t i, uint32_t S, uint32_t 
   return (uint32_t)(((unsigned long long)i * n) / S);
 }
 
-// Block c0/64: coordinates c0 .. c0+63; thread (part = t / 64, c = t % 64) sums sample
-// rows part, part + 4, ... in FP64; the four parts are added in a fixed order.
-__global__ __launch_bounds__(256) void sk_mean(const float *xp, size_t ld, uint32_t n, uint32_t S, uint32_t d,
-                                               double *mu) {
+// Block (c0/64, ch): coordinates c0 .. c0+63 over the chunk's sample rows; thread
+// (part = t / 64, c = t % 64) sums rows part, part + 4, ... of the chunk in FP64, and the
+// four parts are added in a fixed order into part_out[ch][col].
+__global__ __launch_bounds__(256) void sk_mean_part(const float *xp, size_t ld, uint32_t n, uint32_t S, uint32_t d,
+                                                    double *part_out) {
   __shared__ double part[4][64];
   const int t = threadIdx.x, c = t % 64, pt = t / 64;
-  const uint32_t col = blockIdx.x * 64 + c;
+  const uint32_t col = blockIdx.x * 64 + c, ch = blockIdx.y;
+  const uint32_t s0 = ch * kSkChunk, s1 = min(S, s0 + kSkChunk);
   double s = 0.0;
   if (col < d)
-    for (uint32_t i = pt; i < S; i += 4) s += (double)xp[size_t(sample_row(i, S, n)) * ld + col];
+    for (uint32_t i = s0 + pt; i < s1; i += 4) s += (double)xp[size_t(sample_row(i, S, n)) * ld + col];
   part[pt][c] = s;
   __syncthreads();
-  if (pt == 0 && col < d) mu[col] = (((part[0][c] + part[1][c]) + part[2][c]) + part[3][c]) / (double)S;
+  if (pt == 0 && col < d) part_out[size_t(ch) * d + col] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+}
+
+// mu[col] = sum over chunks (in chunk order) / S.
+__global__ void sk_mean_sum(const double *part, uint32_t nch, uint32_t d, uint32_t S, double *mu) {
+  const uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= d) return;
+  double s = 0.0;
+  for (uint32_t c = 0; c < nch; ++c) s += part[size_t(c) * d + col];
+  mu[col] = s / (double)S;
 }
 
 // Partial covariance sums: block (tile ti, tile tj, chunk ch) adds, over the chunk's
@@ -683,7 +694,10 @@ int sketch_build(rbc_exact *h, hipStream_t st) {
   RBC_TRY(part.alloc(size_t(nch) * d * d));
   RBC_TRY(C64.alloc(size_t(d) * d));
   RBC_TRY(od2.alloc(S));
-  hipLaunchKernelGGL(sk_mean, dim3(cdiv(d, 64)), dim3(256), 0, st, h->xp.get(), (size_t)ld, n, S, d, mu64.get());
+  // The mean's chunk partials use part's first nch * d entries before the covariance's.
+  hipLaunchKernelGGL(sk_mean_part, dim3(cdiv(d, 64), nch), dim3(256), 0, st, h->xp.get(), (size_t)ld, n, S, d,
+                     part.get());
+  hipLaunchKernelGGL(sk_mean_sum, dim3(cdiv(d, 64)), dim3(64), 0, st, part.get(), nch, d, S, mu64.get());
   hipLaunchKernelGGL(sk_cov_part, dim3(cdiv(d, 32), cdiv(d, 32), nch), dim3(256), 0, st, h->xp.get(), (size_t)ld, n,
                      S, d, mu64.get(), part.get());
   hipLaunchKernelGGL(sk_cov_sum, dim3(cdiv(d * d, 256)), dim3(256), 0, st, part.get(), nch, d, S, C64.get());
