@@ -11,6 +11,7 @@
 // LDS-staged operands, generic strides so one kernel serves X W^T, dY^T X and dY W),
 // with bias and GELU fused in its epilogue; softmax cross-entropy; GELU backward;
 // column sums; Adam on FP32 master weights with FP16 copies.
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -263,22 +264,31 @@ __global__ __launch_bounds__(256) void gemm_f16_argmax(GemmArgs g) {
     }
 }
 
+// The smallest float above x (x not NaN): -FLT_MAX for -inf, the least denormal for +-0.
+__device__ __forceinline__ float next_float_up(float x) {
+  if (x == -INFINITY) return -FLT_MAX;
+  if (x == 0.f) return __int_as_float(1);
+  const int b = __float_as_int(x);
+  return __int_as_float(b >= 0 ? b + 1 : b - 1);
+}
+
 // Last layer with the top-P lists fused (inference): for every row i, the Pr <= P largest
-// values of (alpha A W^T + b)(i, :) (descending, lowest column among equal values; NaN
-// counts as -inf) go to lists[i*Pr ..], and count[i] gets how many of them it takes for
+// values of (A W^T + b)(i, :) (descending, lowest column among equal values; NaN counts as
+// -inf; alpha must be 0 or 1) go to lists[i*Pr ..], and count[i] gets how many of them it takes for
 // their softmax probabilities to reach tau (0 if all Pr fall short), so the logits never
 // leave the chip.  As gemm_f16_argmax, a block owns 64 rows and walks all N columns in
 // 64-column tiles with the tile of W (and its biases) in LDS, but with the MFMA operands
 // swapped: lane l's accumulators hold 4 columns of ONE row (row l%16 of the query block), so
 // each lane keeps, per query block, its P best (value, column) sorted in registers and an
-// online softmax normaliser (running max, sum of exponentials).  The 8 partial lists of a
-// row (4 lanes x 2 waves) are merged by one thread per row through LDS.
+// online softmax normaliser (running max, sum of exponentials).  The accumulators start at
+// the columns' biases (-inf past the last column, so no bounds test per value).  The 8
+// partial lists of a row (4 lanes x 2 waves) are merged by one thread per row through LDS.
 template <int KS, int P>
 __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float tau, uint32_t *lists,
                                                      uint32_t *count) {
   constexpr int K = 32 * KS, CPR = K / 8, PITCH = CPR + 1;
   __shared__ f16x8 Ws[kGT * PITCH];
-  __shared__ float bs[kGT];
+  __shared__ __attribute__((aligned(16))) float bs[kGT];
   __shared__ float mv[kGT][8][P];
   __shared__ int mi[kGT][8][P];
   __shared__ float pm[kGT][8], ps[kGT][8];
@@ -286,7 +296,6 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
   const int i0 = blockIdx.x * kGT, rb = (w / 2) * 32, cb = (w % 2) * 32;
   const _Float16 *A = static_cast<const _Float16 *>(g.A);
   const _Float16 *W = static_cast<const _Float16 *>(g.B);  // column j of B = row j of W
-  const float alpha = g.alpha != 0.f ? g.alpha : 1.f;
   f16x8 af[2][KS];
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi) {
@@ -318,7 +327,7 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
       const int c = t + 256 * e, j = j0 + c / CPR;
       wr[e] = j < g.N ? *reinterpret_cast<const f16x8 *>(W + int64_t(j) * g.b_cs + (c % CPR) * 8) : f16x8{};
     }
-    if (t < kGT) br = (j0 + t < g.N && g.bias) ? g.bias[j0 + t] : 0.f;
+    if (t < kGT) br = j0 + t < g.N ? (g.bias ? g.bias[j0 + t] : 0.f) : -INFINITY;
   };
   auto tile = [&](int j0, f16x8 (&wr)[KS], float &br) {
     __syncthreads();
@@ -332,9 +341,11 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
     if (j0 + 2 * kGT < g.N) wload(j0 + 2 * kGT, wr, br);
     f32x4 acc[2][2];
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj) {
+      const f32x4 b4 = *reinterpret_cast<const f32x4 *>(&bs[cb + bj * 16 + (l / 16) * 4]);
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
+      for (int bi = 0; bi < 2; ++bi) acc[bi][bj] = b4;
+    }
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
@@ -353,9 +364,7 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
       float m8 = -INFINITY;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const int jc = cb + (e / 4) * 16 + (l / 16) * 4 + e % 4;
-        v[e] = j0 + jc < g.N ? alpha * acc[bi][e / 4][e % 4] + bs[jc] : -INFINITY;
-        if (v[e] != v[e]) v[e] = -INFINITY;
+        v[e] = fmaxf(acc[bi][e / 4][e % 4], -INFINITY);  // v_max_f32: a NaN becomes -inf
         m8 = fmaxf(m8, v[e]);
       }
       if (m8 > mx[bi]) {
@@ -363,8 +372,10 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
         mx[bi] = m8;
       }
       if (mx[bi] != -INFINITY) {
+        // exp(v - mx) as 2^(v log2 e - mx log2 e), one FMA per value; exp2(-inf) = 0.
+        const float ml = mx[bi] * 1.4426950408889634f;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) se[bi] += __expf(v[e] - mx[bi]);  // exp(-inf) = 0
+        for (int e = 0; e < 8; ++e) se[bi] += __builtin_amdgcn_exp2f(__builtin_fmaf(v[e], 1.4426950408889634f, -ml));
       }
       // The 4 lanes of a row (l, l^16, l^32, l^48) each hold P values >= their own P-th,
       // so a value below another lane's P-th is not among the row's P best; nor is one
@@ -378,9 +389,11 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
       const float oth = fmaxf(
           fmaxf(__shfl_xor(own, 16, 64), fmaxf(__shfl_xor(own, 32, 64), __shfl_xor(own, 48, 64))),
           fminf(fminf(q4, __shfl_xor(q4, 16, 64)), fminf(__shfl_xor(q4, 32, 64), __shfl_xor(q4, 48, 64))));
+      // v > own and v >= oth in one compare: v >= max(the next float above own, oth).
+      const float thr = fmaxf(next_float_up(own), oth);
       uint32_t pend = 0;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) pend |= (v[e] > own && v[e] >= oth) ? (1u << e) : 0u;
+      for (int e = 0; e < 8; ++e) pend |= v[e] >= thr ? (1u << e) : 0u;
       while (pend) {
         const int e = __builtin_ctz(pend);
         pend &= pend - 1;
