@@ -274,26 +274,35 @@ __device__ __forceinline__ float next_float_up(float x) {
 
 // Last layer with the top-P lists fused (inference): for every row i, the Pr <= P largest
 // values of (A W^T + b)(i, :) (descending, lowest column among equal values; NaN counts as
-// -inf; alpha must be 0 or 1) go to lists[i*Pr ..], and count[i] gets how many of them it takes for
-// their softmax probabilities to reach tau (0 if all Pr fall short), so the logits never
-// leave the chip.  As gemm_f16_argmax, a block owns 64 rows and walks all N columns in
-// 64-column tiles with the tile of W (and its biases) in LDS, but with the MFMA operands
-// swapped: lane l's accumulators hold 4 columns of ONE row (row l%16 of the query block), so
-// each lane keeps, per query block, its P best (value, column) sorted in registers and an
-// online softmax normaliser (running max, sum of exponentials).  The accumulators start at
-// the columns' biases (-inf past the last column, so no bounds test per value).  The 8
-// partial lists of a row (4 lanes x 2 waves) are merged by one thread per row through LDS.
+// -inf; alpha must be 0 or 1) go to lists[i*Pr ..], and count[i] gets how many of them it
+// takes for their softmax probabilities to reach tau (0 if all Pr fall short), so the
+// logits never leave the chip.  A block owns kToppRows = 128 rows (wave w: rows 32 w ..
+// +31, as two 16-row MFMA blocks whose fragments stay in registers) and walks all N
+// columns in 64-column tiles; the tile of W and its biases are staged in LDS once for the
+// 128 rows.  The MFMA operands are swapped, so lane l's accumulators hold 4 columns of ONE
+// row (row l%16 of a row block): the 4 lanes l%16, +16, +32, +48 of a wave together see
+// every column of a row, and each keeps its P best (value, column) sorted in registers and
+// an online softmax normaliser (running max, sum of exponentials).  The accumulators start
+// at the columns' biases (-inf past the last column, so no bounds test per value).  The 4
+// partial lists of a row are merged by one thread per row through LDS.
+constexpr int kToppRows = 128;
+
 template <int KS, int P>
 __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float tau, uint32_t *lists,
                                                      uint32_t *count) {
   constexpr int K = 32 * KS, CPR = K / 8, PITCH = CPR + 1;
-  __shared__ f16x8 Ws[kGT * PITCH];
+  constexpr int kWsBytes = kGT * PITCH * 16;
+  constexpr int kMergeBytes = kToppRows * 4 * P * 8 + kToppRows * 4 * 8;
+  // The W tile during the column walk; the partial lists and normalisers after it.
+  __shared__ __attribute__((aligned(16))) unsigned char smem[kWsBytes > kMergeBytes ? kWsBytes : kMergeBytes];
   __shared__ __attribute__((aligned(16))) float bs[kGT];
-  __shared__ float mv[kGT][8][P];
-  __shared__ int mi[kGT][8][P];
-  __shared__ float pm[kGT][8], ps[kGT][8];
+  f16x8 *Ws = reinterpret_cast<f16x8 *>(smem);
+  float(*mv)[4][P] = reinterpret_cast<float(*)[4][P]>(smem);
+  int(*mi)[4][P] = reinterpret_cast<int(*)[4][P]>(smem + kToppRows * 4 * P * 4);
+  float(*pm)[4] = reinterpret_cast<float(*)[4]>(smem + kToppRows * 4 * P * 8);
+  float(*ps)[4] = reinterpret_cast<float(*)[4]>(smem + kToppRows * 4 * P * 8 + kToppRows * 4 * 4);
   const int t = threadIdx.x, w = t / 64, l = t % 64;
-  const int i0 = blockIdx.x * kGT, rb = (w / 2) * 32, cb = (w % 2) * 32;
+  const int i0 = blockIdx.x * kToppRows, rb = w * 32;
   const _Float16 *A = static_cast<const _Float16 *>(g.A);
   const _Float16 *W = static_cast<const _Float16 *>(g.B);  // column j of B = row j of W
   f16x8 af[2][KS];
@@ -339,43 +348,44 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
     if (t < kGT) bs[t] = br;
     __syncthreads();
     if (j0 + 2 * kGT < g.N) wload(j0 + 2 * kGT, wr, br);
-    f32x4 acc[2][2];
+    f32x4 acc[2][4];
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj) {
-      const f32x4 b4 = *reinterpret_cast<const f32x4 *>(&bs[cb + bj * 16 + (l / 16) * 4]);
+    for (int bj = 0; bj < 4; ++bj) {
+      const f32x4 b4 = *reinterpret_cast<const f32x4 *>(&bs[bj * 16 + (l / 16) * 4]);
 #pragma unroll
       for (int bi = 0; bi < 2; ++bi) acc[bi][bj] = b4;
     }
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj) {
-        const f16x8 wv = Ws[(cb + bj * 16 + l % 16) * PITCH + ks * 4 + l / 16];
+      for (int bj = 0; bj < 4; ++bj) {
+        const f16x8 wv = Ws[(bj * 16 + l % 16) * PITCH + ks * 4 + l / 16];
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
           acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wv, af[bi][ks], acc[bi][bj], 0, 0, 0);
       }
-    // Lane l holds row rb + bi*16 + l%16, columns j0 + cb + bj*16 + 4(l/16) + r: value
-    // e = 4 bj + r of a query block is column j0 + cb + 16 (e/4) + 4(l/16) + e%4, so a
-    // lane's values come in ascending column order.
+    // Lane l holds row rb + bi*16 + l%16, columns j0 + bj*16 + 4(l/16) + r: value e = 4 bj
+    // + r of a row block is column j0 + 16 (e/4) + 4(l/16) + e%4, so a lane's values come
+    // in ascending column order.
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
-      float v[8];
-      float m8 = -INFINITY;
+      float v[16];
+      float m16 = -INFINITY;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
+      for (int e = 0; e < 16; ++e) {
         v[e] = fmaxf(acc[bi][e / 4][e % 4], -INFINITY);  // v_max_f32: a NaN becomes -inf
-        m8 = fmaxf(m8, v[e]);
+        m16 = fmaxf(m16, v[e]);
       }
-      if (m8 > mx[bi]) {
-        se[bi] = mx[bi] == -INFINITY ? 0.f : se[bi] * __expf(mx[bi] - m8);
-        mx[bi] = m8;
+      if (m16 > mx[bi]) {
+        se[bi] = mx[bi] == -INFINITY ? 0.f : se[bi] * __expf(mx[bi] - m16);
+        mx[bi] = m16;
       }
       if (mx[bi] != -INFINITY) {
         // exp(v - mx) as 2^(v log2 e - mx log2 e), one FMA per value; exp2(-inf) = 0.
         const float ml = mx[bi] * 1.4426950408889634f;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) se[bi] += __builtin_amdgcn_exp2f(__builtin_fmaf(v[e], 1.4426950408889634f, -ml));
+        for (int e = 0; e < 16; ++e)
+          se[bi] += __builtin_amdgcn_exp2f(__builtin_fmaf(v[e], 1.4426950408889634f, -ml));
       }
       // The 4 lanes of a row (l, l^16, l^32, l^48) each hold P values >= their own P-th,
       // so a value below another lane's P-th is not among the row's P best; nor is one
@@ -393,15 +403,15 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
       const float thr = fmaxf(next_float_up(own), oth);
       uint32_t pend = 0;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) pend |= v[e] >= thr ? (1u << e) : 0u;
+      for (int e = 0; e < 16; ++e) pend |= v[e] >= thr ? (1u << e) : 0u;
       while (pend) {
         const int e = __builtin_ctz(pend);
         pend &= pend - 1;
         float x = v[0];
 #pragma unroll
-        for (int s = 1; s < 8; ++s) x = e == s ? v[s] : x;
+        for (int s = 1; s < 16; ++s) x = e == s ? v[s] : x;
         if (!(x > tv[bi][P - 1])) continue;
-        const int j = j0 + cb + (e / 4) * 16 + (l / 16) * 4 + e % 4;
+        const int j = j0 + (e / 4) * 16 + (l / 16) * 4 + e % 4;
         // Branch-free insertion from the bottom: slot s takes slot s-1's entry when x
         // belongs above it, x itself when it belongs exactly there, else keeps its own.
 #pragma unroll
@@ -426,8 +436,9 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
     tile(j0, wr0, br0);
     if (j0 + kGT < g.N) tile(j0 + kGT, wr1, br1);
   }
-  // Partial p = 4 (w % 2) + l/16 of row rb + bi*16 + l%16.
-  const int part = (w % 2) * 4 + l / 16;
+  // Partial p = l/16 of row rb + bi*16 + l%16 (the W tile's LDS is reused).
+  __syncthreads();
+  const int part = l / 16;
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi) {
     const int rr = rb + bi * 16 + l % 16;
@@ -440,21 +451,21 @@ __global__ __launch_bounds__(256) void gemm_f16_topp(GemmArgs g, int Pr, float t
     ps[rr][part] = se[bi];
   }
   __syncthreads();
-  if (t >= kGT || i0 + t >= g.M) return;
+  if (t >= kToppRows || i0 + t >= g.M) return;
   const int rr = t, i = i0 + t;
   float M = -INFINITY;
-  for (int q = 0; q < 8; ++q) M = fmaxf(M, pm[rr][q]);
+  for (int q = 0; q < 4; ++q) M = fmaxf(M, pm[rr][q]);
   float S = 0.f;
-  for (int q = 0; q < 8; ++q)
+  for (int q = 0; q < 4; ++q)
     if (pm[rr][q] != -INFINITY) S += ps[rr][q] * __expf(pm[rr][q] - M);
-  int head[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int head[4] = {0, 0, 0, 0};
   float mass = 0.f;
   uint32_t cnt = 0;
   for (int r = 0; r < Pr; ++r) {
     int bq = -1;
     float bv = -INFINITY;
     int bidx = 0x7FFFFFFF;
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < 4; ++q) {
       if (head[q] >= P) continue;
       const float v = mv[rr][q][head[q]];
       const int j = mi[rr][q][head[q]];
@@ -849,7 +860,7 @@ int mlp_propose_lists(rbc_mlp *m, const float *q, size_t ldq, uint32_t rows, uin
     g.bias = m->Bv(2);
     g.M = B; g.N = m->C; g.K = m->H2;
     uint32_t *lo = lists + size_t(r0) * P, *co = count + r0;
-    const dim3 grid(cdiv(B, kGT)), blk(256);
+    const dim3 grid(cdiv(B, kToppRows)), blk(256);
     const int KS = (int)m->H2 / 32;
 #define RBC_TOPP(ks, pp) hipLaunchKernelGGL((gemm_f16_topp<ks, pp>), grid, blk, 0, st, g, (int)P, tau, lo, co)
 #define RBC_TOPP_P(ks) \
