@@ -36,11 +36,12 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
 }
 
 // One wave per query: BF16 copy (zero padded to ldb), |q~|^2 and |q - q~| (FP64); with
-// qn2, also ||q||^2 (FP64 sum rounded to nearest) and the absolute error bound of the
-// split-BF16 query-rep distances of this query (qr_error).
+// qlo, also the low halves bf16(q - q~) of the split operands (the FP32 difference is
+// exact); with qn2, also ||q||^2 (FP64 sum rounded to nearest) and the absolute error bound
+// of the split-BF16 query-rep distances of this query (qr_error).
 __global__ __launch_bounds__(256) void prep_queries(const float *q, size_t ldq, uint32_t m, uint32_t d,
-                                                    uint32_t ldb, uint16_t *qb, float *qn, float *qe, float *qn2,
-                                                    float *qerr, double c_split, double rep_nmax) {
+                                                    uint32_t ldb, uint16_t *qb, uint16_t *qlo, float *qn, float *qe,
+                                                    float *qn2, float *qerr, double c_split, double rep_nmax) {
   const uint32_t r = blockIdx.x * 4 + threadIdx.x / kWave;
   if (r >= m) return;
   const int l = lane_id();
@@ -49,6 +50,7 @@ __global__ __launch_bounds__(256) void prep_queries(const float *q, size_t ldq, 
     float v = c < d ? q[size_t(r) * ldq + c] : 0.f;
     uint16_t h = f32_to_bf16_rne(v);
     qb[size_t(r) * ldb + c] = h;
+    if (qlo) qlo[size_t(r) * ldb + c] = f32_to_bf16_rne(v - __uint_as_float(uint32_t(h) << 16));
     double hv = (double)__uint_as_float(uint32_t(h) << 16);
     nb += hv * hv;
     double e = (double)v - hv;
@@ -90,11 +92,12 @@ __global__ __launch_bounds__(256) void prep_queries(const float *q, size_t ldq, 
 // lane l holds reps 4 (l/16) .. +3 of query l % 16 and writes them as one 16-byte run of
 // D's row.  A wave computes 64 reps x 64 queries, a block of four waves 128 x 128; the
 // 16 rep tiles of a query tile run on one XCD (block b on XCD b % 8), so the query rows
-// are fetched from HBM once.  The reps' hi/lo rows (R x ldb, zero padded) stay in L2.
+// are fetched from HBM once.  The queries' hi/lo rows come split by prep_queries and the
+// reps' (R x ldb, zero padded) stay in L2: every operand is one 16-byte load.
 constexpr int kQM = 128;  // queries and reps per block
 
-template <int KS, bool VQ>
-__global__ __launch_bounds__(256) void qr_mfma(const float *Q, size_t ldq, uint32_t m, uint32_t d,
+template <int KS>
+__global__ __launch_bounds__(256) void qr_mfma(const uint16_t *qhi, const uint16_t *qlo, uint32_t m,
                                                const uint16_t *rhi, const uint16_t *rlo, uint32_t ldb,
                                                const float *rn2, uint32_t R, const float *qn2, float *D,
                                                uint32_t nqt, uint32_t nrt) {
@@ -127,25 +130,13 @@ __global__ __launch_bounds__(256) void qr_mfma(const float *Q, size_t ldq, uint3
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t q = q0 + j * 16 + (l % 16);
-      float v[8];
-      if (q < m && VQ && c0 + 8 <= d) {
-        const float4 *src = reinterpret_cast<const float4 *>(Q + size_t(q) * ldq + c0);
-        const float4 a = src[0], bb = src[1];
-        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = bb.x, v[5] = bb.y, v[6] = bb.z, v[7] = bb.w;
+      if (q < m) {
+        Bh[j] = *reinterpret_cast<const bf16x8 *>(qhi + size_t(q) * ldb + c0);
+        Bl[j] = *reinterpret_cast<const bf16x8 *>(qlo + size_t(q) * ldb + c0);
       } else {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = (q < m && c0 + e < d) ? Q[size_t(q) * ldq + c0 + e] : 0.f;
+        Bh[j] = bf16x8{};
+        Bl[j] = bf16x8{};
       }
-      typedef __attribute__((__vector_size__(8 * sizeof(uint16_t)))) uint16_t u16x8;
-      u16x8 hv, lv;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const uint16_t h = f32_to_bf16_rne(v[e]);
-        hv[e] = h;
-        lv[e] = f32_to_bf16_rne(v[e] - __uint_as_float(uint32_t(h) << 16));
-      }
-      Bh[j] = __builtin_bit_cast(bf16x8, hv);
-      Bl[j] = __builtin_bit_cast(bf16x8, lv);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -697,9 +688,9 @@ void launch_probe(const Bf16ProbeArgs &a, uint32_t nitems, hipStream_t st) {
 }  // namespace
 
 int bf16_prepare_queries(rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint16_t *qb, float *qn,
-                         float *qe, hipStream_t st, float *qn2, float *qerr) {
-  hipLaunchKernelGGL(prep_queries, dim3(cdiv(m, 4)), dim3(256), 0, st, q, ldq, m, h->d, h->ldb, qb, qn, qe, qn2, qerr,
-                     qr_split_err(h), h->rep_nmax);
+                         float *qe, hipStream_t st, float *qn2, float *qerr, uint16_t *qlo) {
+  hipLaunchKernelGGL(prep_queries, dim3(cdiv(m, 4)), dim3(256), 0, st, q, ldq, m, h->d, h->ldb, qb, qlo, qn, qe, qn2,
+                     qerr, qr_split_err(h), h->rep_nmax);
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;
 }
@@ -710,27 +701,22 @@ double qr_split_err(const rbc_exact *h) {
 }
 
 template <int KS>
-static void launch_qr(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, const float *qn2, float *D, bool vq,
-                      uint32_t grid, uint32_t nqt, uint32_t nrt, hipStream_t st) {
-  if (vq)
-    hipLaunchKernelGGL((qr_mfma<KS, true>), dim3(grid), dim3(256), 0, st, q, ldq, m, h->d, h->reps_b.get(),
-                       h->reps_lo.get(), h->ldb, h->reps_n2.get(), h->R, qn2, D, nqt, nrt);
-  else
-    hipLaunchKernelGGL((qr_mfma<KS, false>), dim3(grid), dim3(256), 0, st, q, ldq, m, h->d, h->reps_b.get(),
-                       h->reps_lo.get(), h->ldb, h->reps_n2.get(), h->R, qn2, D, nqt, nrt);
+static void launch_qr(const rbc_exact *h, const uint16_t *qhi, const uint16_t *qlo, uint32_t m, const float *qn2,
+                      float *D, uint32_t grid, uint32_t nqt, uint32_t nrt, hipStream_t st) {
+  hipLaunchKernelGGL((qr_mfma<KS>), dim3(grid), dim3(256), 0, st, qhi, qlo, m, h->reps_b.get(), h->reps_lo.get(),
+                     h->ldb, h->reps_n2.get(), h->R, qn2, D, nqt, nrt);
 }
 
-int bf16_qr_dist(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, const float *qn2, float *D,
+int bf16_qr_dist(const rbc_exact *h, const uint16_t *qhi, const uint16_t *qlo, uint32_t m, const float *qn2, float *D,
                  hipStream_t st) {
   RBC_ARG_CHECK(h->metric == RBC_L2 && h->ldb <= 128 && h->reps_lo.get(), "split-BF16 q->rep distances need L2, d <= 128");
   const uint32_t nqt = cdiv(m, kQM), nrt = cdiv(h->R, kQM);
   const uint32_t grid = cdiv(nqt, kXCD) * kXCD * nrt;
-  const bool vq = ldq % 4 == 0 && reinterpret_cast<uintptr_t>(q) % 16 == 0;
   switch (h->ldb / 32) {
-    case 1: launch_qr<1>(h, q, ldq, m, qn2, D, vq, grid, nqt, nrt, st); break;
-    case 2: launch_qr<2>(h, q, ldq, m, qn2, D, vq, grid, nqt, nrt, st); break;
-    case 3: launch_qr<3>(h, q, ldq, m, qn2, D, vq, grid, nqt, nrt, st); break;
-    default: launch_qr<4>(h, q, ldq, m, qn2, D, vq, grid, nqt, nrt, st); break;
+    case 1: launch_qr<1>(h, qhi, qlo, m, qn2, D, grid, nqt, nrt, st); break;
+    case 2: launch_qr<2>(h, qhi, qlo, m, qn2, D, grid, nqt, nrt, st); break;
+    case 3: launch_qr<3>(h, qhi, qlo, m, qn2, D, grid, nqt, nrt, st); break;
+    default: launch_qr<4>(h, qhi, qlo, m, qn2, D, grid, nqt, nrt, st); break;
   }
   RBC_HIP_TRY(hipGetLastError());
   return RBC_OK;

@@ -689,12 +689,12 @@ struct Timer {
 // qn, qe) and, for L2, the split-BF16 MFMA distances with their per-query absolute error
 // bounds (qn2, qerr); otherwise FP32 VALU distances (relative error fma_rel_err(dpad)).
 static int rep_distances(rbc_exact *h, const float *qd, size_t ldq, uint32_t m, int precision, float *Dqr,
-                         uint16_t *qb, float *qn, float *qe, float *qn2, float *qerr, hipStream_t st) {
+                         uint16_t *qb, float *qn, float *qe, float *qn2, float *qerr, uint16_t *qlo, hipStream_t st) {
   const uint32_t R = h->R, d = h->d;
   const bool qr_split = precision == RBC_PREC_BF16 && h->metric == RBC_L2;
   if (precision == RBC_PREC_BF16) {
-    RBC_TRY(bf16_prepare_queries(h, qd, ldq, m, qb, qn, qe, st, qn2, qerr));
-    if (qr_split) RBC_TRY(bf16_qr_dist(h, qd, ldq, m, qn2, Dqr, st));
+    RBC_TRY(bf16_prepare_queries(h, qd, ldq, m, qb, qn, qe, st, qn2, qerr, qr_split ? qlo : nullptr));
+    if (qr_split) RBC_TRY(bf16_qr_dist(h, qb, qlo, m, qn2, Dqr, st));
   }
   if (!qr_split) {
     const dim3 g(cdiv(m, kQRT), cdiv(R, kQRT));
@@ -763,13 +763,16 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
     qe = static_cast<float *>(p);
   }
   float *qn2 = nullptr;
+  uint16_t *qlo = nullptr;
   if (precision == RBC_PREC_BF16 && h->metric == RBC_L2) {
     RBC_TRY(ws.get(kQ + 38, size_t(m) * 4, &p));
     qn2 = static_cast<float *>(p);
     RBC_TRY(ws.get(kQ + 39, size_t(m) * 4, &p));
     qerr = static_cast<float *>(p);
+    RBC_TRY(ws.get(kQ + 44, size_t(m) * h->ldb * 2, &p));
+    qlo = static_cast<uint16_t *>(p);
   }
-  RBC_TRY(rep_distances(h, qd, ldq, m, precision, Dqr, qb, qn, qe, qn2, qerr, st));
+  RBC_TRY(rep_distances(h, qd, ldq, m, precision, Dqr, qb, qn, qe, qn2, qerr, qlo, st));
   const bool bf16 = precision == RBC_PREC_BF16;
   // BF16 filter on an index whose data lie close to a low-dimensional subspace: the scan
   // tests the queries' and points' projected sketches (exact_sketch.hip).
@@ -1171,16 +1174,18 @@ extern "C" int rbc_exact_rep_dists(rbc_exact *h, const float *q, uint32_t m, uin
   DevMat Q;
   RBC_TRY(Q.stage(q, m, h->d, ldq, mem, st));
   DevBuf<float> Dd, qn, qe, qn2, qerr;
-  DevBuf<uint16_t> qb;
+  DevBuf<uint16_t> qb, qlo;
   RBC_TRY(Dd.alloc(size_t(m) * h->R));
   if (split) {
     RBC_TRY(qb.alloc(size_t(m) * h->ldb));
+    RBC_TRY(qlo.alloc(size_t(m) * h->ldb));
     RBC_TRY(qn.alloc(m));
     RBC_TRY(qe.alloc(m));
     RBC_TRY(qn2.alloc(m));
     RBC_TRY(qerr.alloc(m));
   }
-  RBC_TRY(rep_distances(h, Q.p, Q.ld, m, precision, Dd.get(), qb.get(), qn.get(), qe.get(), qn2.get(), qerr.get(), st));
+  RBC_TRY(rep_distances(h, Q.p, Q.ld, m, precision, Dd.get(), qb.get(), qn.get(), qe.get(), qn2.get(), qerr.get(),
+                        qlo.get(), st));
   RBC_TRY(rows_out(D, size_t(ldd) * 4, Dd.get(), size_t(h->R) * 4, size_t(h->R) * 4, m, mem, st));
   if (abs_err) {
     if (split)

@@ -32,14 +32,16 @@ int launch_select(const SelectArgs &a, int metric, hipStream_t st);
 
 // BF16 MFMA stages (exact_bf16.hip).
 // With qn2 (and qerr), also ||q||^2 and the absolute error bound of the query's
-// split-BF16 rep distances (bf16_qr_dist).
+// split-BF16 rep distances (bf16_qr_dist); with qlo, the low halves of the split operands
+// (m x ldb, beside qb's high halves).
 int bf16_prepare_queries(rbc_exact *h, const float *q, size_t ldq, uint32_t m, uint16_t *qb, float *qn,
-                         float *qe, hipStream_t st, float *qn2 = nullptr, float *qerr = nullptr);
+                         float *qe, hipStream_t st, float *qn2 = nullptr, float *qerr = nullptr,
+                         uint16_t *qlo = nullptr);
 // Relative constant c of the split-BF16 dot products: |q.r~ - q.r| <= c |q| |r|.
 double qr_split_err(const rbc_exact *h);
-// L2 query-rep distances D~ (m x R) on MFMA from split BF16 operands (qn2 from
-// bf16_prepare_queries); |D~ - D| <= qerr[q] for every rep.
-int bf16_qr_dist(const rbc_exact *h, const float *q, size_t ldq, uint32_t m, const float *qn2, float *D,
+// L2 query-rep distances D~ (m x R) on MFMA from split BF16 operands (qhi = qb, qlo and
+// qn2 from bf16_prepare_queries); |D~ - D| <= qerr[q] for every rep.
+int bf16_qr_dist(const rbc_exact *h, const uint16_t *qhi, const uint16_t *qlo, uint32_t m, const float *qn2, float *D,
                  hipStream_t st);
 // Probe entries e (query e / P): for each, the K' smallest approximate distances of its
 // item's list, ascending, as ids out_i[e*Kp..] and values out_v[e*Kp..].
