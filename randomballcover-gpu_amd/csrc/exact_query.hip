@@ -369,6 +369,14 @@ __device__ __forceinline__ void load_v(const float *p, float (&o)[V]) {
   }
 }
 
+// The largest float <= v (v not NaN), without branches.
+__device__ __forceinline__ float f32_round_down(double v) {
+  const float f = (float)v;
+  const int b = __float_as_int(f);
+  const float below = f > 0.f ? __int_as_float(b - 1) : (f < 0.f ? __int_as_float(b + 1) : -__int_as_float(1));
+  return (double)f > v ? below : f;
+}
+
 template <int NPR, int V>
 __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, uint32_t R,
                                                   const uint32_t *near,
@@ -406,28 +414,30 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
     double Dlo[V];
     const double Gp = G * (1.0 + 2e-12);
     const double Gd = Gp * (1.0 + 1e-13);
+    // The probed lists j0 and j1 are rescanned only if their probe was not complete.
+    if (metric == 0) {
+      // list_lower_bound's test without square roots or divisions: the triangle bound
+      // sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as Dqj (1-e) <= (G' + psi)^2,
+      // and each plane bound num / den <= G' as num <= G' den with den rounded up; both
+      // keep at least what it keeps.  With split-BF16 rep distances, D~ is within qerr[q]
+      // of the true value (absolute), and Dlo and the upper values B_i below carry that
+      // term too.  Branch-free over the lane's lists.
 #pragma unroll
-    for (int v = 0; v < V; ++v) {
-      const uint32_t j = jl + v;
-      Dlo[v] = 0.0;
-      if (j == j0) {
-        keep[v] = (cb & 1u) == 0;  // probed; rescan only if the probe was not complete
-      } else if (j == j1) {
-        keep[v] = (cb & 2u) == 0;
-      } else if (metric == 0) {
-        // list_lower_bound's test without square roots or divisions: the triangle bound
-        // sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as Dqj (1-e) <= (G' + psi)^2,
-        // and each plane bound num / den <= G' as num <= G' den with den rounded up; both
-        // keep at least what it keeps.  With split-BF16 rep distances, D~ is within qerr[q]
-        // of the true value (absolute), and Dlo and the upper values B_i below carry that
-        // term too.
+      for (int v = 0; v < V; ++v) {
+        const uint32_t j = jl + v;
         const double rhs = Gp + psi[v];
         Dlo[v] = (double)dq[v] * (1.0 - e) - Eq;
-        keep[v] = Dlo[v] <= rhs * rhs * (1.0 + 1e-13);
-        tri |= keep[v];
-      } else {
+        const bool kt = Dlo[v] <= rhs * rhs * (1.0 + 1e-13);
+        keep[v] = j == j0 ? (cb & 1u) == 0 : (j == j1 ? (cb & 2u) == 0 : kt);
+        tri |= kt & (j != j0) & (j != j1);
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const uint32_t j = jl + v;
+        Dlo[v] = 0.0;
         const double lb = list_lower_bound((double)dq[v], 0.0, 0.0, psi[v], e, true, metric);
-        keep[v] = lb <= G;
+        keep[v] = j == j0 ? (cb & 1u) == 0 : (j == j1 ? (cb & 2u) == 0 : lb <= G);
       }
     }
     // The planes only matter for lists the triangle bound keeps; their rr_den segments are
@@ -468,21 +478,23 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
       float Gf = (float)Gu;
       if ((double)Gf < Gu) Gf = nextafterf(Gf, INFINITY);
       const float cb1 = (float)(1.0 + e + 0x1p-21);
+      // B_i depends on the plane rep only; a missing plane rep has den = 0, and a plane
+      // with den <= 0 or against the rep's own list never prunes.  Branch-free over the
+      // lane's lists: the probed lists j0 and j1 and those the triangle dropped keep
+      // their decision.
+      float Bi[1 + NPR];
+#pragma unroll
+      for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) Bi[pi] = __builtin_fmaf(Dqi[pi], cb1, Eu);
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        if (!keep[v] || jl + v == j0 || jl + v == j1) continue;
-        const double Ad = Dlo[v] - slack[v];
-        float A = (float)Ad;
-        if ((double)A > Ad) A = nextafterf(A, -INFINITY);
+        const float A = f32_round_down(Dlo[v] - slack[v]);
         bool far = false;
 #pragma unroll
         for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) {
-          const bool ok = ji[pi] < R && ji[pi] != jl + v;
-          const float dn = ok ? den[pi][v] : 0.f;
-          const float dd = dn > 0.f ? dn : INFINITY;  // no plane: never prunes
-          far |= A > __builtin_fmaf(Gf, dd, __builtin_fmaf(ok ? Dqi[pi] : 0.f, cb1, Eu));
+          const float dn = den[pi][v];
+          far |= (dn > 0.f) & (ji[pi] != jl + v) & (A > __builtin_fmaf(Gf, dn, Bi[pi]));
         }
-        if (far) keep[v] = false;
+        keep[v] = keep[v] & !(far & (jl + v != j0) & (jl + v != j1));
       }
     }
 #pragma unroll
