@@ -390,12 +390,16 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
   const int l = lane_id(), w = threadIdx.x / kWave;
   const uint32_t jl = blockIdx.y * (kWave * V) + V * l;  // first list of this lane
   const bool live = jl < R;  // V divides R when V > 1
-  double psi[V], slack[V];
+  // Per list: psi, and the owner slack 2 e psi^2 rounded up; (1 - e) rounded down.
+  double psi[V];
+  float psf[V], slu[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    psi[v] = live ? (double)radius[jl + v] : 0.0;
-    slack[v] = 2.0 * e * psi[v] * psi[v];
+    psf[v] = live ? radius[jl + v] : 0.f;
+    psi[v] = (double)psf[v];
+    slu[v] = -f32_round_down(-(2.0 * e * psi[v] * psi[v]));
   }
+  const float c1m = f32_round_down(1.0 - e);
   uint32_t mask[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) mask[v] = 0;
@@ -411,7 +415,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
     float dq[V];
     load_v<V>(Dqr + size_t(q) * R + jl, dq);
     bool keep[V], tri = false;
-    double Dlo[V];
+    float A[V];
     const double Gp = G * (1.0 + 2e-12);
     const double Gd = Gp * (1.0 + 1e-13);
     // The probed lists j0 and j1 are rescanned only if their probe was not complete.
@@ -420,22 +424,32 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
       // sqrt(Dqj (1-e)) - psi <= G' (G' = G / (1 - 1e-12)) as Dqj (1-e) <= (G' + psi)^2,
       // and each plane bound num / den <= G' as num <= G' den with den rounded up; both
       // keep at least what it keeps.  With split-BF16 rep distances, D~ is within qerr[q]
-      // of the true value (absolute), and Dlo and the upper values B_i below carry that
-      // term too.  Branch-free over the lane's lists.
+      // of the true value (absolute), and the lower value Dlo and the upper values B_i
+      // below carry that term too.  In FP32 with every rounding towards keeping: Dlo =
+      // fma(Dqj, (1-e) rounded down, -Eq) less 2^-22 of its magnitude (an FMA is within
+      // 2^-24 of the exact value), (G' + psi)^2 from G' rounded up, times (1 + 2^-20)
+      // (three roundings of at most 2^-24 each, and the 1e-13).  Branch-free over the
+      // lane's lists.
+      const float Eqf = qerr ? qerr[q] : 0.f;
+      const float Gpf = -f32_round_down(-Gp);
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         const uint32_t j = jl + v;
-        const double rhs = Gp + psi[v];
-        Dlo[v] = (double)dq[v] * (1.0 - e) - Eq;
-        const bool kt = Dlo[v] <= rhs * rhs * (1.0 + 1e-13);
+        const float x = __builtin_fmaf(dq[v], c1m, -Eqf);
+        const float dlo = x - fabsf(x) * 0x1p-22f;
+        const float rhs = Gpf + psf[v];
+        const bool kt = dlo <= (rhs * rhs) * (1.f + 0x1p-20f);
         keep[v] = j == j0 ? (cb & 1u) == 0 : (j == j1 ? (cb & 2u) == 0 : kt);
         tri |= kt & (j != j0) & (j != j1);
+        // The plane tests' left side Dlo - slack, rounded down the same way.
+        const float a = dlo - slu[v];
+        A[v] = a - fabsf(a) * 0x1p-22f;
       }
     } else {
 #pragma unroll
       for (int v = 0; v < V; ++v) {
         const uint32_t j = jl + v;
-        Dlo[v] = 0.0;
+        A[v] = 0.f;
         const double lb = list_lower_bound((double)dq[v], 0.0, 0.0, psi[v], e, true, metric);
         keep[v] = j == j0 ? (cb & 1u) == 0 : (j == j1 ? (cb & 2u) == 0 : lb <= G);
       }
@@ -467,7 +481,7 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
         }
       }
       // The plane test num > G' den in FP32, with every rounding pushed towards keeping:
-      // A <= Dlo - slack (rounded down), and fl(Gf den + B_i) >= G' den + Dqi (1+e) + Eq,
+      // A <= Dlo - slack (above), and fl(Gf den + B_i) >= G' den + Dqi (1+e) + Eq,
       // since Gf >= G' (1 + 8u) and B_i = fl(Dqi fl(1 + e + 8u) + Eu) >= (Dqi (1+e) + Eq)
       // (1 + 6u) with Eu >= Eq (1 + 8u) (u = 2^-24).  A > that implies the exact test.
       // A rep's plane against its own list (ji == j) is skipped (no bound).
@@ -487,12 +501,11 @@ __global__ __launch_bounds__(256) void plan_count(const float *Dqr, uint32_t m, 
       for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) Bi[pi] = __builtin_fmaf(Dqi[pi], cb1, Eu);
 #pragma unroll
       for (int v = 0; v < V; ++v) {
-        const float A = f32_round_down(Dlo[v] - slack[v]);
         bool far = false;
 #pragma unroll
         for (uint32_t pi = 0; pi <= (uint32_t)NPR; ++pi) {
           const float dn = den[pi][v];
-          far |= (dn > 0.f) & (ji[pi] != jl + v) & (A > __builtin_fmaf(Gf, dn, Bi[pi]));
+          far |= (dn > 0.f) & (ji[pi] != jl + v) & (A[v] > __builtin_fmaf(Gf, dn, Bi[pi]));
         }
         keep[v] = keep[v] & !(far & (jl + v != j0) & (jl + v != j1));
       }
