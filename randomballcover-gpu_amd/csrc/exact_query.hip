@@ -586,20 +586,50 @@ __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
   float tv = FLT_MAX;
   uint32_t ti = kDummy;
   const uint32_t cb = a.cert[q];
-  const uint32_t np0 = (cb & 1u) ? a.Kp : 0;
-  const bool two = a.probe_i1 != nullptr && a.key1[q] < a.R && (cb & 2u);
-  const uint32_t np = np0 + (two ? a.Kp : 0);
   const uint32_t nc = a.cand_n ? min(a.cand_n[q], a.cap) : 0;
-  const uint32_t total = np + nc;
-  auto cand = [&](uint32_t e) -> uint32_t {
-    uint32_t id;
-    if (e < np0) id = a.probe_i[size_t(q) * a.Kp + e];
-    else if (e < np) id = a.probe_i1[size_t(q) * a.Kp + (e - np0)];
-    else id = a.cand_i[size_t(q) * a.cap + (e - np)];
-    return id == kDummy ? id : RBC_CHECKED(id, a.n);
-  };
-  rerank_candidates<METRIC, NPL>(tv, ti, total, cand, a.xp, a.ldx, a.inv, qv, a.d);
-  near_tie_guard<METRIC>(tv, ti, total, a.xp, a.ldx, a.inv, qrow, a.d, a.k, a.e_tree, a.stats);
+  if (a.probe_d) {
+    // Start from the probe's exactly re-ranked points (sorted, lanes < Kp), and re-rank
+    // only the scan candidates not among them, 64 at a time, compacted through LDS.
+    __shared__ uint32_t comp[4][kWave];
+    const int w = threadIdx.x / kWave;
+    uint32_t pid = kDummy;
+    if (l < (int)a.Kp) {
+      pid = a.probe_i[size_t(q) * a.Kp + l];
+      if (pid != kDummy) {
+        ti = RBC_CHECKED(pid, a.n);
+        tv = a.probe_d[size_t(q) * a.Kp + l];
+      }
+    }
+    uint32_t seen = (uint32_t)__popcll(__ballot(pid != kDummy));
+    for (uint32_t b0 = 0; b0 < nc; b0 += kWave) {
+      uint32_t id = b0 + l < nc ? a.cand_i[size_t(q) * a.cap + b0 + l] : kDummy;
+      for (uint32_t j = 0; j < a.Kp; ++j)
+        if (id == (uint32_t)__shfl((int)pid, (int)j, kWave)) id = kDummy;
+      const uint64_t bal = __ballot(id != kDummy);
+      const uint32_t nv = (uint32_t)__popcll(bal);
+      if (id != kDummy) comp[w][__popcll(bal & ((1ull << l) - 1ull))] = id;
+      __builtin_amdgcn_wave_barrier();
+      auto cand = [&](uint32_t e) -> uint32_t { return RBC_CHECKED(comp[w][e], a.n); };
+      rerank_candidates<METRIC, NPL>(tv, ti, nv, cand, a.xp, a.ldx, a.inv, qv, a.d);
+      __builtin_amdgcn_wave_barrier();
+      seen += nv;
+    }
+    near_tie_guard<METRIC>(tv, ti, seen, a.xp, a.ldx, a.inv, qrow, a.d, a.k, a.e_tree, a.stats);
+  } else {
+    const uint32_t np0 = (cb & 1u) ? a.Kp : 0;
+    const bool two = a.probe_i1 != nullptr && a.key1[q] < a.R && (cb & 2u);
+    const uint32_t np = np0 + (two ? a.Kp : 0);
+    const uint32_t total = np + nc;
+    auto cand = [&](uint32_t e) -> uint32_t {
+      uint32_t id;
+      if (e < np0) id = a.probe_i[size_t(q) * a.Kp + e];
+      else if (e < np) id = a.probe_i1[size_t(q) * a.Kp + (e - np0)];
+      else id = a.cand_i[size_t(q) * a.cap + (e - np)];
+      return id == kDummy ? id : RBC_CHECKED(id, a.n);
+    };
+    rerank_candidates<METRIC, NPL>(tv, ti, total, cand, a.xp, a.ldx, a.inv, qv, a.d);
+    near_tie_guard<METRIC>(tv, ti, total, a.xp, a.ldx, a.inv, qrow, a.d, a.k, a.e_tree, a.stats);
+  }
   if (l < (int)a.k) {
     a.out_d[size_t(q) * a.ldo + l] = tv;
     a.out_i[size_t(q) * a.ldo + l] = ti;
@@ -1075,6 +1105,9 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   sa.k = k;
   sa.Kp = KpUse;
   sa.probe_i = probe_i;
+  // The BF16 probe's results come from probe_merge's tree-ordered re-rank (the values
+  // select would compute); the FP32 probe's from the tile engine (another order).
+  sa.probe_d = bprobe ? probe_d : nullptr;
   sa.probe_i1 = probe_i1;
   sa.key1 = key1;
   sa.R = R;

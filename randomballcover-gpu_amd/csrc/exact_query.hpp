@@ -14,6 +14,10 @@ struct SelectArgs {
   uint32_t m, d, k, Kp;
   uint32_t n;                 // database rows (bounds checks)
   const uint32_t *probe_i;   // [m][Kp] first-probe candidates (original ids)
+  // With probe_d: probe_i/probe_d are the probe's K' distinct points re-ranked exactly as
+  // select re-ranks (tree-ordered FP32), ascending; select starts from them and re-ranks
+  // only the scan candidates not among them.  NULL: probe_i are candidates like the scan's.
+  const float *probe_d;
   const uint32_t *probe_i1;  // [m][Kp] second-probe candidates, or NULL
   const uint32_t *key1;      // second-probe list per query (R = none)
   uint32_t R;
