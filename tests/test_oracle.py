@@ -269,6 +269,33 @@ def test_exact_owners_and_radii():
     assert (below < exact * (1 + 1e-15)).all()
 
 
+def test_exact_owners_against_fp64_argmin():
+    # Independent of the oracle's FP32 arithmetic: NumPy's FP64 distances.  Where the two
+    # nearest reps are farther apart than the FP32 error bound, the owner is the FP64
+    # argmin; elsewhere its FP64 distance is within that bound of the minimum.
+    x = datagen.gmm(4000, 20, 0)
+    reps = np.random.default_rng(6).choice(4000, 50, replace=False).astype(np.uint32)
+    own = oracle.exact_owners(x, reps)
+    X, Rr = x.astype(np.float64), x[reps].astype(np.float64)
+    D = ((X[:, None, :] - Rr[None, :, :]) ** 2).sum(-1)
+    best, arg = D.min(1), D.argmin(1)
+    second = np.partition(D, 1, axis=1)[:, 1]
+    e = 4 * (20 + 4) * 2.0 ** -24
+    clear = second > best * (1 + e)
+    assert clear.mean() > 0.99
+    np.testing.assert_array_equal(own[clear], arg[clear])
+    assert (D[np.arange(len(x)), own] <= best * (1 + e)).all()
+    # A small-integer lattice: FP32 distances are exact, ties are common (repeated rows),
+    # and the owner is the FP64 argmin with the lowest rep index among equals.
+    xi = np.random.default_rng(3).integers(-3, 4, (3000, 4)).astype(np.float32)
+    ri = np.random.default_rng(4).choice(3000, 40, replace=False).astype(np.uint32)
+    oi = oracle.exact_owners(xi, ri)
+    Di = ((xi[:, None, :].astype(np.float64) - xi[ri][None, :, :].astype(np.float64)) ** 2).sum(-1)
+    ties = (Di == Di.min(1)[:, None]).sum(1) > 1
+    assert ties.mean() > 0.1
+    np.testing.assert_array_equal(oi, Di.argmin(1))
+
+
 def test_golden_fixture_reproduces():
     path = os.path.join(os.path.dirname(__file__), "golden", "sample_shaped_fp64_knn.npz")
     g = np.load(path)
