@@ -355,13 +355,15 @@ class ExactRBC:
                                       ds.ctypes.data, k, MEM_HOST, None, ctypes.byref(st)))
         return (ids, ds, st.as_dict()) if stats else (ids, ds)
 
-    def kquery_device(self, q, k, ids, ds, precision=PREC_FP32, stream=None):
-        """q, ids, ds are device tensors ([m,d] float32, [m,k] int32/uint32, [m,k] float32)."""
-        st = QueryStats()
+    def kquery_device(self, q, k, ids, ds, precision=PREC_FP32, stream=None, stats=True):
+        """q, ids, ds are device tensors ([m,d] float32, [m,k] int32/uint32, [m,k] float32).
+        Returns the query's statistics (per-stage hipEvent times, pair counts), or None with
+        stats=False (no events recorded, no statistics copied back)."""
+        st = QueryStats() if stats else None
         _check(lib().rbc_exact_kquery(self.h, _ptr(q), q.shape[0], q.stride(0), k, precision,
                                       _ptr(ids), _ptr(ds), ids.stride(0), MEM_DEVICE, stream,
-                                      ctypes.byref(st)))
-        return st.as_dict()
+                                      ctypes.byref(st) if stats else None))
+        return st.as_dict() if stats else None
 
     def set_proposal(self, mlp, mass=None):
         """rbc_exact_set_proposal (and, when given, rbc_exact_set_proposal_mass)."""
