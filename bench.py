@@ -5,9 +5,9 @@ with z ~ N(0, I_16) (the ``lowrank16_n4M_d128`` generator), m = 131,072 queries 
 the same model, 2,048 representatives, k = 10.  One step = one exact k-query of all m
 queries with inputs and outputs resident in HBM (build and proposal training excluded,
 timed separately).  The timed path is the BF16 MFMA filter + exact FP32 re-rank with the
-learned list proposal (MLP d -> 256 -> 256 -> R, FP16, trained on the GPU on 2^20
-(query, owner list of its exact nearest neighbour) pairs produced by the exact search
-itself).  Beside it, once each on the same index and queries: the same search without the
+learned list proposal (MLP d -> 256 -> 256 -> R, FP16, trained on the GPU with soft
+targets: the owner lists of the k exact nearest neighbours of 2^20 training queries, found
+by the exact search itself).  Beside it, once each on the same index and queries: the same search without the
 proposal, and the plain FP32 exact RBC.
 
 ``roofline`` is for the dominant kernel, the list scan: FLOPs the scan evaluates per
