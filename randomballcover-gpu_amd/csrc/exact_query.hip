@@ -609,7 +609,10 @@ __global__ __launch_bounds__(256) void select_kernel(SelectArgs a) {
       const uint32_t nv = (uint32_t)__popcll(bal);
       if (id != kDummy) comp[w][__popcll(bal & ((1ull << l) - 1ull))] = id;
       __builtin_amdgcn_wave_barrier();
-      auto cand = [&](uint32_t e) -> uint32_t { return RBC_CHECKED(comp[w][e], a.n); };
+      auto cand = [&](uint32_t e) -> uint32_t {
+        const uint32_t id = comp[w][e];
+        return RBC_CHECKED(id, a.n);
+      };
       rerank_candidates<METRIC, NPL>(tv, ti, nv, cand, a.xp, a.ldx, a.inv, qv, a.d);
       __builtin_amdgcn_wave_barrier();
       seen += nv;
