@@ -849,8 +849,8 @@ static int exact_query_run(rbc_exact *h, const float *qd, size_t ldq, uint32_t m
   // The BF16 probe keeps at most 32 candidates, so it serves k <= 32.  It scans the Pp
   // lists of the smallest lane minima of each query's rep distances in one pass.  Its K'
   // best by approximate distance are re-ranked exactly, and gamma is the k-th of them:
-  // K' = k + 8 on full BF16 rows, k + 2 on sketches (on the low-rank C3 set, k + 2, 4
-  // and 8 leave the same plan, k + 2 re-ranks 1/3 fewer rows).
+  // K' = k + 8 on full BF16 rows, k + 2 on sketches (on low-rank data, n = 4M, d = 128,
+  // 16-dimensional, k + 2, 4 and 8 leave the same plan; k + 2 re-ranks 1/3 fewer rows).
   const bool bprobe = bf16 && k <= 32;
   const uint32_t KpUse = bprobe ? std::min<uint32_t>(32, k + (sketch ? 2u : 8u)) : Kp;
   const uint32_t Pp = bprobe ? std::min<uint32_t>(h->probe_lists, R) : 0;
