@@ -158,30 +158,41 @@ constexpr uint32_t kSkProjRows = 32;
 __global__ __launch_bounds__(256) void sk_project(const float *x, size_t ldx, uint32_t rows, uint32_t d,
                                                   const float *P, uint32_t ldP, const float *mu, double gam_d1,
                                                   int database, uint16_t *out, float *err, float *n2) {
-  __shared__ float Pt[kSkMaxD][kSkW];
-  __shared__ float xs[8][kSkMaxD];
-  __shared__ float xa[8][kSkMaxD];
+  // P by rows (coordinate i of row j at PT[j][i], rows padded by 4 floats so that the 16
+  // lanes of a 16-byte read pass fall on distinct banks), read 4 coordinates at a time.
+  constexpr int kPad = kSkMaxD + 4;
+  __shared__ __attribute__((aligned(16))) float PT[kSkW][kPad];
+  __shared__ __attribute__((aligned(16))) float xs[8][kSkMaxD];
+  __shared__ __attribute__((aligned(16))) float xa[8][kSkMaxD];
   const int t = threadIdx.x, r = t / kSkW, j = t % kSkW;
-  for (uint32_t e = t; e < d * kSkW; e += 256) {
-    const uint32_t i = e / kSkW, jj = e % kSkW;
-    Pt[i][jj] = jj < (uint32_t)kSkDims ? P[size_t(jj) * ldP + i] : 0.f;
+  for (uint32_t e = t; e < kSkW * kSkMaxD; e += 256) {
+    const uint32_t jj = e / kSkMaxD, i = e % kSkMaxD;
+    PT[jj][i] = jj < (uint32_t)kSkDims && i < d ? P[size_t(jj) * ldP + i] : 0.f;
   }
   const double afac = 1.0 + 2.02 * (double)d * 0x1p-24;
   for (uint32_t round = 0; round < kSkProjRows / 8; ++round) {
     const uint32_t row = blockIdx.x * kSkProjRows + round * 8 + r;
     __syncthreads();  // P staged; the previous round's rows read
-    for (uint32_t i = j; i < d; i += kSkW) {
-      const float v = row < rows ? x[size_t(row) * ldx + i] : 0.f;
-      xs[r][i] = v - mu[i];
+    for (uint32_t i = j; i < kSkMaxD; i += kSkW) {
+      const float v = row < rows && i < d ? x[size_t(row) * ldx + i] : 0.f;
+      xs[r][i] = i < d ? v - mu[i] : 0.f;
       // |x_i - mu_i| rounded up, for the bound (the FP32 difference may round down).
-      xa[r][i] = f32_up(fabs((double)v - (double)mu[i]));
+      xa[r][i] = i < d ? f32_up(fabs((double)v - (double)mu[i])) : 0.f;
     }
     __syncthreads();
     float y = 0.f, af = 0.f;
-    for (uint32_t i = 0; i < d; ++i) {
-      const float p = Pt[i][j];
-      y = __builtin_fmaf(p, xs[r][i], y);
-      af = __builtin_fmaf(fabsf(p), xa[r][i], af);
+    for (uint32_t i = 0; i < d; i += 4) {
+      const float4 p4 = *reinterpret_cast<const float4 *>(&PT[j][i]);
+      const float4 x4 = *reinterpret_cast<const float4 *>(&xs[r][i]);
+      const float4 a4 = *reinterpret_cast<const float4 *>(&xa[r][i]);
+      y = __builtin_fmaf(p4.x, x4.x, y);
+      y = __builtin_fmaf(p4.y, x4.y, y);
+      y = __builtin_fmaf(p4.z, x4.z, y);
+      y = __builtin_fmaf(p4.w, x4.w, y);
+      af = __builtin_fmaf(fabsf(p4.x), a4.x, af);
+      af = __builtin_fmaf(fabsf(p4.y), a4.y, af);
+      af = __builtin_fmaf(fabsf(p4.z), a4.z, af);
+      af = __builtin_fmaf(fabsf(p4.w), a4.w, af);
     }
     double q2 = 0.0, b2 = 0.0, y2 = 0.0;
     uint16_t h = 0;
